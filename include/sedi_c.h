@@ -1,0 +1,172 @@
+/*
+ * sedi_c.h -- C ABI of libsedi_amd, the MI355X (gfx950) implementation of
+ * intel/sedi's stochastic diag(A^-1) estimator path.
+ *
+ * Every entry point replaces one reference interface, cited as file:line of
+ * intel/sedi.  Arrays are dense n1*n2*n3 doubles, i1 fastest (Slice3d.hpp:30-32),
+ * without ghost cells; cells outside the grid count as 0, which is what the
+ * reference's zero ghosts give (Cg.hpp:57-62).  An array argument may be a
+ * host pointer or a HIP device pointer (detected with hipPointerGetAttributes);
+ * host arrays are staged through library-owned device memory.
+ *
+ * Return values: >= 0 is success (for a solve, the reference's result code
+ * 0..3); < 0 is a runtime error (SEDI_ERR_*) and sedi_last_error() describes
+ * it.  No C++ exception crosses this boundary.
+ *
+ * Threading: a sedi_solver / sedi_estimator handle belongs to one host thread
+ * at a time; different handles may be used from different threads.
+ */
+#ifndef SEDI_C_H
+#define SEDI_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- codes ---------------------------------------------------------------- */
+#define SEDI_OK               0
+#define SEDI_ERR_ARG        (-1)   /* bad shape, pointer or option                */
+#define SEDI_ERR_HIP        (-2)   /* HIP runtime error                            */
+#define SEDI_ERR_NOMEM      (-3)   /* device allocation failed / over the budget   */
+#define SEDI_ERR_STATE      (-4)   /* handle used out of order                     */
+#define SEDI_ERR_UNSUPPORTED (-5)  /* option not available in this build           */
+
+/* Solver result codes: Cg.hpp:31-34, PipelinedCg.hpp:31-34, BiCgStab.hpp:32-37. */
+#define SEDI_SOLUTION_FOUND   0
+#define SEDI_REACHED_MAXITER  1
+#define SEDI_BREAKDOWN_RHO    2   /* BiCGStab only */
+#define SEDI_BREAKDOWN_OMEGA  3   /* BiCGStab only */
+
+/* Solver kinds: the SOLVER numbering of stochastic-estimator.cc:134-142. */
+#define SEDI_BICGSTAB 1
+#define SEDI_CG       2
+#define SEDI_PCG      3
+
+/* Arithmetic of a solve.  FP64: the reference's (real_t = double,
+ * stochastic-estimator.cc:29).  FP32_MIXED: FP32 vectors and stencil, FP64
+ * dots, scalars, residual replacement and diagonal accumulator. */
+#define SEDI_FP64        0
+#define SEDI_FP32_MIXED  1
+
+/* Global grid: DomainDecomposition's ng1, ng2, ng3 (DomainDecomposition.hpp:8-9). */
+typedef struct sedi_grid { int64_t n1, n2, n3; } sedi_grid;
+/* Stencil<double>(a, b, c) (Stencil.hpp:18-19); the estimator uses (6, 1, 1)
+ * (stochastic-estimator.cc:74). */
+typedef struct sedi_stencil { double a, b, c; } sedi_stencil;
+
+typedef struct sedi_solver sedi_solver;
+typedef struct sedi_estimator sedi_estimator;
+
+const char *sedi_last_error(void);
+int sedi_version(void);
+/* Selects the HIP device for this host thread (hipSetDevice) and loads the
+ * kernels.  Optional: every entry point initialises lazily. */
+int sedi_init(int device);
+
+/* ---- operator ------------------------------------------------------------- */
+/* y = A x on the whole grid.  Replaces Stencil<double>::apply
+ * (Stencil.hpp:21-36) together with DistrAsyncStencil's
+ * initiate/compute_local/finalize (DistrAsyncStencil.hpp:52-86) and the halo
+ * exchange of CommHalo (CommHalo.hpp:84-181): on one GPU the grid is one
+ * domain and its boundary is predicated in the kernel. */
+int sedi_stencil_apply(const sedi_grid *grid, const sedi_stencil *stencil,
+                       const double *x, double *y);
+
+/* ---- solvers -------------------------------------------------------------- */
+/* Replaces the solver constructors Cg(dd, nghost, mode, timers) (Cg.hpp:48),
+ * PipelinedCg(...) (PipelinedCg.hpp:48) and BiCgStab(...) (BiCgStab.hpp:51):
+ * allocates the solver's device temporaries once for reuse by every solve.
+ * mode: the -f bits (bit 0 synchronous stencil, bit 1 synchronous dots,
+ * Cg.hpp:36-40); accepted for compatibility, they change nothing on one GPU. */
+int sedi_solver_create(const sedi_grid *grid, const sedi_stencil *stencil,
+                       int kind, int mode, sedi_solver **out);
+
+/* Replaces solver_t::solve(A, comm_halo, x, b, iM, maxiter, tol, &final_err,
+ * &final_iter) (Cg.hpp:70-82, PipelinedCg.hpp:72-84, BiCgStab.hpp:77-89).
+ * x: initial guess in, solution out.  inv_diag: the Jacobi vector iM, or NULL
+ * for the constant 1/a that the estimator uses (stochastic-estimator.cc:103).
+ * Returns the result code 0..3 and always sets *final_err and *final_iter. */
+int sedi_solve(sedi_solver *solver, const double *b, double *x,
+               const double *inv_diag, int maxiter, double tol,
+               double *final_err, int *final_iter);
+
+/* The err values that the reference prints after each iteration
+ * ("Cg: after iter=%d, err=%.10e", Cg.hpp:192; PipelinedCg.hpp:228;
+ * BiCgStab.hpp:228) for the last solve.  Copies min(cap, count) values and
+ * returns count. */
+int sedi_solver_err_history(sedi_solver *solver, double *out, int cap);
+
+void sedi_solver_destroy(sedi_solver *solver);
+
+/* ---- probes --------------------------------------------------------------- */
+/* Replaces DistrRngStd::fill_plusminusone (DistrRngStd.hpp:22-42):
+ * v[c] = +1 if the c-th std::uniform_real_distribution<double>(0,1) draw from
+ * std::mt19937(seed), after skipping first_draw 32-bit outputs, is > 0.5,
+ * else -1.  Each cell takes two outputs.  The reference re-seeds from
+ * std::random_device on each call; this ABI takes the seed explicitly. */
+int sedi_rademacher_fill(uint32_t seed, uint64_t first_draw, int64_t ncells,
+                         double *v);
+
+/* ---- estimator ------------------------------------------------------------ */
+/* The sample loop of stochastic-estimator.cc:57-232 (CLI -g -s -e -m -i -f)
+ * with seeded probes: sample k solves A x = v_k from x = 0 with v_k cells
+ * drawn from outputs [2Nk, 2N(k+1)) of std::mt19937(seed), accumulates
+ * t += x.*v, q += v.*v (:195,198) and returns D = t./q (:216). */
+typedef struct sedi_estimate_args {
+    sedi_grid grid;
+    sedi_stencil stencil;   /* (6,1,1) in the reference */
+    int kind;               /* SEDI_BICGSTAB / SEDI_CG / SEDI_PCG        (-S) */
+    int mode;               /* -f bits, accepted and ignored              (-f) */
+    int samples;            /* number of probes                           (-s) */
+    int maxiter;            /* solver iteration cap                        (-m) */
+    double tol;             /* solver tolerance                            (-e) */
+    int ignore_errors;      /* keep going after a failed solve             (-i) */
+    uint32_t seed;          /* mt19937 seed of the probe stream                 */
+    int precision;          /* SEDI_FP64 or SEDI_FP32_MIXED                     */
+    int batch_k;            /* probes solved together (1 = plain, one at a time) */
+} sedi_estimate_args;
+
+/* Allocates everything the run needs (excluded from the timed region, as the
+ * reference's TIMER_GLOBAL excludes setup: stochastic-estimator.cc:151-153). */
+int sedi_estimator_create(const sedi_estimate_args *args, sedi_estimator **out);
+
+/* Runs the sample loop.  D (n1*n2*n3, may be NULL), iters/errs/codes (per
+ * sample, may be NULL).  *seconds (may be NULL) is the wall time of the loop,
+ * the reference's TIMER_GLOBAL scope (stochastic-estimator.cc:152-153,211-212).
+ * Returns 0, or 100 + code of the first failed solve when ignore_errors is 0
+ * (the MPI_Abort code of stochastic-estimator.cc:181-184); iters/errs/codes
+ * then hold the samples done so far. */
+int sedi_estimator_run(sedi_estimator *est, double *D, int *iters, double *errs,
+                       int *codes, double *seconds);
+
+/* Optional line sink for the reference's stdout lines, called from
+ * sedi_estimator_run: "Solving for sample %d/%d..." (stochastic-estimator.cc:160),
+ * "<Solver>: after iter=%d, err=%.10e" after each solve (Cg.hpp:192,
+ * PipelinedCg.hpp:228, BiCgStab.hpp:228) and "Incomplete convergence in
+ * solver, errcode=%d" (stochastic-estimator.cc:182).  Lines have no '\n'. */
+typedef void (*sedi_log_fn)(void *user, const char *line);
+int sedi_estimator_set_log(sedi_estimator *est, sedi_log_fn fn, void *user);
+
+void sedi_estimator_destroy(sedi_estimator *est);
+
+/* create + run + destroy. */
+int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
+                  double *errs, int *codes, double *seconds);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Kernel timing with HIP events recorded on the launching stream around every
+ * launch of the named kernel ("*" = all kernels, NULL = off).  The reference's
+ * Timers (Timers.hpp:14-55) time phases with MPI_Wtime; this is the device
+ * counterpart. */
+int sedi_profile_enable(const char *kernel_name);
+/* Fetches (launches, total milliseconds) of a kernel timed since the last
+ * sedi_profile_reset().  Returns 0, or SEDI_ERR_ARG if it was never timed. */
+int sedi_profile_get(const char *kernel_name, int64_t *launches, double *total_ms);
+int sedi_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEDI_C_H */

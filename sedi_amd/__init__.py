@@ -1,0 +1,281 @@
+"""sedi_amd -- MI355X (gfx950) stochastic diag(A^-1) estimator, a drop-in for
+intel/sedi's solver path.
+
+Python mirror of the reference's operator/solver interface over the C ABI of
+libsedi_amd.so (include/sedi_c.h):
+
+    Stencil(a, b, c).apply(x)           Stencil<double>::apply  (Stencil.hpp:21)
+    Cg / PipelinedCg / BiCgStab(grid)   solver constructors     (Cg.hpp:48, ...)
+      .solve(x, b, iM, maxiter, tol)    solver_t::solve         (Cg.hpp:70-82)
+        -> (result_code, final_err, final_iter)
+    rademacher_fill(seed, first, n)     DistrRngStd::fill_plusminusone
+    estimate(...)                       stochastic-estimator.cc:158-216
+
+Arrays are numpy float64, n1*n2*n3 cells with i1 fastest (reshape a
+(n3, n2, n1) C-ordered array to 1-D).  Everything runs in the HIP library; there
+is no CPU fallback: if libsedi_amd.so is missing or no GPU is present, the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEGA",
+           "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
+           "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
+           "Estimator", "estimate", "profile_enable", "profile_get", "profile_reset"]
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(PKG_DIR, "libsedi_amd.so")
+
+SOLUTION_FOUND, REACHED_MAXITER, BREAKDOWN_RHO, BREAKDOWN_OMEGA = 0, 1, 2, 3
+BICGSTAB, CG, PCG = 1, 2, 3
+FP64, FP32_MIXED = 0, 1
+SOLVER_NAMES = {BICGSTAB: "BiCgStab", CG: "Cg", PCG: "PipelinedCg"}
+
+
+class SediError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sedi error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("n1", ctypes.c_int64), ("n2", ctypes.c_int64), ("n3", ctypes.c_int64)]
+
+
+class _Stencil(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double)]
+
+
+class _EstArgs(ctypes.Structure):
+    _fields_ = [("grid", _Grid), ("stencil", _Stencil), ("kind", ctypes.c_int),
+                ("mode", ctypes.c_int), ("samples", ctypes.c_int), ("maxiter", ctypes.c_int),
+                ("tol", ctypes.c_double), ("ignore_errors", ctypes.c_int),
+                ("seed", ctypes.c_uint32), ("precision", ctypes.c_int), ("batch_k", ctypes.c_int)]
+
+
+LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
+
+_lib = None
+
+
+def lib():
+    """Loads libsedi_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise SediError(-5, "%s not built; run `python -m sedi_amd.build`" % lib_path)
+        L = ctypes.CDLL(lib_path)
+        vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        L.sedi_last_error.restype = ctypes.c_char_p
+        L.sedi_init.argtypes = [ctypes.c_int]
+        L.sedi_stencil_apply.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Stencil), vp, vp]
+        L.sedi_solver_create.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Stencil),
+                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.sedi_solve.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_double, dp, ip]
+        L.sedi_solver_err_history.argtypes = [vp, dp, ctypes.c_int]
+        L.sedi_solver_destroy.argtypes = [vp]
+        L.sedi_solver_destroy.restype = None
+        L.sedi_rademacher_fill.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, vp]
+        L.sedi_estimator_create.argtypes = [ctypes.POINTER(_EstArgs), ctypes.POINTER(vp)]
+        L.sedi_estimator_run.argtypes = [vp, vp, ip, dp, ip, dp]
+        L.sedi_estimator_set_log.argtypes = [vp, LOG_FN, vp]
+        L.sedi_estimator_destroy.argtypes = [vp]
+        L.sedi_estimator_destroy.restype = None
+        L.sedi_estimate.argtypes = [ctypes.POINTER(_EstArgs), vp, ip, dp, ip, dp]
+        L.sedi_profile_enable.argtypes = [ctypes.c_char_p]
+        L.sedi_profile_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise SediError(rc, lib().sedi_last_error().decode())
+    return rc
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d cells, got %d" % (n, a.size))
+    return a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _grid(grid):
+    n1, n2, n3 = (int(v) for v in grid)
+    return _Grid(n1, n2, n3), n1 * n2 * n3
+
+
+class Stencil:
+    """Stencil<double>(a, b, c) (Stencil.hpp:18-19)."""
+
+    def __init__(self, a=6.0, b=1.0, c=1.0):
+        self.a, self.b, self.c = float(a), float(b), float(c)
+
+    def _c(self):
+        return _Stencil(self.a, self.b, self.c)
+
+    def apply(self, grid, x, y=None):
+        """y = A x on the whole grid with zero boundary (Stencil.hpp:21-54)."""
+        g, n = _grid(grid)
+        x = _f64(x, n)
+        if y is None:
+            y = np.empty(n)
+        _check(lib().sedi_stencil_apply(ctypes.byref(g), ctypes.byref(self._c()), _ptr(x), _ptr(y)))
+        return y
+
+
+class Solver:
+    """Base of Cg / PipelinedCg / BiCgStab: allocates device temporaries once."""
+    kind = None
+
+    def __init__(self, grid, stencil=None, mode=0):
+        self.grid = tuple(int(v) for v in grid)
+        self.stencil = stencil or Stencil()
+        g, self.n = _grid(self.grid)
+        h = ctypes.c_void_p()
+        _check(lib().sedi_solver_create(ctypes.byref(g), ctypes.byref(self.stencil._c()),
+                                        self.kind, int(mode), ctypes.byref(h)))
+        self._h = h
+
+    def solve(self, x, b, iM=None, maxiter=1000, tol=1e-4):
+        """Solves A x = b from the guess in x (updated in place).
+
+        Returns (result_code, final_err, final_iter) like solver_t::solve.
+        """
+        if not isinstance(x, np.ndarray) or x.dtype != np.float64 or not x.flags.c_contiguous:
+            raise TypeError("x must be a C-contiguous float64 numpy array (updated in place)")
+        if x.size != self.n:
+            raise ValueError("x has %d cells, grid has %d" % (x.size, self.n))
+        b = _f64(b, self.n)
+        m = None if iM is None else _f64(iM, self.n)
+        fe = ctypes.c_double()
+        fi = ctypes.c_int()
+        rc = lib().sedi_solve(self._h, _ptr(b), _ptr(x), None if m is None else _ptr(m),
+                              int(maxiter), float(tol), ctypes.byref(fe), ctypes.byref(fi))
+        _check(rc)
+        return rc, fe.value, fi.value
+
+    def err_history(self):
+        n = _check(lib().sedi_solver_err_history(self._h, None, 0))
+        out = np.zeros(n)
+        lib().sedi_solver_err_history(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedi_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Cg(Solver):
+    kind = CG
+
+
+class PipelinedCg(Solver):
+    kind = PCG
+
+
+class BiCgStab(Solver):
+    kind = BICGSTAB
+
+
+SOLVERS = {CG: Cg, PCG: PipelinedCg, BICGSTAB: BiCgStab}
+
+
+def rademacher_fill(seed, first_draw, ncells):
+    """+-1 probe cells from std::mt19937(seed) after first_draw outputs."""
+    v = np.empty(int(ncells))
+    _check(lib().sedi_rademacher_fill(int(seed), int(first_draw), int(ncells), _ptr(v)))
+    return v
+
+
+class Estimator:
+    """The sample loop of stochastic-estimator.cc:158-216 (seeded probes)."""
+
+    def __init__(self, grid, kind=CG, samples=8, maxiter=1000, tol=1e-4, seed=1234,
+                 ignore_errors=False, stencil=None, mode=0, precision=FP64, batch_k=1):
+        st = stencil or Stencil()
+        g, self.n = _grid(grid)
+        self.samples = int(samples)
+        self.args = _EstArgs(g, st._c(), int(kind), int(mode), self.samples, int(maxiter),
+                             float(tol), int(bool(ignore_errors)), int(seed) & 0xffffffff,
+                             int(precision), int(batch_k))
+        h = ctypes.c_void_p()
+        _check(lib().sedi_estimator_create(ctypes.byref(self.args), ctypes.byref(h)))
+        self._h = h
+        self._log = None
+
+    def set_log(self, fn):
+        """fn(line: str) receives the reference's stdout lines."""
+        if fn is None:
+            self._log = None
+            lib().sedi_estimator_set_log(self._h, LOG_FN(), None)
+            return
+        self._log = LOG_FN(lambda user, line: fn(line.decode()))
+        _check(lib().sedi_estimator_set_log(self._h, self._log, None))
+
+    def run(self, want_D=True):
+        """Returns dict(status, D, iters, errs, codes, seconds)."""
+        D = np.zeros(self.n) if want_D else None
+        iters = np.zeros(self.samples, dtype=np.int32)
+        codes = np.full(self.samples, -1, dtype=np.int32)
+        errs = np.zeros(self.samples)
+        secs = ctypes.c_double()
+        ip = ctypes.POINTER(ctypes.c_int)
+        rc = lib().sedi_estimator_run(self._h, None if D is None else _ptr(D),
+                                      iters.ctypes.data_as(ip),
+                                      errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      codes.ctypes.data_as(ip), ctypes.byref(secs))
+        _check(rc)
+        return dict(status=rc, D=D, iters=iters, errs=errs, codes=codes, seconds=secs.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedi_estimator_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate(grid, kind=CG, samples=8, maxiter=1000, tol=1e-4, seed=1234, ignore_errors=False,
+             **kw):
+    est = Estimator(grid, kind, samples, maxiter, tol, seed, ignore_errors, **kw)
+    try:
+        return est.run()
+    finally:
+        est.close()
+
+
+def profile_enable(kernel_name):
+    """Times every launch of `kernel_name` ("*" = all, None = off) with HIP events."""
+    _check(lib().sedi_profile_enable(None if kernel_name is None else kernel_name.encode()))
+
+
+def profile_get(kernel_name):
+    """(launches, total_ms) of a timed kernel."""
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    _check(lib().sedi_profile_get(kernel_name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, ms.value
+
+
+def profile_reset():
+    _check(lib().sedi_profile_reset())

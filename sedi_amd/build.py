@@ -1,0 +1,83 @@
+"""Builds libsedi_amd.so (HIP for gfx950) and the sedi-estimator driver, in-tree.
+
+    python -m sedi_amd.build          # or: python sedi_amd/build.py
+
+Objects go to sedi_amd/_build/, the library to sedi_amd/libsedi_amd.so and the
+CLI driver to sedi_amd/bin/sedi-estimator.  Every compile line carries
+-ffp-contract=off: the kernels round each product and difference separately,
+as the reference's expressions do (Stencil.hpp:43-50, SliceOps.hpp:38-40).
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libsedi_amd.so")
+BIN = os.path.join(PKG, "bin", "sedi-estimator")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = os.environ.get("SEDI_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
+          "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+# Packed-FP32 instructions are switched off for every device compile.
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+LIB_SOURCES = ["runtime.cpp", "api.cpp", "plain.hip", "rng.hip", "estimator.hip"]
+DRIVER_SOURCE = os.path.join(CSRC, "driver", "sedi_estimator.cpp")
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, src + ".o")
+    # every source is HIP (the .cpp files hold host code that includes the
+    # device headers), so all of them get the gfx950 device flags
+    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + ["-c", path, "-o", obj]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s%s" % (" ".join(cmd), res.stdout, res.stderr))
+    # hipcc's host pass warns about the device-only target feature; keep the rest
+    warn = "\n".join(l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l
+                     and "target-feature" not in l).strip()
+    return obj, warn
+
+
+def build(verbose=True):
+    if not os.path.exists(HIPCC):
+        raise RuntimeError("hipcc not found at %s" % HIPCC)
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    jobs = min(len(LIB_SOURCES), max(1, (os.cpu_count() or 4)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(_compile, LIB_SOURCES))
+    objs = []
+    for obj, warn in results:
+        objs.append(obj)
+        if warn and verbose:
+            print(warn, file=sys.stderr)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    # CLI driver: plain C++ host code over the C ABI
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DRIVER_SOURCE,
+           "-o", BIN + ".tmp", "-L" + PKG, "-lsedi_amd", "-Wl,-rpath,$ORIGIN/.."]
+    subprocess.run(cmd, check=True)
+    os.replace(BIN + ".tmp", BIN)
+    if verbose:
+        print("built", LIB, "and", BIN)
+    return LIB
+
+
+def clean():
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build()

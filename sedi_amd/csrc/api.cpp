@@ -1,0 +1,193 @@
+// api.cpp -- the extern "C" entry points of include/sedi_c.h.
+#include <hip/hip_runtime.h>
+
+#include <exception>
+#include <new>
+
+#include "estimator.h"
+#include "profiler.h"
+#include "plain.h"
+#include "rng.h"
+#include "runtime.h"
+#include "sedi_c.h"
+
+struct sedi_solver {
+    sedi::PlainSolver impl;
+};
+
+struct sedi_estimator {
+    sedi::Estimator impl;
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        sedi::set_error("host allocation failed");
+        return SEDI_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        sedi::set_error("internal error: %s", e.what());
+        return SEDI_ERR_STATE;
+    } catch (...) {
+        sedi::set_error("internal error");
+        return SEDI_ERR_STATE;
+    }
+}
+
+int check_grid(const sedi_grid* g, sedi::Shape* s) {
+    SEDI_REQUIRE(g, SEDI_ERR_ARG, "grid is NULL");
+    SEDI_REQUIRE(g->n1 > 0 && g->n2 > 0 && g->n3 > 0, SEDI_ERR_ARG,
+                 "grid dimensions must be positive (got %lldx%lldx%lld)", (long long)g->n1,
+                 (long long)g->n2, (long long)g->n3);
+    SEDI_REQUIRE(g->n1 < (1LL << 30) && g->n2 < (1LL << 30) && g->n3 < (1LL << 30), SEDI_ERR_ARG,
+                 "grid dimension too large");
+    *s = sedi::Shape{(int)g->n1, (int)g->n2, (int)g->n3};
+    return SEDI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sedi_stencil_apply(const sedi_grid* grid, const sedi_stencil* st, const double* x, double* y) {
+    return guarded([&]() -> int {
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        SEDI_REQUIRE(st && x && y, SEDI_ERR_ARG, "stencil, x and y must not be NULL");
+        if ((rc = sedi::ensure_device())) return rc;
+        const size_t bytes = sizeof(double) * (size_t)s.n();
+        sedi::DevBuf xs, ys;
+        const double* xd = x;
+        double* yd = y;
+        const bool yhost = !sedi::is_device_pointer(y);
+        if (!sedi::is_device_pointer(x)) {
+            SEDI_HIP(xs.reserve(bytes));
+            SEDI_HIP(hipMemcpy(xs.p, x, bytes, hipMemcpyHostToDevice));
+            xd = xs.as<double>();
+        }
+        if (yhost) {
+            SEDI_HIP(ys.reserve(bytes));
+            yd = ys.as<double>();
+        }
+        if ((rc = sedi::stencil_apply_dev(s, st->a, st->b, st->c, xd, yd, nullptr))) return rc;
+        SEDI_HIP(hipDeviceSynchronize());
+        sedi::profile_flush();
+        if (yhost) SEDI_HIP(hipMemcpy(y, yd, bytes, hipMemcpyDeviceToHost));
+        return SEDI_OK;
+    });
+}
+
+int sedi_solver_create(const sedi_grid* grid, const sedi_stencil* st, int kind, int mode,
+                       sedi_solver** out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(out && st, SEDI_ERR_ARG, "stencil and out must not be NULL");
+        *out = nullptr;
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        if ((rc = sedi::ensure_device())) return rc;
+        sedi_solver* h = new sedi_solver();
+        rc = h->impl.create(s, st->a, st->b, st->c, kind, mode);
+        if (rc) {
+            delete h;
+            return rc;
+        }
+        *out = h;
+        return SEDI_OK;
+    });
+}
+
+int sedi_solve(sedi_solver* h, const double* b, double* x, const double* inv_diag, int maxiter,
+               double tol, double* final_err, int* final_iter) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(h, SEDI_ERR_ARG, "solver is NULL");
+        return h->impl.solve_any(b, x, inv_diag, maxiter, tol, final_err, final_iter);
+    });
+}
+
+int sedi_solver_err_history(sedi_solver* h, double* out, int cap) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(h, SEDI_ERR_ARG, "solver is NULL");
+        return h->impl.err_history(out, cap);
+    });
+}
+
+void sedi_solver_destroy(sedi_solver* h) { delete h; }
+
+int sedi_rademacher_fill(uint32_t seed, uint64_t first_draw, int64_t ncells, double* v) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(ncells >= 0, SEDI_ERR_ARG, "ncells must be >= 0");
+        if (ncells == 0) return SEDI_OK;
+        SEDI_REQUIRE(v, SEDI_ERR_ARG, "v is NULL");
+        int rc = sedi::ensure_device();
+        if (rc) return rc;
+        sedi::MtStream mt;
+        hipStream_t s = nullptr;
+        const size_t bytes = sizeof(double) * (size_t)ncells;
+        sedi::DevBuf vs;
+        double* vd = v;
+        const bool host = !sedi::is_device_pointer(v);
+        if (host) {
+            SEDI_HIP(vs.reserve(bytes));
+            vd = vs.as<double>();
+        }
+        if ((rc = mt.seed(seed, s))) return rc;
+        if ((rc = mt.skip(first_draw, s))) return rc;
+        if ((rc = mt.fill(vd, ncells, s))) return rc;
+        SEDI_HIP(hipDeviceSynchronize());
+        sedi::profile_flush();
+        if (host) SEDI_HIP(hipMemcpy(v, vd, bytes, hipMemcpyDeviceToHost));
+        return SEDI_OK;
+    });
+}
+
+int sedi_estimator_create(const sedi_estimate_args* a, sedi_estimator** out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(a && out, SEDI_ERR_ARG, "args and out must not be NULL");
+        *out = nullptr;
+        int rc = sedi::ensure_device();
+        if (rc) return rc;
+        sedi_estimator* h = new sedi_estimator();
+        rc = h->impl.create(*a);
+        if (rc) {
+            delete h;
+            return rc;
+        }
+        *out = h;
+        return SEDI_OK;
+    });
+}
+
+int sedi_estimator_set_log(sedi_estimator* est, sedi_log_fn fn, void* user) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(est, SEDI_ERR_ARG, "estimator is NULL");
+        est->impl.set_log(fn, user);
+        return SEDI_OK;
+    });
+}
+
+int sedi_estimator_run(sedi_estimator* est, double* D, int* iters, double* errs, int* codes,
+                       double* seconds) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(est, SEDI_ERR_ARG, "estimator is NULL");
+        return est->impl.run(D, iters, errs, codes, seconds);
+    });
+}
+
+void sedi_estimator_destroy(sedi_estimator* est) { delete est; }
+
+int sedi_estimate(const sedi_estimate_args* a, double* D, int* iters, double* errs, int* codes,
+                  double* seconds) {
+    sedi_estimator* est = nullptr;
+    int rc = sedi_estimator_create(a, &est);
+    if (rc) return rc;
+    rc = sedi_estimator_run(est, D, iters, errs, codes, seconds);
+    sedi_estimator_destroy(est);
+    return rc;
+}
+
+}  // extern "C"

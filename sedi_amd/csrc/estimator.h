@@ -1,0 +1,36 @@
+// estimator.h -- the sample loop (sedi_estimator).
+#pragma once
+
+#include <vector>
+
+#include "plain.h"
+#include "rng.h"
+
+namespace sedi {
+
+int fill_dev(double* x, long long n, double v, hipStream_t s);
+
+class Estimator {
+  public:
+    ~Estimator();
+    int create(const sedi_estimate_args& a);
+    int run(double* D, int* iters, double* errs, int* codes, double* seconds);
+    void set_log(sedi_log_fn fn, void* user) { log_fn_ = fn; log_user_ = user; }
+
+  private:
+    void log(const char* line);
+    sedi_estimate_args args_{};
+    Shape shape_{};
+    PlainSolver solver_;
+    MtStream rng_;
+    hipStream_t rng_stream_ = nullptr;
+    hipEvent_t ev_v_[2] = {nullptr, nullptr};     // probe k ready
+    hipEvent_t ev_free_[2] = {nullptr, nullptr};  // probe buffer no longer read
+    DevBuf buf_;
+    double* v_[2] = {nullptr, nullptr};
+    double *x_ = nullptr, *t_ = nullptr, *q_ = nullptr;
+    sedi_log_fn log_fn_ = nullptr;
+    void* log_user_ = nullptr;
+};
+
+}  // namespace sedi

@@ -1,0 +1,170 @@
+// estimator.hip -- the stochastic diag(A^-1) sample loop on one GPU.
+//
+// stochastic-estimator.cc:158-216: for each sample, v = probe, x = 0,
+// solve A x = v, t += x.*v, q += v.*v; finally D = t./q.  The probe stream
+// (rng.hip) runs on its own HIP stream one sample ahead of the solver, into a
+// double buffer guarded by events, so probe generation overlaps the solves.
+#include "estimator.h"
+#include "profiler.h"
+
+#include <chrono>
+#include <cstdio>
+
+namespace sedi {
+
+__global__ void fill_kernel(double* x, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// t += x.*v ; q += v.*v  (SliceOps::accumulate_product, SliceOps.hpp:96-108,
+// called at stochastic-estimator.cc:195,198)
+__global__ void accumulate_kernel(const double* __restrict__ x, const double* __restrict__ v,
+                                  double* __restrict__ t, double* __restrict__ q, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double vv = v[i];
+        t[i] = t[i] + x[i] * vv;
+        q[i] = q[i] + vv * vv;
+    }
+}
+
+// D = t./q (SliceOps::divide, SliceOps.hpp:68-79; stochastic-estimator.cc:216)
+__global__ void divide_kernel(const double* __restrict__ t, const double* __restrict__ q,
+                              double* __restrict__ D, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        D[i] = t[i] / q[i];
+}
+
+static dim3 ew_grid(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+int fill_dev(double* x, long long n, double v, hipStream_t s) {
+    SEDI_HIP(launch("fill_kernel", fill_kernel, ew_grid(n), dim3(256), s, x, n, v));
+    return SEDI_OK;
+}
+
+static const char* solver_name(int kind) {
+    return kind == SEDI_CG ? "Cg" : (kind == SEDI_PCG ? "PipelinedCg" : "BiCgStab");
+}
+
+Estimator::~Estimator() {
+    if (rng_stream_) (void)hipStreamSynchronize(rng_stream_);
+    for (auto& e : ev_v_) if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_free_) if (e) (void)hipEventDestroy(e);
+    if (rng_stream_) (void)hipStreamDestroy(rng_stream_);
+}
+
+int Estimator::create(const sedi_estimate_args& a) {
+    SEDI_REQUIRE(a.grid.n1 > 0 && a.grid.n2 > 0 && a.grid.n3 > 0, SEDI_ERR_ARG, "bad grid");
+    SEDI_REQUIRE(a.grid.n1 < (1 << 30) && a.grid.n2 < (1 << 30) && a.grid.n3 < (1 << 30), SEDI_ERR_ARG,
+                 "grid dimension too large");
+    SEDI_REQUIRE(a.samples >= 0 && a.maxiter >= 0, SEDI_ERR_ARG, "samples and maxiter must be >= 0");
+    SEDI_REQUIRE(a.precision == SEDI_FP64, SEDI_ERR_UNSUPPORTED,
+                 "precision %d is not available in the plain estimator", a.precision);
+    SEDI_REQUIRE(a.batch_k <= 1, SEDI_ERR_UNSUPPORTED, "batched estimation is not available here");
+    args_ = a;
+    shape_ = Shape{(int)a.grid.n1, (int)a.grid.n2, (int)a.grid.n3};
+    int rc = solver_.create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, a.kind, a.mode);
+    if (rc) return rc;
+    const size_t n = (size_t)shape_.n();
+    SEDI_HIP(buf_.reserve(sizeof(double) * n * 5));
+    v_[0] = buf_.as<double>();
+    v_[1] = v_[0] + n;
+    x_ = v_[1] + n;
+    t_ = x_ + n;
+    q_ = t_ + n;
+    SEDI_HIP(hipStreamCreateWithFlags(&rng_stream_, hipStreamNonBlocking));
+    for (int k = 0; k < 2; k++) {
+        SEDI_HIP(hipEventCreateWithFlags(&ev_v_[k], hipEventDisableTiming));
+        SEDI_HIP(hipEventCreateWithFlags(&ev_free_[k], hipEventDisableTiming));
+    }
+    return rng_.init();
+}
+
+void Estimator::log(const char* line) {
+    if (log_fn_) log_fn_(log_user_, line);
+}
+
+int Estimator::run(double* D, int* iters, double* errs, int* codes, double* seconds) {
+    const long long n = shape_.n();
+    hipStream_t s = solver_.stream();
+    // set-up outside the timed scope: t = q = 0, stream seeded, first probe
+    int rc = fill_dev(t_, n, 0.0, s);
+    if (!rc) rc = fill_dev(q_, n, 0.0, s);
+    if (rc) return rc;
+    SEDI_HIP(hipEventRecord(ev_free_[0], s));
+    SEDI_HIP(hipEventRecord(ev_free_[1], s));
+    if ((rc = rng_.seed(args_.seed, rng_stream_))) return rc;
+    if (args_.samples > 0) {
+        if ((rc = rng_.fill(v_[0], n, rng_stream_))) return rc;
+        SEDI_HIP(hipEventRecord(ev_v_[0], rng_stream_));
+    }
+    SEDI_HIP(hipStreamSynchronize(s));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    int status = 0;
+    char line[160];
+    std::vector<double> hist;
+    for (int k = 0; k < args_.samples; k++) {
+        const int cur = k & 1, nxt = cur ^ 1;
+        snprintf(line, sizeof(line), "Solving for sample %d/%d...", k + 1, args_.samples);
+        log(line);
+        if (k + 1 < args_.samples) {
+            SEDI_HIP(hipStreamWaitEvent(rng_stream_, ev_free_[nxt], 0));
+            if ((rc = rng_.fill(v_[nxt], n, rng_stream_))) return rc;
+            SEDI_HIP(hipEventRecord(ev_v_[nxt], rng_stream_));
+        }
+        SEDI_HIP(hipStreamWaitEvent(s, ev_v_[cur], 0));
+        if ((rc = fill_dev(x_, n, 0.0, s))) return rc;  // stochastic-estimator.cc:175
+        double fe = 0.0;
+        int fi = 0;
+        const int code = solver_.solve_dev(v_[cur], x_, nullptr, args_.maxiter, args_.tol, &fe, &fi);
+        if (code < 0) return code;
+        if (log_fn_) {
+            const int nh = solver_.err_history(nullptr, 0);
+            hist.resize(nh);
+            solver_.err_history(hist.data(), nh);
+            for (int i = 0; i < nh; i++) {
+                snprintf(line, sizeof(line), "%s: after iter=%d, err=%.10e", solver_name(args_.kind), i + 1,
+                         hist[i]);
+                log(line);
+            }
+        }
+        if (iters) iters[k] = fi;
+        if (errs) errs[k] = fe;
+        if (codes) codes[k] = code;
+        if (!args_.ignore_errors && code != 0) {  // stochastic-estimator.cc:181-184
+            snprintf(line, sizeof(line), "Incomplete convergence in solver, errcode=%d", code);
+            log(line);
+            status = 100 + code;
+            break;
+        }
+        SEDI_HIP(launch("accumulate_kernel", accumulate_kernel, ew_grid(n), dim3(256), s,
+                        (const double*)x_, (const double*)v_[cur], t_, q_, n));
+        SEDI_HIP(hipEventRecord(ev_free_[cur], s));
+    }
+    if (D) {
+        double* dd = D;
+        const bool host = !is_device_pointer(D);
+        if (host) dd = x_;  // x_ is free once the loop is over
+        SEDI_HIP(launch("divide_kernel", divide_kernel, ew_grid(n), dim3(256), s, (const double*)t_,
+                        (const double*)q_, dd, n));
+        SEDI_HIP(hipStreamSynchronize(s));
+        if (host) SEDI_HIP(hipMemcpy(D, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    SEDI_HIP(hipStreamSynchronize(s));
+    SEDI_HIP(hipStreamSynchronize(rng_stream_));
+    const auto t1 = std::chrono::steady_clock::now();
+    profile_flush();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return status;
+}
+
+}  // namespace sedi

@@ -1,0 +1,717 @@
+// plain.hip -- one-probe-at-a-time FP64 CG, pipelined CG and BiCGStab.
+//
+// Each solver iteration is a short sequence of fused passes (march.h).  A pass
+// streams the vectors it needs once, recomputes the stencil of its input
+// field on the fly, applies the pointwise updates of the reference's
+// recurrences in the reference's arithmetic order, and reduces its dot
+// products deterministically; the pass's last workgroup then advances the
+// solver scalars and the convergence test on the device.  Vectors that a pass
+// both reads at halo cells and rewrites are double-buffered by iteration
+// parity, so no workgroup reads a value another one has already overwritten.
+//
+// Per-iteration streamed vectors (FP64, N cells; the algorithmic bytes):
+//   CG        2 passes: [r, p_old -> p] + [x, r, p -> x, r]          = 8 N w
+//   PipelinedCg 1 pass: x,r,u,w,z,q,s,p read and written             = 16 N w
+//   BiCGStab  3 passes: [r,p,v,r^ -> p,v] + [r,v] + [x,p,r,v,r^ -> x,r] = 15 N w
+#include "plain.h"
+#include "profiler.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace sedi {
+
+// ---------------------------------------------------------------------------
+// device-side scalar updates (run by thread 0 of a pass's last workgroup)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void set_done(DevState* st, int code, int fi, double fe) {
+    st->code = code;
+    st->final_iter = fi;
+    st->final_err = fe;
+    st->done = 1;
+}
+
+// BiCgStab.hpp:235-263: the outcome checks in the reference's order, for the
+// value the loop variable had when the loop ended (maxiter+1 if it ran out).
+__device__ __forceinline__ void bicgstab_outcome(DevState* st, int iter) {
+    const double tol2 = st->tol * st->tol;
+    if (st->err <= st->tol) set_done(st, 0, iter, st->err);
+    else if (st->ss <= tol2) set_done(st, 0, iter, sqrt(st->ss / st->bb));
+    else if (st->omega == 0.0) set_done(st, 3, iter, st->err);
+    else if (st->rho == 0.0) set_done(st, 2, iter, st->err);
+    else set_done(st, 1, st->maxiter, st->err);
+}
+
+// Cg.hpp:196-206 / PipelinedCg.hpp:232-242 when the loop ran out.
+__device__ __forceinline__ void cg_outcome_exhausted(DevState* st) {
+    if (st->err <= st->tol) set_done(st, 0, st->maxiter + 1, st->err);
+    else set_done(st, 1, st->maxiter, st->err);
+}
+
+__global__ void init_state_kernel(DevState* st, int kind, int maxiter, double tol) {
+    if (threadIdx.x == 0) {
+        DevState s;
+        memset(&s, 0, sizeof(s));
+        s.kind = kind;
+        s.maxiter = maxiter;
+        s.tol = tol;
+        s.omega = 1.0;  // BiCgStab.hpp:148
+        *st = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// passes
+// ---------------------------------------------------------------------------
+// y = A x (Stencil.hpp:38-54).
+struct PassApply {
+    static constexpr int ND = 1;
+    double a, b, c;
+    const double* x;
+    double* y;
+    struct Cell { double x; };
+    __device__ Cell load(long long i) const { return {x[i]}; }
+    __device__ double f(const Cell& c) const { return c.x; }
+    __device__ void finish(long long i, const Cell&, double, double Af, double (&)[ND]) const {
+        y[i] = Af;
+    }
+};
+
+// r = b - A x  (Cg.hpp:119-123; PipelinedCg.hpp:113-117; BiCgStab.hpp:131-135)
+// with the dots <b,b>, <r,r> and <r, r.*iM>; optionally r_hat = r
+// (BiCgStab.hpp:154).
+struct PassResidual {
+    static constexpr int ND = 3;
+    double a, b, c;
+    const double* x;
+    const double* rhs;
+    double* r;
+    double* rhat;
+    Precond iM;
+    struct Cell { double x; };
+    __device__ Cell load(long long i) const { return {x[i]}; }
+    __device__ double f(const Cell& c) const { return c.x; }
+    __device__ void finish(long long i, const Cell&, double, double Af, double (&acc)[ND]) const {
+        const double bv = rhs[i];
+        const double rv = 1.0 * bv + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
+        r[i] = rv;
+        if (rhat) rhat[i] = rv;
+        acc[0] += bv * bv;
+        acc[1] += rv * rv;
+        acc[2] += rv * (rv * iM.at(i));
+    }
+};
+
+// CG direction: p = u + beta*p with u = r.*iM (Cg.hpp:175,188; p = u at the
+// first iteration, :144-147), then s = A p and <s,p> (:152-165).
+struct PassCgDirection {
+    static constexpr int ND = 1;
+    double a, b, c;
+    const double* r;
+    const double* pold;
+    double* pnew;
+    Precond iM;
+    double beta;
+    bool first;
+    struct Cell { double p; };
+    __device__ Cell load(long long i) const {
+        const double u = r[i] * iM.at(i);
+        if (first) return {u};
+        return {1.0 * u + beta * pold[i]};
+    }
+    __device__ double f(const Cell& c) const { return c.p; }
+    __device__ void finish(long long i, const Cell&, double p, double s, double (&acc)[ND]) const {
+        pnew[i] = p;
+        acc[0] += s * p;
+    }
+};
+
+// CG update: s = A p recomputed; x += alpha p; r -= alpha s; u = r.*iM;
+// <r,u>, <r,r> (Cg.hpp:168-182).
+struct PassCgUpdate {
+    static constexpr int ND = 2;
+    double a, b, c;
+    const double* p;
+    double* x;
+    double* r;
+    Precond iM;
+    double alpha;
+    struct Cell { double p; };
+    __device__ Cell load(long long i) const { return {p[i]}; }
+    __device__ double f(const Cell& c) const { return c.p; }
+    __device__ void finish(long long i, const Cell&, double pv, double s, double (&acc)[ND]) const {
+        const double xv = 1.0 * x[i] + alpha * pv;
+        const double rv = 1.0 * r[i] + (-alpha) * s;
+        x[i] = xv;
+        r[i] = rv;
+        const double u = rv * iM.at(i);
+        acc[0] += rv * u;
+        acc[1] += rv * rv;
+    }
+};
+
+// Pipelined CG set-up: u = r.*iM, w = A u (PipelinedCg.hpp:138-143), z=q=s=p=0
+// (:146-149), and the first iteration's <r,u>, <w,u>, <r,r> (:155,170).
+struct PassPcgInit {
+    static constexpr int ND = 3;
+    double a, b, c;
+    const double* r;
+    double* u;
+    double* w;
+    double *z, *q, *s, *p;
+    Precond iM;
+    struct Cell { double r, u; };
+    __device__ Cell load(long long i) const {
+        const double rv = r[i];
+        return {rv, rv * iM.at(i)};
+    }
+    __device__ double f(const Cell& c) const { return c.u; }
+    __device__ void finish(long long i, const Cell& c, double uv, double wv, double (&acc)[ND]) const {
+        u[i] = uv;
+        w[i] = wv;
+        z[i] = 0.0; q[i] = 0.0; s[i] = 0.0; p[i] = 0.0;
+        acc[0] += c.r * uv;
+        acc[1] += wv * uv;
+        acc[2] += c.r * c.r;
+    }
+};
+
+// Pipelined CG iteration (PipelinedCg.hpp:157-223): m = w.*iM, n = A m, the
+// eight recurrences in the reference's order, and the next <r,u>, <w,u>, <r,r>.
+struct PassPcgIter {
+    static constexpr int ND = 3;
+    double a, b, c;
+    const double* wold;
+    double* wnew;
+    double *x, *r, *u, *z, *q, *s, *p;
+    Precond iM;
+    double alpha, beta;
+    struct Cell { double w, m; };
+    __device__ Cell load(long long i) const {
+        const double wv = wold[i];
+        return {wv, wv * iM.at(i)};
+    }
+    __device__ double f(const Cell& c) const { return c.m; }
+    __device__ void finish(long long i, const Cell& c, double m, double n, double (&acc)[ND]) const {
+        const double qv = 1.0 * m + beta * q[i];
+        const double sv = 1.0 * c.w + beta * s[i];
+        const double uo = u[i];
+        const double pv = 1.0 * uo + beta * p[i];
+        const double zv = 1.0 * n + beta * z[i];
+        const double rv = 1.0 * r[i] + (-alpha) * sv;
+        const double xv = 1.0 * x[i] + alpha * pv;
+        const double uv = 1.0 * uo + (-alpha) * qv;
+        const double wv = 1.0 * c.w + (-alpha) * zv;
+        q[i] = qv; s[i] = sv; p[i] = pv; z[i] = zv;
+        r[i] = rv; x[i] = xv; u[i] = uv; wnew[i] = wv;
+        acc[0] += rv * uv;
+        acc[1] += wv * uv;
+        acc[2] += rv * rv;
+    }
+};
+
+// BiCGStab direction (BiCgStab.hpp:165-187): p = r + beta*p - beta*omega*v
+// (p = r at the first iteration), p^ = p.*iM, v = A p^, <v, r^>.
+struct PassBicgDirection {
+    static constexpr int ND = 1;
+    double a, b, c;
+    const double *r, *pold, *vold, *rhat;
+    double *pnew, *vnew;
+    Precond iM;
+    double beta, cbw;
+    bool first;
+    struct Cell { double p, ph; };
+    __device__ Cell load(long long i) const {
+        const double rv = r[i];
+        double pv;
+        if (first) pv = rv;
+        else pv = 1.0 * rv + beta * pold[i] + cbw * vold[i];  // combine3
+        return {pv, pv * iM.at(i)};
+    }
+    __device__ double f(const Cell& c) const { return c.ph; }
+    __device__ void finish(long long i, const Cell& c, double, double v, double (&acc)[ND]) const {
+        pnew[i] = c.p;
+        vnew[i] = v;
+        acc[0] += v * rhat[i];
+    }
+};
+
+// BiCGStab half step (BiCgStab.hpp:190-213): s = r - alpha v, s^ = s.*iM,
+// t = A s^, <s,s>, <t,s>, <t,t>; s and t are not stored.
+struct PassBicgHalf {
+    static constexpr int ND = 3;
+    double a, b, c;
+    const double *r, *v;
+    Precond iM;
+    double alpha;
+    struct Cell { double s, sh; };
+    __device__ Cell load(long long i) const {
+        const double sv = 1.0 * r[i] + (-alpha) * v[i];
+        return {sv, sv * iM.at(i)};
+    }
+    __device__ double f(const Cell& c) const { return c.sh; }
+    __device__ void finish(long long, const Cell& c, double, double t, double (&acc)[ND]) const {
+        acc[0] += c.s * c.s;
+        acc[1] += t * c.s;
+        acc[2] += t * t;
+    }
+};
+
+// BiCGStab update (BiCgStab.hpp:217-224 and :160 of the next iteration):
+// s, s^, t recomputed; x = x + alpha p^ + omega s^; r = s - omega t;
+// <r,r> and the next rho = <r, r^>.
+struct PassBicgUpdate {
+    static constexpr int ND = 2;
+    double a, b, c;
+    const double *r, *v, *p, *rhat;
+    double *x, *rnew;
+    Precond iM;
+    double alpha, omega;
+    struct Cell { double s, sh; };
+    __device__ Cell load(long long i) const {
+        const double sv = 1.0 * r[i] + (-alpha) * v[i];
+        return {sv, sv * iM.at(i)};
+    }
+    __device__ double f(const Cell& c) const { return c.sh; }
+    __device__ void finish(long long i, const Cell& c, double, double t, double (&acc)[ND]) const {
+        const double ph = p[i] * iM.at(i);
+        const double xv = 1.0 * x[i] + alpha * ph + omega * c.sh;  // combine3
+        const double rv = 1.0 * c.s + (-omega) * t;
+        x[i] = xv;
+        rnew[i] = rv;
+        acc[0] += rv * rv;
+        acc[1] += rv * rhat[i];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+#define PASS_SHARED                                  \
+    __shared__ double lds[2][TY + 2][TX + 2];        \
+    __shared__ double red[16];                       \
+    __shared__ int lastflag
+
+template <class P>
+__device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
+                                          double (&lds)[2][TY + 2][TX + 2],
+                                          double (&acc)[P::ND]) {
+#pragma unroll
+    for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
+    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x) march_tile(p, g, t, lds, acc);
+}
+
+__global__ __launch_bounds__(NTHREADS) void stencil_apply_kernel(TileGrid g, double a, double b,
+                                                                 double c, const double* x,
+                                                                 double* y) {
+    __shared__ double lds[2][TY + 2][TX + 2];
+    PassApply P{a, b, c, x, y};
+    double acc[1];
+    run_tiles(P, g, lds, acc);
+}
+
+__global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
+                                                            double c, const double* x,
+                                                            const double* rhs, double* r,
+                                                            double* rhat, Precond iM) {
+    PASS_SHARED;
+    PassResidual P{a, b, c, x, rhs, r, rhat, iM};
+    double acc[3];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    double bb = acc[0];
+    if (bb == 0.0) bb = 1.0;  // Cg.hpp:115
+    st->bb = bb;
+    const double err = sqrt(acc[1] / bb);
+    st->err = err;
+    if (err < st->tol) {  // Cg.hpp:130-134 (strict <, unlike the loop's <=)
+        set_done(st, 0, 0, err);
+        return;
+    }
+    if (st->kind == SEDI_CG) {
+        st->ru = acc[2];
+        st->iter = 1;
+    } else if (st->kind == SEDI_BICGSTAB) {
+        // rho = <r, r^> with r^ = r (BiCgStab.hpp:154,160)
+        st->rho = acc[1];
+        st->rho_prev = 0.0;
+        st->alpha = 0.0;
+        st->omega = 1.0;
+        st->ss = 0.0;
+        st->iter = 1;
+        if (st->maxiter < 1) bicgstab_outcome(st, st->maxiter + 1);
+        else if (st->rho == 0.0) bicgstab_outcome(st, 1);  // :163
+        return;
+    }
+    if (st->maxiter < 1) cg_outcome_exhausted(st);
+}
+
+__global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
+                                                                double b, double c, const double* r,
+                                                                double* p0, double* p1, Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int it = cst->iter;
+    PassCgDirection P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
+    double acc[1];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    st->alpha = st->ru / acc[0];  // Cg.hpp:166
+}
+
+__global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
+                                                             double c, const double* p0,
+                                                             const double* p1, double* x, double* r,
+                                                             Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int it = cst->iter;
+    PassCgUpdate P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
+    double acc[2];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    const double ru_new = acc[0];
+    st->beta = ru_new / st->ru;  // Cg.hpp:185
+    st->ru = ru_new;
+    const double err = sqrt(acc[1] / st->bb);  // :191
+    st->err = err;
+    ctl.hist[it - 1] = err;
+    st->nhist = it;
+    if (err <= st->tol) set_done(st, 0, it, err);  // :193,197-201
+    else if (it >= st->maxiter) cg_outcome_exhausted(st);
+    else st->iter = it + 1;
+}
+
+__global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl, double a, double b,
+                                                            double c, const double* r, double* u,
+                                                            double* w, double* z, double* q,
+                                                            double* s, double* p, Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    PassPcgInit P{a, b, c, r, u, w, z, q, s, p, iM};
+    double acc[3];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    st->gamma = acc[0];  // PipelinedCg.hpp:171-173 at iter 1
+    st->delta = acc[1];
+    st->beta = 0.0;      // :179-180
+    st->alpha = acc[0] / acc[1];
+    st->err_pending = sqrt(acc[2] / st->bb);
+    st->iter = 1;
+}
+
+__global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
+                                                            double c, double* w0, double* w1,
+                                                            double* x, double* r, double* u,
+                                                            double* z, double* q, double* s,
+                                                            double* p, Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int it = cst->iter;
+    // iteration 1 reads w from w0 (written by pcg_init_kernel)
+    PassPcgIter P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
+                  cst->alpha, cst->beta};
+    double acc[3];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    const double err = st->err_pending;  // rr before this iteration's update (:227)
+    st->err = err;
+    ctl.hist[it - 1] = err;
+    st->nhist = it;
+    if (err <= st->tol) { set_done(st, 0, it, err); return; }
+    if (it >= st->maxiter) { cg_outcome_exhausted(st); return; }
+    // scalars of iteration it+1 (PipelinedCg.hpp:167-178)
+    const double gamma_old = st->gamma;
+    const double gamma = acc[0], delta = acc[1];
+    const double beta = gamma / gamma_old;
+    const double alpha = 1.0 / (delta / gamma - beta / st->alpha);
+    st->gamma = gamma;
+    st->delta = delta;
+    st->beta = beta;
+    st->alpha = alpha;
+    st->err_pending = sqrt(acc[2] / st->bb);
+    st->iter = it + 1;
+}
+
+__global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
+    TileGrid g, Ctl ctl, double a, double b, double c, double* const* vv, Precond iM) {
+    // vv: r0, r1, rhat, p0, p1, v0, v1
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int it = cst->iter;
+    const int cur = it & 1, prv = cur ^ 1;
+    PassBicgDirection P{a, b, c, vv[0 + cur], vv[3 + prv], vv[5 + prv], vv[2],
+                        vv[3 + cur], vv[5 + cur], iM, cst->beta, cst->cbw, it == 1};
+    double acc[1];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    st->alpha = st->rho / acc[0];  // BiCgStab.hpp:187
+}
+
+__global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl ctl, double a,
+                                                                 double b, double c,
+                                                                 double* const* vv, Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int cur = cst->iter & 1;
+    PassBicgHalf P{a, b, c, vv[0 + cur], vv[5 + cur], iM, cst->alpha};
+    double acc[3];
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    st->ss = acc[0];                               // :193-194
+    if (acc[0] < st->tol * st->tol) st->sexit = 1;  // :195 (absolute test)
+    else st->omega = acc[1] / acc[2];              // :213
+}
+
+__global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, Ctl ctl, double a,
+                                                                   double b, double c,
+                                                                   double* const* vv, double* x,
+                                                                   Precond iM) {
+    const DevState* cst = ctl.st;
+    if (cst->done) return;
+    PASS_SHARED;
+    const int it = cst->iter;
+    const int cur = it & 1;
+    double acc[2];
+    if (cst->sexit) {
+        // x = x + alpha p^ and stop (BiCgStab.hpp:195-199)
+        const double alpha = cst->alpha;
+        const double* p = vv[3 + cur];
+        const long long n = (long long)g.n1 * g.n2 * g.n3;
+        for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
+             i += (long long)gridDim.x * NTHREADS) {
+            const double ph = p[i] * iM.at(i);
+            x[i] = 1.0 * x[i] + alpha * ph;
+        }
+        acc[0] = acc[1] = 0.0;
+        if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+        bicgstab_outcome(ctl.st, it);
+        return;
+    }
+    PassBicgUpdate P{a, b, c, vv[0 + cur], vv[5 + cur], vv[3 + cur], vv[2],
+                     x, vv[0 + (cur ^ 1)], iM, cst->alpha, cst->omega};
+    run_tiles(P, g, lds, acc);
+    if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    DevState* st = ctl.st;
+    const double err = sqrt(acc[0] / st->bb);  // :223-227
+    st->err = err;
+    ctl.hist[st->nhist] = err;
+    st->nhist += 1;
+    if (err <= st->tol || st->omega == 0.0) { bicgstab_outcome(st, it); return; }  // :229-230
+    if (it >= st->maxiter) { bicgstab_outcome(st, st->maxiter + 1); return; }
+    st->rho_prev = st->rho;  // :232
+    st->rho = acc[1];        // next iteration's :160
+    if (st->rho == 0.0) { bicgstab_outcome(st, it + 1); return; }  // :163
+    const double beta = (st->rho / st->rho_prev) * (st->alpha / st->omega);  // :167
+    st->beta = beta;
+    st->cbw = -beta * st->omega;  // the combine3 coefficient of :169
+    st->iter = it + 1;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int pick_zc(const Shape& s) {
+    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + TY - 1) / TY);
+    long long zc = (long long)s.n3 * nxy / 2048;
+    if (zc < 2) zc = 2;
+    if (zc > 64) zc = 64;
+    return (int)zc;
+}
+
+static int max_blocks() {
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached = 8 * cus;
+    }
+    return cached;
+}
+
+int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
+                      hipStream_t stream) {
+    const TileGrid g = make_tile_grid(s, pick_zc(s));
+    const int nb = (int)std::min<long long>(g.ntiles, max_blocks());
+    SEDI_HIP(launch("stencil_apply_kernel", stencil_apply_kernel, dim3(nb), dim3(NTHREADS), stream,
+                    g, a, b, c, x, y));
+    return SEDI_OK;
+}
+
+PlainSolver::~PlainSolver() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int PlainSolver::create(const Shape& s, double a, double b, double c, int kind, int mode) {
+    SEDI_REQUIRE(s.n1 > 0 && s.n2 > 0 && s.n3 > 0, SEDI_ERR_ARG, "grid dimensions must be positive");
+    SEDI_REQUIRE(kind == SEDI_CG || kind == SEDI_PCG || kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
+                 "unknown solver kind %d", kind);
+    shape_ = s;
+    a_ = a; b_ = b; c_ = c;
+    kind_ = kind;
+    mode_ = mode;
+    tg_ = make_tile_grid(s, pick_zc(s));
+    nblocks_ = (int)std::min<long long>(tg_.ntiles, max_blocks());
+    SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SEDI_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
+    SEDI_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
+    SEDI_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(DevState), hipHostMallocDefault));
+    // control block: state | counter | partials
+    const size_t ctl_bytes = 512 + sizeof(double) * 3 * (size_t)nblocks_;
+    SEDI_HIP(ctl_.reserve(ctl_bytes));
+    SEDI_HIP(hipMemset(ctl_.p, 0, ctl_bytes));
+    const int nv = kind == SEDI_CG ? 3 : (kind == SEDI_PCG ? 8 : 7);
+    const size_t n = (size_t)s.n();
+    SEDI_HIP(vec_.reserve(nv * n * sizeof(double)));
+    SEDI_HIP(hipMemset(vec_.p, 0, nv * n * sizeof(double)));
+    for (int k = 0; k < nv; k++) v_[k] = vec_.as<double>() + k * n;
+    if (kind == SEDI_BICGSTAB) {
+        // BiCGStab's vector table (r0 r1 r^ p0 p1 v0 v1) sits in the control
+        // block after the counter; its kernels pick buffers by parity.
+        double* table[7] = {v_[0], v_[1], v_[2], v_[3], v_[4], v_[5], v_[6]};
+        SEDI_HIP(hipMemcpy(static_cast<char*>(ctl_.p) + 384, table, sizeof(table), hipMemcpyHostToDevice));
+    }
+    return SEDI_OK;
+}
+
+int PlainSolver::launch_iterations(int count, Precond pc, const double* b, double* x) {
+    (void)b;
+    Ctl ctl{reinterpret_cast<DevState*>(ctl_.p),
+            reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
+            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
+    const dim3 grid(nblocks_), block(NTHREADS);
+    for (int k = 0; k < count; k++) {
+        if (kind_ == SEDI_CG) {
+            SEDI_HIP(launch("cg_direction_kernel", cg_direction_kernel, grid, block, stream_, tg_, ctl,
+                            a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
+            SEDI_HIP(launch("cg_update_kernel", cg_update_kernel, grid, block, stream_, tg_, ctl, a_,
+                            b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
+        } else if (kind_ == SEDI_PCG) {
+            SEDI_HIP(launch("pcg_iter_kernel", pcg_iter_kernel, grid, block, stream_, tg_, ctl, a_, b_,
+                            c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
+        } else {
+            double* const* vv = reinterpret_cast<double* const*>(static_cast<char*>(ctl_.p) + 384);
+            SEDI_HIP(launch("bicgstab_direction_kernel", bicgstab_direction_kernel, grid, block,
+                            stream_, tg_, ctl, a_, b_, c_, vv, pc));
+            SEDI_HIP(launch("bicgstab_half_kernel", bicgstab_half_kernel, grid, block, stream_, tg_,
+                            ctl, a_, b_, c_, vv, pc));
+            SEDI_HIP(launch("bicgstab_update_kernel", bicgstab_update_kernel, grid, block, stream_,
+                            tg_, ctl, a_, b_, c_, vv, x, pc));
+        }
+    }
+    return SEDI_OK;
+}
+
+int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int maxiter, double tol,
+                           double* final_err, int* final_iter) {
+    SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
+    SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)std::max(maxiter, 1)));
+    DevState* st = reinterpret_cast<DevState*>(ctl_.p);
+    Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
+            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
+    const Precond pc{iM, 1.0 / a_};
+    const dim3 grid(nblocks_), block(NTHREADS);
+
+    SEDI_HIP(launch("init_state_kernel", init_state_kernel, dim3(1), dim3(64), stream_, st, kind_,
+                    maxiter, tol));
+    if (kind_ == SEDI_BICGSTAB) {
+        // r_1 lives in r[1] (buffers alternate with the iteration's parity)
+        SEDI_HIP(launch("residual_kernel", residual_kernel, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)x, b, v_[1], v_[2], pc));
+    } else {
+        SEDI_HIP(launch("residual_kernel", residual_kernel, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
+    }
+    if (kind_ == SEDI_PCG)
+        SEDI_HIP(launch("pcg_init_kernel", pcg_init_kernel, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
+
+    // Iterations go out in chunks; the host looks at the device's done flag
+    // of the previous chunk while the next one runs, so the GPU never idles
+    // on the host.  Kernels launched after convergence return at once.
+    const int chunk = 16;
+    int launched = 0, slot = 0;
+    bool pending = false;
+    while (launched < maxiter) {
+        const int cnt = std::min(chunk, maxiter - launched);
+        int rc = launch_iterations(cnt, pc, b, x);
+        if (rc) return rc;
+        launched += cnt;
+        SEDI_HIP(hipMemcpyAsync(&pinned_[slot], st, sizeof(DevState), hipMemcpyDeviceToHost, stream_));
+        SEDI_HIP(hipEventRecord(ev_[slot], stream_));
+        if (pending) {
+            SEDI_HIP(hipEventSynchronize(ev_[slot ^ 1]));
+            if (pinned_[slot ^ 1].done) break;
+        }
+        pending = true;
+        slot ^= 1;
+    }
+    SEDI_HIP(hipStreamSynchronize(stream_));
+    profile_flush();
+    DevState hs;
+    SEDI_HIP(hipMemcpy(&hs, st, sizeof(DevState), hipMemcpyDeviceToHost));
+    SEDI_REQUIRE(hs.done, SEDI_ERR_STATE, "solver stopped without a result (iter %d)", hs.iter);
+    nhist_ = hs.nhist;
+    hist_host_.resize(std::max(nhist_, 0));
+    if (nhist_ > 0)
+        SEDI_HIP(hipMemcpy(hist_host_.data(), hist_.p, sizeof(double) * nhist_, hipMemcpyDeviceToHost));
+    if (final_err) *final_err = hs.final_err;
+    if (final_iter) *final_iter = hs.final_iter;
+    return hs.code;
+}
+
+int PlainSolver::solve_any(const double* b, double* x, const double* iM, int maxiter, double tol,
+                           double* final_err, int* final_iter) {
+    SEDI_REQUIRE(b && x, SEDI_ERR_ARG, "b and x must not be NULL");
+    const size_t bytes = sizeof(double) * (size_t)shape_.n();
+    const double* bd = b;
+    double* xd = x;
+    const double* md = iM;
+    const bool xhost = !is_device_pointer(x);
+    if (!is_device_pointer(b)) {
+        SEDI_HIP(stage_b_.reserve(bytes));
+        SEDI_HIP(hipMemcpy(stage_b_.p, b, bytes, hipMemcpyHostToDevice));
+        bd = stage_b_.as<double>();
+    }
+    if (xhost) {
+        SEDI_HIP(stage_x_.reserve(bytes));
+        SEDI_HIP(hipMemcpy(stage_x_.p, x, bytes, hipMemcpyHostToDevice));
+        xd = stage_x_.as<double>();
+    }
+    if (iM && !is_device_pointer(iM)) {
+        SEDI_HIP(stage_m_.reserve(bytes));
+        SEDI_HIP(hipMemcpy(stage_m_.p, iM, bytes, hipMemcpyHostToDevice));
+        md = stage_m_.as<double>();
+    }
+    const int rc = solve_dev(bd, xd, md, maxiter, tol, final_err, final_iter);
+    if (rc < 0) return rc;
+    if (xhost) SEDI_HIP(hipMemcpy(x, xd, bytes, hipMemcpyDeviceToHost));
+    return rc;
+}
+
+int PlainSolver::err_history(double* out, int cap) const {
+    if (out)
+        for (int k = 0; k < std::min(cap, nhist_); k++) out[k] = hist_host_[k];
+    return nhist_;
+}
+
+}  // namespace sedi

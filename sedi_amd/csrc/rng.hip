@@ -1,0 +1,176 @@
+// rng.hip -- Rademacher probes from a device-resident mt19937 stream.
+//
+// The reference fills each probe with std::mt19937 and
+// std::uniform_real_distribution<double>(0,1), +1 iff the draw is > 0.5, cells
+// in i3,i2,i1 order (DistrRngStd.hpp:27-41).  Here the stream is
+// std::mt19937(seed) per run: cell c of sample k takes 32-bit outputs 2Nk+2c
+// (x0) and 2Nk+2c+1 (x1) and the draw is libstdc++'s generate_canonical:
+// (x0 + x1 * 2^32) / 2^64 in double arithmetic.
+//
+// mt19937 (Matsumoto & Nishimura 1998) regenerates its 624-word state in
+// place; word i depends on words i, i+1 and i+397 (mod 624) of the state as
+// it stands, so a block regeneration splits into three dependent phases of
+// 227, 227 and 170 words.  One workgroup keeps the state in LDS, runs the
+// three phases with a barrier each (double-buffered state, so a phase never
+// reads a word it overwrites), tempers the new words as it produces them and
+// turns each pair of outputs into a probe value.  The stream is sequential by
+// nature; the estimator runs it on its own HIP stream, one probe ahead of the
+// solver, on a single CU.
+#include "rng.h"
+#include "profiler.h"
+
+namespace sedi {
+
+namespace {
+constexpr int MT_N = 624, MT_M = 397;
+constexpr uint32_t MT_A = 0x9908b0dfu, UPPER = 0x80000000u, LOWER = 0x7fffffffu;
+
+__device__ __forceinline__ uint32_t temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+__device__ __forceinline__ uint32_t twist(uint32_t cur, uint32_t next, uint32_t far) {
+    const uint32_t y = (cur & UPPER) | (next & LOWER);
+    return far ^ (y >> 1) ^ ((y & 1u) ? MT_A : 0u);
+}
+
+// libstdc++ generate_canonical<double,53> on two outputs, then `> 0.5`.
+__device__ __forceinline__ double rademacher(uint32_t x0, uint32_t x1) {
+    double sum = (double)x0 * 1.0;
+    sum = sum + (double)x1 * 4294967296.0;
+    double ret = sum / 18446744073709551616.0;
+    if (ret >= 1.0) ret = 0.99999999999999989;  // nextafter(1, 0)
+    return ret > 0.5 ? 1.0 : -1.0;
+}
+}  // namespace
+
+__global__ void mt_seed_kernel(MtState* s, uint32_t seed) {
+    if (threadIdx.x == 0) {
+        uint32_t v = seed;
+        s->mt[0] = v;
+        for (int i = 1; i < MT_N; i++) {
+            v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+            s->mt[i] = v;
+        }
+        s->idx = MT_N;
+        s->carry_valid = 0;
+        s->carry = 0;
+    }
+}
+
+// Produces `ncells` probe values from the stream (or only advances it by
+// `skip` outputs when out == nullptr).  One workgroup of 256 lanes.
+__global__ __launch_bounds__(256) void mt_fill_kernel(MtState* s, double* out, long long ncells,
+                                                      unsigned long long skip) {
+    __shared__ uint32_t S[2][MT_N];   // state, double-buffered by block
+    __shared__ uint32_t O[2][MT_N];   // tempered outputs of the block
+    const int t = threadIdx.x;
+    for (int i = t; i < MT_N; i += 256) S[0][i] = s->mt[i];
+    int idx = s->idx;
+    int carry_valid = s->carry_valid;
+    uint32_t carry = s->carry;
+    __syncthreads();
+    int cur = 0;
+    if (idx < MT_N)
+        for (int i = t; i < MT_N; i += 256) O[0][i] = temper(S[0][i]);
+    __syncthreads();
+
+    long long done = 0;  // cells written
+    unsigned long long skipped = 0;
+    const bool skipping = (out == nullptr);
+    while (skipping ? (skipped < skip) : (done < ncells)) {
+        if (idx >= MT_N) {
+            // regenerate: three phases, reading S[cur], writing S[nxt] and O[nxt]
+            const int nxt = cur ^ 1;
+            if (t < 227) {
+                const uint32_t v = twist(S[cur][t], S[cur][t + 1], S[cur][t + MT_M]);
+                S[nxt][t] = v;
+                O[nxt][t] = temper(v);
+            }
+            __syncthreads();
+            if (t < 227) {
+                const int i = t + 227;
+                const uint32_t v = twist(S[cur][i], S[cur][i + 1], S[nxt][i - 227]);
+                S[nxt][i] = v;
+                O[nxt][i] = temper(v);
+            }
+            __syncthreads();
+            if (t < 170) {
+                const int i = t + 454;
+                const uint32_t nx = (i + 1 < MT_N) ? S[cur][i + 1] : S[nxt][0];
+                const uint32_t v = twist(S[cur][i], nx, S[nxt][i - 227]);
+                S[nxt][i] = v;
+                O[nxt][i] = temper(v);
+            }
+            __syncthreads();
+            cur = nxt;
+            idx = 0;
+        }
+        if (skipping) {
+            const unsigned long long avail = (unsigned long long)(MT_N - idx);
+            const unsigned long long take = (skip - skipped) < avail ? (skip - skipped) : avail;
+            idx += (int)take;
+            skipped += take;
+            continue;
+        }
+        const uint32_t* o = O[cur];
+        if (carry_valid) {
+            if (t == 0) out[done] = rademacher(carry, o[idx]);
+            idx += 1;
+            done += 1;
+            carry_valid = 0;
+            continue;
+        }
+        long long pairs = (MT_N - idx) / 2;
+        if (pairs > ncells - done) pairs = ncells - done;
+        for (int p = t; p < pairs; p += 256) out[done + p] = rademacher(o[idx + 2 * p], o[idx + 2 * p + 1]);
+        idx += 2 * (int)pairs;
+        done += pairs;
+        if (done < ncells && idx == MT_N - 1) {
+            carry = o[idx];
+            carry_valid = 1;
+            idx = MT_N;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int i = t; i < MT_N; i += 256) s->mt[i] = S[cur][i];
+    if (t == 0) {
+        s->idx = idx;
+        s->carry_valid = carry_valid;
+        s->carry = carry;
+    }
+}
+
+int MtStream::init() {
+    if (state_.p) return SEDI_OK;
+    SEDI_HIP(state_.reserve(sizeof(MtState)));
+    return SEDI_OK;
+}
+
+int MtStream::seed(uint32_t seed, hipStream_t stream) {
+    int rc = init();
+    if (rc) return rc;
+    SEDI_HIP(launch("mt_seed_kernel", mt_seed_kernel, dim3(1), dim3(64), stream, state_.as<MtState>(), seed));
+    return SEDI_OK;
+}
+
+int MtStream::skip(unsigned long long draws, hipStream_t stream) {
+    if (!draws) return SEDI_OK;
+    SEDI_HIP(launch("mt_fill_kernel", mt_fill_kernel, dim3(1), dim3(256), stream, state_.as<MtState>(),
+                    (double*)nullptr, 0LL, draws));
+    return SEDI_OK;
+}
+
+int MtStream::fill(double* v, long long ncells, hipStream_t stream) {
+    if (ncells <= 0) return SEDI_OK;
+    SEDI_HIP(launch("mt_fill_kernel", mt_fill_kernel, dim3(1), dim3(256), stream, state_.as<MtState>(), v,
+                    ncells, 0ULL));
+    return SEDI_OK;
+}
+
+}  // namespace sedi

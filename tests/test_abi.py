@@ -1,0 +1,60 @@
+"""The C ABI library loads and exports every entry point include/sedi_c.h declares
+(no compute call: this runs without a GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "sedi_c.h")
+LIB = os.path.join(ROOT, "sedi_amd", "libsedi_amd.so")
+CLI = os.path.join(ROOT, "sedi_amd", "bin", "sedi-estimator")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:int|void|char|double)\s*\*?\s*(sedi_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ["sedi_stencil_apply", "sedi_solver_create", "sedi_solve", "sedi_solver_destroy",
+                 "sedi_rademacher_fill", "sedi_estimate", "sedi_estimator_create",
+                 "sedi_estimator_run", "sedi_last_error"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.fail("libsedi_amd.so is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_reports_errors_without_gpu_work():
+    lib = ctypes.CDLL(LIB)
+    lib.sedi_last_error.restype = ctypes.c_char_p
+    lib.sedi_solver_create.argtypes = [ctypes.c_void_p] * 5
+    # NULL arguments are rejected before any device call
+    rc = lib.sedi_solver_create(None, None, 2, 0, None)
+    assert rc == -1
+    assert b"NULL" in lib.sedi_last_error()
+    assert lib.sedi_version() == 1
+
+
+def test_cli_usage_errors():
+    if not os.path.exists(CLI):
+        pytest.fail("sedi-estimator is not built")
+    # -g and -p are required (CmdLineParser.hpp:81-93): usage text, exit 1
+    r = subprocess.run([CLI], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    r = subprocess.run([CLI, "-g", "8x8x8"], capture_output=True, text=True)
+    assert r.returncode == 1
+    r = subprocess.run([CLI, "-g", "8x8", "-p", "1x1x1"], capture_output=True, text=True)
+    assert r.returncode == 1
