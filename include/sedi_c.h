@@ -109,6 +109,13 @@ void sedi_solver_destroy(sedi_solver *solver);
 int sedi_rademacher_fill(uint32_t seed, uint64_t first_draw, int64_t ncells,
                          double *v);
 
+/* Host-only utility, no reference counterpart: the 624-word state of
+ * std::mt19937(seed) after `draws` outputs, oldest word first, computed by
+ * jump-ahead (x^draws mod the generator's characteristic polynomial) instead
+ * of generating the draws.  The batched estimator uses it to start every
+ * probe's stream segment in parallel. */
+int sedi_mt_state_after(uint32_t seed, uint64_t draws, uint32_t *state624);
+
 /* ---- estimator ------------------------------------------------------------ */
 /* The sample loop of stochastic-estimator.cc:57-232 (CLI -g -s -e -m -i -f)
  * with seeded probes: sample k solves A x = v_k from x = 0 with v_k cells

@@ -50,6 +50,12 @@ void oracle_mt_seed(oracle_mt19937 *s, uint32_t seed)
     s->idx = MT_N;
 }
 
+void oracle_mt_from_words(oracle_mt19937 *s, const uint32_t *words)
+{
+    memcpy(s->mt, words, sizeof(s->mt));
+    s->idx = MT_N;
+}
+
 static void mt_regenerate(oracle_mt19937 *s)
 {
     for (int i = 0; i < MT_N; i++) {

@@ -30,6 +30,9 @@ typedef struct oracle_mt19937 {
 
 void     oracle_mt_seed(oracle_mt19937 *s, uint32_t seed);
 uint32_t oracle_mt_next(oracle_mt19937 *s);
+/* Continue from 624 given state words (oldest first); the next output is the
+ * tempered word generated after them. */
+void     oracle_mt_from_words(oracle_mt19937 *s, const uint32_t *words);
 /* One std::uniform_real_distribution<double>(0,1) draw (libstdc++
  * generate_canonical<double,53>: two 32-bit outputs). */
 double   oracle_uniform01(oracle_mt19937 *s);

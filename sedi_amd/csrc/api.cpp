@@ -5,6 +5,7 @@
 #include <new>
 
 #include "estimator.h"
+#include "mt_jump.h"
 #include "profiler.h"
 #include "plain.h"
 #include "rng.h"
@@ -135,12 +136,22 @@ int sedi_rademacher_fill(uint32_t seed, uint64_t first_draw, int64_t ncells, dou
             SEDI_HIP(vs.reserve(bytes));
             vd = vs.as<double>();
         }
-        if ((rc = mt.seed(seed, s))) return rc;
-        if ((rc = mt.skip(first_draw, s))) return rc;
+        if ((rc = mt.seed_at(seed, first_draw, s))) return rc;
         if ((rc = mt.fill(vd, ncells, s))) return rc;
         SEDI_HIP(hipDeviceSynchronize());
         sedi::profile_flush();
         if (host) SEDI_HIP(hipMemcpy(v, vd, bytes, hipMemcpyDeviceToHost));
+        return SEDI_OK;
+    });
+}
+
+int sedi_mt_state_after(uint32_t seed, uint64_t draws, uint32_t* state624) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(state624, SEDI_ERR_ARG, "state624 is NULL");
+        SEDI_REQUIRE(sedi::mt_charpoly_ok(), SEDI_ERR_STATE, "mt19937 characteristic polynomial unavailable");
+        uint32_t s0[624];
+        sedi::mt_seed_state(seed, s0);
+        sedi::mt_jump_state(s0, draws, state624);
         return SEDI_OK;
     });
 }

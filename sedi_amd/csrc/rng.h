@@ -20,6 +20,8 @@ class MtStream {
   public:
     int init();
     int seed(uint32_t seed, hipStream_t stream);
+    // seed, then jump ahead by first_draw outputs on the host
+    int seed_at(uint32_t seed, unsigned long long first_draw, hipStream_t stream);
     int skip(unsigned long long draws, hipStream_t stream);
     int fill(double* v, long long ncells, hipStream_t stream);
 

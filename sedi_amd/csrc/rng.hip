@@ -17,6 +17,7 @@
 // nature; the estimator runs it on its own HIP stream, one probe ahead of the
 // solver, on a single CU.
 #include "rng.h"
+#include "mt_jump.h"
 #include "profiler.h"
 
 namespace sedi {
@@ -156,6 +157,21 @@ int MtStream::seed(uint32_t seed, hipStream_t stream) {
     int rc = init();
     if (rc) return rc;
     SEDI_HIP(launch("mt_seed_kernel", mt_seed_kernel, dim3(1), dim3(64), stream, state_.as<MtState>(), seed));
+    return SEDI_OK;
+}
+
+int MtStream::seed_at(uint32_t seed, unsigned long long first_draw, hipStream_t stream) {
+    int rc = init();
+    if (rc) return rc;
+    MtState h;
+    uint32_t s0[624];
+    mt_seed_state(seed, s0);
+    mt_jump_state(s0, first_draw, h.mt);
+    h.idx = 624;
+    h.carry_valid = 0;
+    h.carry = 0;
+    SEDI_HIP(hipMemcpyAsync(state_.p, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+    SEDI_HIP(hipStreamSynchronize(stream));
     return SEDI_OK;
 }
 
