@@ -48,7 +48,49 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-iters", type=int, default=300,
                    help="iterations of the bounded CPU-baseline sample")
+    p.add_argument("--stencil", action="store_true",
+                   help="time the standalone 7-point stencil kernel (y = A x, FP64) at 256^3 and exit")
+    p.add_argument("--pmc-workload", action="store_true",
+                   help="run a short fixed workload for rocprofv3 --pmc (1 probe per solver, "
+                        "10 iterations each, at 256^3) and exit")
     return p.parse_args()
+
+
+def pmc_workload():
+    """One probe per solver, capped at 10 iterations (-m 10 -i), on the 256^3
+    grid: a few dozen dispatches of every kernel, including the estimator's
+    streaming accumulate_kernel (reads x, v, t, q; writes t, q), whose known
+    traffic calibrates the counters."""
+    import sedi_amd
+    for kind in (2, 3, 1):
+        est = sedi_amd.Estimator(GRID, kind=kind, samples=1, maxiter=10, tol=TOL, seed=SEED,
+                                 ignore_errors=True)
+        est.run(want_D=False)
+        est.close()
+    print("pmc workload done", flush=True)
+
+
+def stencil_only(reps=20):
+    """y = A x at 256^3 through sedi_stencil_apply with device-resident x, y
+    (allocated by the library's own staging on the first call), kernel time
+    from HIP events; algorithmic bytes 2*N*8 per launch (read x, write y)."""
+    import numpy as np
+    import sedi_amd
+    n = GRID[0] * GRID[1] * GRID[2]
+    x = np.random.default_rng(0).standard_normal(n)
+    st = sedi_amd.Stencil()
+    st.apply(GRID, x)  # warm-up
+    sedi_amd.profile_reset()
+    sedi_amd.profile_enable("stencil_apply_kernel")
+    for _ in range(reps):
+        st.apply(GRID, x)
+    sedi_amd.profile_enable(None)
+    launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
+    avg = ms / 1e3 / launches
+    gbs = 2 * n * 8 / avg / 1e9
+    print(json.dumps({"kernel": "stencil_apply_kernel", "grid": list(GRID), "launches": launches,
+                      "avg_launch_us": round(avg * 1e6, 2), "achieved_GBs": round(gbs, 1),
+                      "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
 
 
 def traffic_from_profiles(kernel):
@@ -119,6 +161,12 @@ def cpu_baseline(kind, gpu_iters_mean, iters_cap):
 
 def main():
     args = parse()
+    if args.pmc_workload:
+        pmc_workload()
+        return
+    if args.stencil:
+        stencil_only()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)

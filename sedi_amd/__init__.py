@@ -27,7 +27,8 @@ __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEG
            "Estimator", "estimate", "profile_enable", "profile_get", "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-lib_path = os.path.join(PKG_DIR, "libsedi_amd.so")
+# SEDI_AMD_LIB selects another build of the library (e.g. a tuning variant)
+lib_path = os.environ.get("SEDI_AMD_LIB") or os.path.join(PKG_DIR, "libsedi_amd.so")
 
 SOLUTION_FOUND, REACHED_MAXITER, BREAKDOWN_RHO, BREAKDOWN_OMEGA = 0, 1, 2, 3
 BICGSTAB, CG, PCG = 1, 2, 3

@@ -33,12 +33,12 @@ LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "plain.hip", "rng.hip", 
 DRIVER_SOURCE = os.path.join(CSRC, "driver", "sedi_estimator.cpp")
 
 
-def _compile(src):
+def _compile(src, extra=(), build_dir=BUILD):
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, src + ".o")
+    obj = os.path.join(build_dir, src + ".o")
     # every source is HIP (the .cpp files hold host code that includes the
     # device headers), so all of them get the gfx950 device flags
-    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + ["-c", path, "-o", obj]
+    cmd = [HIPCC, "-x", "hip"] + DEVICE + COMMON + list(extra) + ["-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s%s" % (" ".join(cmd), res.stdout, res.stderr))
@@ -48,23 +48,27 @@ def _compile(src):
     return obj, warn
 
 
-def build(verbose=True):
+def build(verbose=True, extra=(), lib=LIB, build_dir=BUILD, driver=True):
+    """Compiles the library (and the CLI driver unless driver=False); `extra`
+    adds compiler flags, `lib`/`build_dir` redirect the outputs (variants)."""
     if not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
-    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(build_dir, exist_ok=True)
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     jobs = min(len(LIB_SOURCES), max(1, (os.cpu_count() or 4)))
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(_compile, LIB_SOURCES))
+        results = list(ex.map(lambda s: _compile(s, extra, build_dir), LIB_SOURCES))
     objs = []
     for obj, warn in results:
         objs.append(obj)
         if warn and verbose:
             print(warn, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+    if not driver:
+        return lib
     # CLI driver: plain C++ host code over the C ABI
     cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DRIVER_SOURCE,
            "-o", BIN + ".tmp", "-L" + PKG, "-lsedi_amd", "-Wl,-rpath,$ORIGIN/.."]

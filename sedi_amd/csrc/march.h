@@ -7,12 +7,23 @@
 // neighbours come from registers (z) and an LDS plane (x, y).
 //
 // Tiling (gfx950, wave64): a workgroup is 4 waves = 256 lanes covering a tile
-// of TX=64 cells along x (one wave row, 512 contiguous bytes per vector) by
-// TY=4 rows along y, and marches over a chunk of ZC planes along z.  For each
-// plane it evaluates f at its 256 cells plus the 2*64 + 2*4 in-plane halo
-// cells (lanes 0..31 of every wave take half a y-halo row, lanes 32/33 the two
-// x-halo cells of their row), stores them in a double-buffered LDS plane and
-// needs one workgroup barrier per plane.  The z neighbours stay in registers.
+// of TX=64 cells along x (one wave row, 512 contiguous bytes per FP64 vector)
+// by TY=4 rows along y, and marches over a chunk of ZC planes along z.  For
+// each plane it evaluates f at its 256 cells plus the 2*64 + 2*4 in-plane
+// halo cells (lanes 0..31 of every wave take half a y-halo row, lanes 32/33
+// the two x-halo cells of their row), stores them in a double-buffered LDS
+// plane and needs one workgroup barrier per plane.  The z neighbours stay in
+// registers.
+//
+// Latency: all loads are issued D planes before they are consumed, into a
+// ring of registers whose slots are compile-time constants (the z loop is
+// unrolled by the ring size), so every wave keeps D planes of loads in flight
+// across the per-plane barrier.
+//
+// Tile order: workgroup b runs on XCD b % 8.  When the tile count allows, the
+// tiles are renumbered so that each XCD gets a contiguous run of tiles (whole
+// xy layers of a z chunk); a tile's x/y halo cells are then read by workgroups
+// of the same XCD at about the same time and hit that XCD's L2.
 //
 // Boundary: f is 0 outside the grid (the reference's zero ghost cells,
 // Cg.hpp:57-62; DistrAsyncStencil.hpp:74-86 with non-periodic neighbours,
@@ -32,6 +43,7 @@ namespace sedi {
 constexpr int TX = 64;
 constexpr int TY = 4;
 constexpr int NTHREADS = TX * TY;
+constexpr int NXCD = 8;
 
 struct TileGrid {
     int n1, n2, n3;
@@ -80,24 +92,45 @@ __device__ __forceinline__ void block_sum(double (&acc)[ND], double* red /* LDS 
 
 // Cross-workgroup reduction: each workgroup publishes its ND partial sums; the
 // last one to arrive sums all partials in a fixed order and returns true with
-// the totals in thread 0's `tot` (the other threads of that workgroup see
+// the totals in thread 0's `acc` (the other threads of that workgroup see
 // true too).  The arrival counter is re-armed by the last workgroup, so no
 // memset is needed between launches.
+//
+// Publication without a device-wide fence: a release fence at agent scope
+// writes back the whole L2 of the XCD (the XCDs' L2s are not coherent), once
+// per workgroup.  Instead the partials are stored with agent-scope atomic
+// stores (written through to the coherence point), the lane waits until they
+// have completed (vmcnt 0) before the agent-scope arrival increment, and the
+// last workgroup reads them with agent-scope atomic loads.
 template <int ND>
 __device__ bool grid_sum(double (&acc)[ND], double* partials, unsigned* counter,
                          double* red /* LDS >= 4*ND doubles */, int* flag /* LDS */) {
     block_sum<ND>(acc, red);
     const int nb = gridDim.x;
     if (threadIdx.x == 0) {
+#ifdef SEDI_GRID_SUM_FENCE
 #pragma unroll
         for (int d = 0; d < ND; d++) partials[(long long)d * nb + blockIdx.x] = acc[d];
         __threadfence();
         const unsigned prev = atomicAdd(counter, 1u);
+#else
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+            __hip_atomic_store(partials + (long long)d * nb + blockIdx.x, acc[d], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);  // all counters 0: the stores have completed
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+#endif
         *flag = (prev == (unsigned)(nb - 1));
     }
     __syncthreads();
     if (!*flag) return false;
+#ifdef SEDI_GRID_SUM_FENCE
     __threadfence();
+#endif
     double loc[ND];
 #pragma unroll
     for (int d = 0; d < ND; d++) {
@@ -117,19 +150,41 @@ __device__ bool grid_sum(double (&acc)[ND], double* partials, unsigned* counter,
     return true;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a workgroup
+// fence on all memory and waits for every outstanding global load before the
+// barrier, which would drain the prefetch ring once per plane; the march
+// communicates between waves through LDS only, so it needs just this.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Tile number handled by the k-th step of workgroup b's grid-stride loop.
+__device__ __forceinline__ long long tile_of(const TileGrid& g, long long t) {
+    if ((g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
+        const long long per = g.ntiles / NXCD;
+        return (t % NXCD) * per + t / NXCD;
+    }
+    return t;
+}
+
 // ---- the march -------------------------------------------------------------
 // Pass interface:
-//   struct Cell;                                   values loaded at one cell
-//   __device__ Cell load(long long idx) const;     loads what f needs
-//   __device__ double f(const Cell&) const;         the stencil input field
-//   __device__ void finish(long long idx, const Cell& c, double fc, double Af,
-//                          double (&acc)[ND]) const;   pointwise update at a cell
 //   static constexpr int ND;                        number of dot products
+//   static constexpr int D;                         prefetch distance (planes)
+//   struct Raw;   Raw load(long long idx) const;    what f needs at a cell
+//   double f(const Raw&) const;                     the stencil input field
+//   struct Aux;   Aux load_aux(long long idx) const;  other inputs of the update
+//   void finish(long long idx, const Raw&, const Aux&, double f, double Af,
+//               double (&acc)[ND]) const;           pointwise update at a cell
 // and the stencil coefficients a, b, c.
 template <class P>
 __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long long tile,
                                            double (&lds)[2][TY + 2][TX + 2],
                                            double (&acc)[P::ND]) {
+    constexpr int D = P::D;
+    constexpr int R = D + 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long t_xy = (long long)g.nbx * g.nby;
     const int bz = (int)(tile / t_xy);
@@ -139,13 +194,17 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     const int i0 = bx * TX, j0 = by * TY;
     const int zs = bz * g.zc;
     const int ze = min(zs + g.zc, g.n3);
+    const int n3 = g.n3;
 
     const int i = i0 + lane, j = j0 + w;
     const bool c_in = (i < g.n1) && (j < g.n2);
-    const long long cxy = (long long)i + (long long)j * g.s2;
+    // every load below is unconditional from a clamped, in-grid address and
+    // out-of-grid values are replaced by 0 afterwards: the body is straight-line
+    // code, so the compiler's wait counts on the prefetch ring stay exact
+    const long long cxy = (long long)min(i, g.n1 - 1) + (long long)min(j, g.n2 - 1) * g.s2;
 
     // the one extra (halo) cell this lane evaluates per plane
-    int ex_i = 0, ex_j = 0, ex_r = 0, ex_c = 0;
+    int ex_i = i, ex_j = j, ex_r = 0, ex_c = 0;
     bool ex_act = false;
     if (lane < 32) {
         ex_j = (w < 2) ? j0 - 1 : j0 + TY;
@@ -154,47 +213,70 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
         ex_c = 1 + (w & 1) * 32 + lane;
         ex_act = true;
     } else if (lane == 32) {
-        ex_i = i0 - 1; ex_j = j; ex_r = w + 1; ex_c = 0; ex_act = true;
+        ex_i = i0 - 1; ex_r = w + 1; ex_c = 0; ex_act = true;
     } else if (lane == 33) {
-        ex_i = i0 + TX; ex_j = j; ex_r = w + 1; ex_c = TX + 1; ex_act = true;
+        ex_i = i0 + TX; ex_r = w + 1; ex_c = TX + 1; ex_act = true;
     }
     const bool ex_in = ex_act && ex_i >= 0 && ex_i < g.n1 && ex_j >= 0 && ex_j < g.n2;
-    const long long exy = (long long)ex_i + (long long)ex_j * g.s2;
+    const long long exy = (long long)min(max(ex_i, 0), g.n1 - 1) + (long long)min(max(ex_j, 0), g.n2 - 1) * g.s2;
+    if (!ex_act) { ex_r = w + 1; ex_c = lane + 1; }  // harmless duplicate of the centre slot
 
-    typename P::Cell cm, cc, cn;
-    double fm = 0.0, fc = 0.0, fn = 0.0;
-    if (c_in && zs - 1 >= 0) { cm = p.load(cxy + (long long)(zs - 1) * g.s3); fm = p.f(cm); }
-    if (c_in) { cc = p.load(cxy + (long long)zs * g.s3); fc = p.f(cc); }
+    // rings: f-inputs of plane zz live in slot (zz - zs + 1) % R, update
+    // inputs of plane zz in slot (zz - zs) % R
+    typename P::Raw rc[R], rx[R];
+    typename P::Aux ra[R];
+    auto zoff = [&](int zz) { return (long long)min(max(zz, 0), n3 - 1) * g.s3; };
+    auto ld_f = [&](int zz, int slot) {
+        rc[slot] = p.load(cxy + zoff(zz));
+        rx[slot] = p.load(exy + zoff(zz));
+    };
+    auto ld_a = [&](int zz, int slot) { ra[slot] = p.load_aux(cxy + zoff(zz)); };
+
+    // prologue: planes zs-1 and zs now, zs+1 .. zs+D in flight, update inputs
+    // of planes zs .. zs+D-1 in flight
+    ld_f(zs - 1, 0);
+    ld_f(zs, 1 % R);
+#pragma unroll
+    for (int k = 1; k < D; k++) ld_f(zs + k, (k + 1) % R);
+#pragma unroll
+    for (int k = 0; k < D; k++) ld_a(zs + k, k);
+    double fm = (c_in && zs >= 1) ? p.f(rc[0]) : 0.0;
+    double fc = c_in ? p.f(rc[1 % R]) : 0.0;
+    typename P::Raw cc = rc[1 % R];
+    const double fe0 = ex_in ? p.f(rx[1 % R]) : 0.0;
     lds[0][w + 1][lane + 1] = fc;
-    if (ex_act) {
-        double fe = 0.0;
-        if (ex_in) fe = p.f(p.load(exy + (long long)zs * g.s3));
-        lds[0][ex_r][ex_c] = fe;
-    }
-    __syncthreads();
+    if (ex_act) lds[0][ex_r][ex_c] = fe0;
+    ld_f(zs + D, (D + 1) % R);
+    lds_barrier();
 
-    for (int z = zs; z < ze; z++) {
-        const int b = (z - zs) & 1;
-        const bool has_next = z + 1 < g.n3;
-        fn = 0.0;
-        if (c_in && has_next) { cn = p.load(cxy + (long long)(z + 1) * g.s3); fn = p.f(cn); }
-        if (z + 1 < ze) {
+    for (int zb = zs; zb < ze; zb += R) {
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            const int z = zb + u;
+            if (z >= ze) break;
+            const int b = (z - zs) & 1;
+            // prefetch plane z+1+D (f-inputs) and plane z+D (update inputs)
+            ld_f(z + 1 + D, (u + 1) % R);
+            ld_a(z + D, (u + D) % R);
+            // f on plane z+1 (centre and halo) into the other LDS buffer
+            const bool nxt = z + 1 < n3;
+            const typename P::Raw cn = rc[(u + 2) % R];
+            const double fn = (c_in && nxt) ? p.f(cn) : 0.0;
+            const double fe = (ex_in && nxt) ? p.f(rx[(u + 2) % R]) : 0.0;
             lds[b ^ 1][w + 1][lane + 1] = fn;
-            if (ex_act) {
-                double fe = 0.0;
-                if (ex_in) fe = p.f(p.load(exy + (long long)(z + 1) * g.s3));
-                lds[b ^ 1][ex_r][ex_c] = fe;
-            }
+            if (ex_act) lds[b ^ 1][ex_r][ex_c] = fe;
+            const double xm = lds[b][w + 1][lane];
+            const double xp = lds[b][w + 1][lane + 2];
+            const double ym = lds[b][w][lane + 1];
+            const double yp = lds[b][w + 2][lane + 1];
+            const double Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
+                              - p.b * xp - p.b * yp - p.b * fn;
+            if (c_in) p.finish(cxy + (long long)z * g.s3, cc, ra[u], fc, Af, acc);
+            lds_barrier();
+            fm = fc;
+            fc = fn;
+            cc = cn;
         }
-        const double xm = lds[b][w + 1][lane];
-        const double xp = lds[b][w + 1][lane + 2];
-        const double ym = lds[b][w][lane + 1];
-        const double yp = lds[b][w + 2][lane + 1];
-        const double Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
-                          - p.b * xp - p.b * yp - p.b * fn;
-        if (c_in) p.finish(cxy + (long long)z * g.s3, cc, fc, Af, acc);
-        __syncthreads();
-        fm = fc; fc = fn; cc = cn;
     }
 }
 

@@ -47,7 +47,8 @@ class PlainSolver {
     const Shape& shape() const { return shape_; }
 
   private:
-    int launch_iterations(int count, Precond pc, const double* b, double* x);
+    template <class PC> int launch_setup(PC pc, const double* b, double* x);
+    template <class PC> int launch_iterations(int count, PC pc, double* x);
     Shape shape_{};
     double a_ = 6.0, b_ = 1.0, c_ = 1.0;
     int kind_ = 2, mode_ = 0;

@@ -65,14 +65,16 @@ __global__ void init_state_kernel(DevState* st, int kind, int maxiter, double to
 // ---------------------------------------------------------------------------
 // y = A x (Stencil.hpp:38-54).
 struct PassApply {
-    static constexpr int ND = 1;
+    static constexpr int ND = 1, D = 2;
     double a, b, c;
     const double* x;
     double* y;
-    struct Cell { double x; };
-    __device__ Cell load(long long i) const { return {x[i]}; }
-    __device__ double f(const Cell& c) const { return c.x; }
-    __device__ void finish(long long i, const Cell&, double, double Af, double (&)[ND]) const {
+    struct Raw { double x; };
+    struct Aux {};
+    __device__ Raw load(long long i) const { return {x[i]}; }
+    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ double f(const Raw& r) const { return r.x; }
+    __device__ void finish(long long i, const Raw&, const Aux&, double, double Af, double (&)[ND]) const {
         y[i] = Af;
     }
 };
@@ -80,47 +82,51 @@ struct PassApply {
 // r = b - A x  (Cg.hpp:119-123; PipelinedCg.hpp:113-117; BiCgStab.hpp:131-135)
 // with the dots <b,b>, <r,r> and <r, r.*iM>; optionally r_hat = r
 // (BiCgStab.hpp:154).
+template <class PC>
 struct PassResidual {
-    static constexpr int ND = 3;
+    static constexpr int ND = 3, D = 2;
     double a, b, c;
     const double* x;
     const double* rhs;
     double* r;
     double* rhat;
-    Precond iM;
-    struct Cell { double x; };
-    __device__ Cell load(long long i) const { return {x[i]}; }
-    __device__ double f(const Cell& c) const { return c.x; }
-    __device__ void finish(long long i, const Cell&, double, double Af, double (&acc)[ND]) const {
-        const double bv = rhs[i];
-        const double rv = 1.0 * bv + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
+    PC iM;
+    struct Raw { double x; };
+    struct Aux { double b, im; };
+    __device__ Raw load(long long i) const { return {x[i]}; }
+    __device__ Aux load_aux(long long i) const { return {rhs[i], iM.at(i)}; }
+    __device__ double f(const Raw& r) const { return r.x; }
+    __device__ void finish(long long i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
+        const double rv = 1.0 * q.b + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
         r[i] = rv;
         if (rhat) rhat[i] = rv;
-        acc[0] += bv * bv;
+        acc[0] += q.b * q.b;
         acc[1] += rv * rv;
-        acc[2] += rv * (rv * iM.at(i));
+        acc[2] += rv * (rv * q.im);
     }
 };
 
 // CG direction: p = u + beta*p with u = r.*iM (Cg.hpp:175,188; p = u at the
 // first iteration, :144-147), then s = A p and <s,p> (:152-165).
+template <class PC>
 struct PassCgDirection {
-    static constexpr int ND = 1;
+    static constexpr int ND = 1, D = 2;
     double a, b, c;
     const double* r;
     const double* pold;
     double* pnew;
-    Precond iM;
+    PC iM;
     double beta;
     bool first;
-    struct Cell { double p; };
-    __device__ Cell load(long long i) const {
-        const double u = r[i] * iM.at(i);
-        if (first) return {u};
-        return {1.0 * u + beta * pold[i]};
+    struct Raw { double r, p, im; };
+    struct Aux {};
+    __device__ Raw load(long long i) const { return {r[i], pold[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ double f(const Raw& q) const {
+        const double u = q.r * q.im;
+        return first ? u : 1.0 * u + beta * q.p;
     }
-    __device__ double f(const Cell& c) const { return c.p; }
-    __device__ void finish(long long i, const Cell&, double p, double s, double (&acc)[ND]) const {
+    __device__ void finish(long long i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
         pnew[i] = p;
         acc[0] += s * p;
     }
@@ -128,23 +134,26 @@ struct PassCgDirection {
 
 // CG update: s = A p recomputed; x += alpha p; r -= alpha s; u = r.*iM;
 // <r,u>, <r,r> (Cg.hpp:168-182).
+template <class PC>
 struct PassCgUpdate {
-    static constexpr int ND = 2;
+    static constexpr int ND = 2, D = 2;
     double a, b, c;
     const double* p;
     double* x;
     double* r;
-    Precond iM;
+    PC iM;
     double alpha;
-    struct Cell { double p; };
-    __device__ Cell load(long long i) const { return {p[i]}; }
-    __device__ double f(const Cell& c) const { return c.p; }
-    __device__ void finish(long long i, const Cell&, double pv, double s, double (&acc)[ND]) const {
-        const double xv = 1.0 * x[i] + alpha * pv;
-        const double rv = 1.0 * r[i] + (-alpha) * s;
+    struct Raw { double p; };
+    struct Aux { double x, r, im; };
+    __device__ Raw load(long long i) const { return {p[i]}; }
+    __device__ Aux load_aux(long long i) const { return {x[i], r[i], iM.at(i)}; }
+    __device__ double f(const Raw& q) const { return q.p; }
+    __device__ void finish(long long i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
+        const double xv = 1.0 * q.x + alpha * pv;
+        const double rv = 1.0 * q.r + (-alpha) * s;
         x[i] = xv;
         r[i] = rv;
-        const double u = rv * iM.at(i);
+        const double u = rv * q.im;
         acc[0] += rv * u;
         acc[1] += rv * rv;
     }
@@ -152,21 +161,21 @@ struct PassCgUpdate {
 
 // Pipelined CG set-up: u = r.*iM, w = A u (PipelinedCg.hpp:138-143), z=q=s=p=0
 // (:146-149), and the first iteration's <r,u>, <w,u>, <r,r> (:155,170).
+template <class PC>
 struct PassPcgInit {
-    static constexpr int ND = 3;
+    static constexpr int ND = 3, D = 2;
     double a, b, c;
     const double* r;
     double* u;
     double* w;
     double *z, *q, *s, *p;
-    Precond iM;
-    struct Cell { double r, u; };
-    __device__ Cell load(long long i) const {
-        const double rv = r[i];
-        return {rv, rv * iM.at(i)};
-    }
-    __device__ double f(const Cell& c) const { return c.u; }
-    __device__ void finish(long long i, const Cell& c, double uv, double wv, double (&acc)[ND]) const {
+    PC iM;
+    struct Raw { double r, im; };
+    struct Aux {};
+    __device__ Raw load(long long i) const { return {r[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ double f(const Raw& c) const { return c.r * c.im; }
+    __device__ void finish(long long i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
         u[i] = uv;
         w[i] = wv;
         z[i] = 0.0; q[i] = 0.0; s[i] = 0.0; p[i] = 0.0;
@@ -178,29 +187,28 @@ struct PassPcgInit {
 
 // Pipelined CG iteration (PipelinedCg.hpp:157-223): m = w.*iM, n = A m, the
 // eight recurrences in the reference's order, and the next <r,u>, <w,u>, <r,r>.
+template <class PC>
 struct PassPcgIter {
-    static constexpr int ND = 3;
+    static constexpr int ND = 3, D = 1;
     double a, b, c;
     const double* wold;
     double* wnew;
     double *x, *r, *u, *z, *q, *s, *p;
-    Precond iM;
+    PC iM;
     double alpha, beta;
-    struct Cell { double w, m; };
-    __device__ Cell load(long long i) const {
-        const double wv = wold[i];
-        return {wv, wv * iM.at(i)};
-    }
-    __device__ double f(const Cell& c) const { return c.m; }
-    __device__ void finish(long long i, const Cell& c, double m, double n, double (&acc)[ND]) const {
-        const double qv = 1.0 * m + beta * q[i];
-        const double sv = 1.0 * c.w + beta * s[i];
-        const double uo = u[i];
-        const double pv = 1.0 * uo + beta * p[i];
-        const double zv = 1.0 * n + beta * z[i];
-        const double rv = 1.0 * r[i] + (-alpha) * sv;
-        const double xv = 1.0 * x[i] + alpha * pv;
-        const double uv = 1.0 * uo + (-alpha) * qv;
+    struct Raw { double w, im; };
+    struct Aux { double q, s, u, p, z, r, x; };
+    __device__ Raw load(long long i) const { return {wold[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long i) const { return {q[i], s[i], u[i], p[i], z[i], r[i], x[i]}; }
+    __device__ double f(const Raw& c) const { return c.w * c.im; }
+    __device__ void finish(long long i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
+        const double qv = 1.0 * m + beta * o.q;
+        const double sv = 1.0 * c.w + beta * o.s;
+        const double pv = 1.0 * o.u + beta * o.p;
+        const double zv = 1.0 * n + beta * o.z;
+        const double rv = 1.0 * o.r + (-alpha) * sv;
+        const double xv = 1.0 * o.x + alpha * pv;
+        const double uv = 1.0 * o.u + (-alpha) * qv;
         const double wv = 1.0 * c.w + (-alpha) * zv;
         q[i] = qv; s[i] = sv; p[i] = pv; z[i] = zv;
         r[i] = rv; x[i] = xv; u[i] = uv; wnew[i] = wv;
@@ -212,47 +220,49 @@ struct PassPcgIter {
 
 // BiCGStab direction (BiCgStab.hpp:165-187): p = r + beta*p - beta*omega*v
 // (p = r at the first iteration), p^ = p.*iM, v = A p^, <v, r^>.
+template <class PC>
 struct PassBicgDirection {
-    static constexpr int ND = 1;
+    static constexpr int ND = 1, D = 2;
     double a, b, c;
     const double *r, *pold, *vold, *rhat;
     double *pnew, *vnew;
-    Precond iM;
+    PC iM;
     double beta, cbw;
     bool first;
-    struct Cell { double p, ph; };
-    __device__ Cell load(long long i) const {
-        const double rv = r[i];
-        double pv;
-        if (first) pv = rv;
-        else pv = 1.0 * rv + beta * pold[i] + cbw * vold[i];  // combine3
-        return {pv, pv * iM.at(i)};
+    struct Raw { double r, p, v, im; };
+    struct Aux { double rhat; };
+    __device__ Raw load(long long i) const { return {r[i], pold[i], vold[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long i) const { return {rhat[i]}; }
+    __device__ double pdir(const Raw& q) const {
+        return first ? q.r : 1.0 * q.r + beta * q.p + cbw * q.v;  // combine3 (:169)
     }
-    __device__ double f(const Cell& c) const { return c.ph; }
-    __device__ void finish(long long i, const Cell& c, double, double v, double (&acc)[ND]) const {
-        pnew[i] = c.p;
+    __device__ double f(const Raw& q) const { return pdir(q) * q.im; }
+    __device__ void finish(long long i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
+        pnew[i] = pdir(q);
         vnew[i] = v;
-        acc[0] += v * rhat[i];
+        acc[0] += v * o.rhat;
     }
 };
 
 // BiCGStab half step (BiCgStab.hpp:190-213): s = r - alpha v, s^ = s.*iM,
 // t = A s^, <s,s>, <t,s>, <t,t>; s and t are not stored.
+template <class PC>
 struct PassBicgHalf {
-    static constexpr int ND = 3;
+    static constexpr int ND = 3, D = 2;
     double a, b, c;
     const double *r, *v;
-    Precond iM;
+    PC iM;
     double alpha;
-    struct Cell { double s, sh; };
-    __device__ Cell load(long long i) const {
-        const double sv = 1.0 * r[i] + (-alpha) * v[i];
-        return {sv, sv * iM.at(i)};
-    }
-    __device__ double f(const Cell& c) const { return c.sh; }
-    __device__ void finish(long long, const Cell& c, double, double t, double (&acc)[ND]) const {
-        acc[0] += c.s * c.s;
-        acc[1] += t * c.s;
+    struct Raw { double r, v, im; };
+    struct Aux {};
+    __device__ Raw load(long long i) const { return {r[i], v[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
+    __device__ double f(const Raw& q) const { return sval(q) * q.im; }
+    __device__ void finish(long long, const Raw& q, const Aux&, double, double t, double (&acc)[ND]) const {
+        const double sv = sval(q);
+        acc[0] += sv * sv;
+        acc[1] += t * sv;
         acc[2] += t * t;
     }
 };
@@ -260,27 +270,29 @@ struct PassBicgHalf {
 // BiCGStab update (BiCgStab.hpp:217-224 and :160 of the next iteration):
 // s, s^, t recomputed; x = x + alpha p^ + omega s^; r = s - omega t;
 // <r,r> and the next rho = <r, r^>.
+template <class PC>
 struct PassBicgUpdate {
-    static constexpr int ND = 2;
+    static constexpr int ND = 2, D = 2;
     double a, b, c;
     const double *r, *v, *p, *rhat;
     double *x, *rnew;
-    Precond iM;
+    PC iM;
     double alpha, omega;
-    struct Cell { double s, sh; };
-    __device__ Cell load(long long i) const {
-        const double sv = 1.0 * r[i] + (-alpha) * v[i];
-        return {sv, sv * iM.at(i)};
-    }
-    __device__ double f(const Cell& c) const { return c.sh; }
-    __device__ void finish(long long i, const Cell& c, double, double t, double (&acc)[ND]) const {
-        const double ph = p[i] * iM.at(i);
-        const double xv = 1.0 * x[i] + alpha * ph + omega * c.sh;  // combine3
-        const double rv = 1.0 * c.s + (-omega) * t;
+    struct Raw { double r, v, im; };
+    struct Aux { double p, x, rhat; };
+    __device__ Raw load(long long i) const { return {r[i], v[i], iM.at(i)}; }
+    __device__ Aux load_aux(long long i) const { return {p[i], x[i], rhat[i]}; }
+    __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
+    __device__ double f(const Raw& q) const { return sval(q) * q.im; }
+    __device__ void finish(long long i, const Raw& q, const Aux& o, double sh, double t, double (&acc)[ND]) const {
+        const double sv = sval(q);
+        const double ph = o.p * q.im;
+        const double xv = 1.0 * o.x + alpha * ph + omega * sh;  // combine3 (:217)
+        const double rv = 1.0 * sv + (-omega) * t;
         x[i] = xv;
         rnew[i] = rv;
         acc[0] += rv * rv;
-        acc[1] += rv * rhat[i];
+        acc[1] += rv * o.rhat;
     }
 };
 
@@ -298,7 +310,7 @@ __device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
                                           double (&acc)[P::ND]) {
 #pragma unroll
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
-    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x) march_tile(p, g, t, lds, acc);
+    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x) march_tile(p, g, tile_of(g, t), lds, acc);
 }
 
 __global__ __launch_bounds__(NTHREADS) void stencil_apply_kernel(TileGrid g, double a, double b,
@@ -310,12 +322,13 @@ __global__ __launch_bounds__(NTHREADS) void stencil_apply_kernel(TileGrid g, dou
     run_tiles(P, g, lds, acc);
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* x,
                                                             const double* rhs, double* r,
-                                                            double* rhat, Precond iM) {
+                                                            double* rhat, PC iM) {
     PASS_SHARED;
-    PassResidual P{a, b, c, x, rhs, r, rhat, iM};
+    PassResidual<PC> P{a, b, c, x, rhs, r, rhat, iM};
     double acc[3];
     run_tiles(P, g, lds, acc);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -347,14 +360,15 @@ __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl,
     if (st->maxiter < 1) cg_outcome_exhausted(st);
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
                                                                 double b, double c, const double* r,
-                                                                double* p0, double* p1, Precond iM) {
+                                                                double* p0, double* p1, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    PassCgDirection P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
+    PassCgDirection<PC> P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
     double acc[1];
     run_tiles(P, g, lds, acc);
     if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -362,15 +376,16 @@ __global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl 
     st->alpha = st->ru / acc[0];  // Cg.hpp:166
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                              double c, const double* p0,
                                                              const double* p1, double* x, double* r,
-                                                             Precond iM) {
+                                                             PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    PassCgUpdate P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
+    PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
     double acc[2];
     run_tiles(P, g, lds, acc);
     if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -387,14 +402,15 @@ __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl
     else st->iter = it + 1;
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* r, double* u,
                                                             double* w, double* z, double* q,
-                                                            double* s, double* p, Precond iM) {
+                                                            double* s, double* p, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
-    PassPcgInit P{a, b, c, r, u, w, z, q, s, p, iM};
+    PassPcgInit<PC> P{a, b, c, r, u, w, z, q, s, p, iM};
     double acc[3];
     run_tiles(P, g, lds, acc);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -407,17 +423,18 @@ __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl,
     st->iter = 1;
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, double* w0, double* w1,
                                                             double* x, double* r, double* u,
                                                             double* z, double* q, double* s,
-                                                            double* p, Precond iM) {
+                                                            double* p, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
     // iteration 1 reads w from w0 (written by pcg_init_kernel)
-    PassPcgIter P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
+    PassPcgIter<PC> P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
                   cst->alpha, cst->beta};
     double acc[3];
     run_tiles(P, g, lds, acc);
@@ -442,15 +459,16 @@ __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl,
     st->iter = it + 1;
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
-    TileGrid g, Ctl ctl, double a, double b, double c, double* const* vv, Precond iM) {
+    TileGrid g, Ctl ctl, double a, double b, double c, double* const* vv, PC iM) {
     // vv: r0, r1, rhat, p0, p1, v0, v1
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
     const int cur = it & 1, prv = cur ^ 1;
-    PassBicgDirection P{a, b, c, vv[0 + cur], vv[3 + prv], vv[5 + prv], vv[2],
+    PassBicgDirection<PC> P{a, b, c, vv[0 + cur], vv[3 + prv], vv[5 + prv], vv[2],
                         vv[3 + cur], vv[5 + cur], iM, cst->beta, cst->cbw, it == 1};
     double acc[1];
     run_tiles(P, g, lds, acc);
@@ -459,14 +477,15 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
     st->alpha = st->rho / acc[0];  // BiCgStab.hpp:187
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl ctl, double a,
                                                                  double b, double c,
-                                                                 double* const* vv, Precond iM) {
+                                                                 double* const* vv, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int cur = cst->iter & 1;
-    PassBicgHalf P{a, b, c, vv[0 + cur], vv[5 + cur], iM, cst->alpha};
+    PassBicgHalf<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], iM, cst->alpha};
     double acc[3];
     run_tiles(P, g, lds, acc);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -476,10 +495,11 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl
     else st->omega = acc[1] / acc[2];              // :213
 }
 
+template <class PC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, Ctl ctl, double a,
                                                                    double b, double c,
                                                                    double* const* vv, double* x,
-                                                                   Precond iM) {
+                                                                   PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
@@ -501,7 +521,7 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
         bicgstab_outcome(ctl.st, it);
         return;
     }
-    PassBicgUpdate P{a, b, c, vv[0 + cur], vv[5 + cur], vv[3 + cur], vv[2],
+    PassBicgUpdate<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], vv[3 + cur], vv[2],
                      x, vv[0 + (cur ^ 1)], iM, cst->alpha, cst->omega};
     run_tiles(P, g, lds, acc);
     if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -593,28 +613,48 @@ int PlainSolver::create(const Shape& s, double a, double b, double c, int kind, 
     return SEDI_OK;
 }
 
-int PlainSolver::launch_iterations(int count, Precond pc, const double* b, double* x) {
-    (void)b;
+template <class PC>
+int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
+    DevState* st = reinterpret_cast<DevState*>(ctl_.p);
+    Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
+            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
+    const dim3 grid(nblocks_), block(NTHREADS);
+    if (kind_ == SEDI_BICGSTAB) {
+        // r_1 lives in r[1] (buffers alternate with the iteration's parity)
+        SEDI_HIP(launch("residual_kernel", residual_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)x, b, v_[1], v_[2], pc));
+    } else {
+        SEDI_HIP(launch("residual_kernel", residual_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
+    }
+    if (kind_ == SEDI_PCG)
+        SEDI_HIP(launch("pcg_init_kernel", pcg_init_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
+                        c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
+    return SEDI_OK;
+}
+
+template <class PC>
+int PlainSolver::launch_iterations(int count, PC pc, double* x) {
     Ctl ctl{reinterpret_cast<DevState*>(ctl_.p),
             reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
     const dim3 grid(nblocks_), block(NTHREADS);
     for (int k = 0; k < count; k++) {
         if (kind_ == SEDI_CG) {
-            SEDI_HIP(launch("cg_direction_kernel", cg_direction_kernel, grid, block, stream_, tg_, ctl,
+            SEDI_HIP(launch("cg_direction_kernel", cg_direction_kernel<PC>, grid, block, stream_, tg_, ctl,
                             a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
-            SEDI_HIP(launch("cg_update_kernel", cg_update_kernel, grid, block, stream_, tg_, ctl, a_,
+            SEDI_HIP(launch("cg_update_kernel", cg_update_kernel<PC>, grid, block, stream_, tg_, ctl, a_,
                             b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
         } else if (kind_ == SEDI_PCG) {
-            SEDI_HIP(launch("pcg_iter_kernel", pcg_iter_kernel, grid, block, stream_, tg_, ctl, a_, b_,
+            SEDI_HIP(launch("pcg_iter_kernel", pcg_iter_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
                             c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
         } else {
             double* const* vv = reinterpret_cast<double* const*>(static_cast<char*>(ctl_.p) + 384);
-            SEDI_HIP(launch("bicgstab_direction_kernel", bicgstab_direction_kernel, grid, block,
+            SEDI_HIP(launch("bicgstab_direction_kernel", bicgstab_direction_kernel<PC>, grid, block,
                             stream_, tg_, ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(launch("bicgstab_half_kernel", bicgstab_half_kernel, grid, block, stream_, tg_,
+            SEDI_HIP(launch("bicgstab_half_kernel", bicgstab_half_kernel<PC>, grid, block, stream_, tg_,
                             ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(launch("bicgstab_update_kernel", bicgstab_update_kernel, grid, block, stream_,
+            SEDI_HIP(launch("bicgstab_update_kernel", bicgstab_update_kernel<PC>, grid, block, stream_,
                             tg_, ctl, a_, b_, c_, vv, x, pc));
         }
     }
@@ -626,24 +666,13 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
     SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
     SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)std::max(maxiter, 1)));
     DevState* st = reinterpret_cast<DevState*>(ctl_.p);
-    Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
-            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const Precond pc{iM, 1.0 / a_};
-    const dim3 grid(nblocks_), block(NTHREADS);
+    const PrecondScalar pcs{1.0 / a_};  // stochastic-estimator.cc:103
+    const PrecondVec pcv{iM};
 
     SEDI_HIP(launch("init_state_kernel", init_state_kernel, dim3(1), dim3(64), stream_, st, kind_,
                     maxiter, tol));
-    if (kind_ == SEDI_BICGSTAB) {
-        // r_1 lives in r[1] (buffers alternate with the iteration's parity)
-        SEDI_HIP(launch("residual_kernel", residual_kernel, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)x, b, v_[1], v_[2], pc));
-    } else {
-        SEDI_HIP(launch("residual_kernel", residual_kernel, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
-    }
-    if (kind_ == SEDI_PCG)
-        SEDI_HIP(launch("pcg_init_kernel", pcg_init_kernel, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
+    int rc = iM ? launch_setup(pcv, b, x) : launch_setup(pcs, b, x);
+    if (rc) return rc;
 
     // Iterations go out in chunks; the host looks at the device's done flag
     // of the previous chunk while the next one runs, so the GPU never idles
@@ -653,7 +682,7 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
     bool pending = false;
     while (launched < maxiter) {
         const int cnt = std::min(chunk, maxiter - launched);
-        int rc = launch_iterations(cnt, pc, b, x);
+        rc = iM ? launch_iterations(cnt, pcv, x) : launch_iterations(cnt, pcs, x);
         if (rc) return rc;
         launched += cnt;
         SEDI_HIP(hipMemcpyAsync(&pinned_[slot], st, sizeof(DevState), hipMemcpyDeviceToHost, stream_));

@@ -38,12 +38,18 @@ struct Shape {
     long long n() const { return (long long)n1 * n2 * n3; }
 };
 
-// Scalar or vector inverse diagonal (the reference's precond vector,
-// stochastic-estimator.cc:102-103, is the constant 1/a).
-struct Precond {
-    const double* v;
+// Inverse diagonal of the Jacobi preconditioner.  The estimator's is the
+// constant 1/a (precond.all_cells.fill(1.0/stencil.a),
+// stochastic-estimator.cc:102-103), which kernels instantiated with
+// PrecondScalar keep in a scalar register; sedi_solve with an explicit iM
+// vector uses PrecondVec.
+struct PrecondScalar {
     double s;
-    __device__ __forceinline__ double at(long long idx) const { return v ? v[idx] : s; }
+    __device__ __forceinline__ double at(long long) const { return s; }
+};
+struct PrecondVec {
+    const double* v;
+    __device__ __forceinline__ double at(long long idx) const { return v[idx]; }
 };
 
 }  // namespace sedi
