@@ -38,6 +38,8 @@
 
 #include "sedi_internal.h"
 
+#include <algorithm>
+
 namespace sedi {
 
 constexpr int TX = 64;
@@ -63,6 +65,38 @@ inline TileGrid make_tile_grid(const Shape& s, int zc) {
     t.s3 = (long long)s.n1 * s.n2;
     t.ntiles = (long long)t.nbx * t.nby * t.nbz;
     return t;
+}
+
+// Launch geometry of one kernel on one grid: tiles and workgroup count.
+struct KGeom {
+    TileGrid tg;
+    int nblocks;
+};
+
+// Chooses the z chunk so that the tiles fill the workgroups that are resident
+// at once (`resident` = occupancy x CUs): the makespan of a pass is about
+// rounds * (chunk + prologue) plane steps of one workgroup, where a round is
+// one tile per resident workgroup and the prologue costs ~2 + D planes.
+// The march is compiled for chunks of 8, 16 and 32 planes.
+constexpr int ZC_CHOICES[3] = {8, 16, 32};
+
+inline KGeom plan_geometry(const Shape& s, int resident, int D) {
+    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + TY - 1) / TY);
+    long long best_cost = -1;
+    int best_zc = 16;
+    for (int zc : ZC_CHOICES) {
+        const long long nt = nxy * ((s.n3 + zc - 1) / zc);
+        const long long rounds = (nt + resident - 1) / resident;
+        const long long cost = rounds * (zc + 2 + D);
+        if (best_cost < 0 || cost < best_cost || (cost == best_cost && zc > best_zc)) {
+            best_cost = cost;
+            best_zc = zc;
+        }
+    }
+    KGeom k;
+    k.tg = make_tile_grid(s, best_zc);
+    k.nblocks = (int)std::min<long long>(k.tg.ntiles, resident);
+    return k;
 }
 
 // ---- wave / block reductions (fixed order => run-to-run deterministic) ----
@@ -160,26 +194,53 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Tile number handled by the k-th step of workgroup b's grid-stride loop.
-__device__ __forceinline__ long long tile_of(const TileGrid& g, long long t) {
+// Tile handled by step t of the grid-stride loop.  `reverse` walks each
+// XCD's run of tiles backwards: consecutive passes alternate direction, so a
+// pass starts on the tiles the previous pass touched last, whose vectors are
+// still in the 256 MB Infinity Cache.
+__device__ __forceinline__ long long tile_of(const TileGrid& g, long long t, bool reverse) {
     if ((g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
         const long long per = g.ntiles / NXCD;
-        return (t % NXCD) * per + t / NXCD;
+        const long long k = t / NXCD;
+        return (t % NXCD) * per + (reverse ? per - 1 - k : k);
     }
-    return t;
+    return reverse ? g.ntiles - 1 - t : t;
+}
+
+// ---- byte-offset memory access ---------------------------------------------
+// Vectors of the plain path hold fewer than 2^29 cells (checked on the host),
+// so a cell's byte offset fits in 32 bits; loads then take the vector's base
+// from scalar registers plus one 32-bit offset register, with no 64-bit
+// address arithmetic per access.
+using boff_t = unsigned;
+
+template <class T>
+__device__ __forceinline__ T ldb(const T* base, boff_t bo) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bo);
+}
+template <class T>
+__device__ __forceinline__ void stb(T* base, boff_t bo, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + bo) = v;
 }
 
 // ---- the march -------------------------------------------------------------
-// Pass interface:
+// Pass interface (all offsets are byte offsets of FP64 cells):
 //   static constexpr int ND;                        number of dot products
 //   static constexpr int D;                         prefetch distance (planes)
-//   struct Raw;   Raw load(long long idx) const;    what f needs at a cell
+//   struct Raw;   Raw load(boff_t) const;           what f needs at a cell
 //   double f(const Raw&) const;                     the stencil input field
-//   struct Aux;   Aux load_aux(long long idx) const;  other inputs of the update
-//   void finish(long long idx, const Raw&, const Aux&, double f, double Af,
+//   struct Aux;   Aux load_aux(boff_t) const;       other inputs of the update
+//   void finish(boff_t, const Raw&, const Aux&, double f, double Af,
 //               double (&acc)[ND]) const;           pointwise update at a cell
 // and the stencil coefficients a, b, c.
-template <class P>
+//
+// ZC (planes per tile) is a compile-time constant and the march over the
+// tile's planes is fully unrolled: the prefetch ring's slots are then fixed
+// registers with no loop back-edge to reconcile, and the compiler's wait
+// counts keep D planes of loads in flight all the way through.  A tile that
+// runs past the last plane still executes every step (clamped loads, zeroed
+// values) but does not store.
+template <class P, int ZC>
 __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long long tile,
                                            double (&lds)[2][TY + 2][TX + 2],
                                            double (&acc)[P::ND]) {
@@ -192,16 +253,16 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     const int by = (int)(rem / g.nbx);
     const int bx = (int)(rem - (long long)by * g.nbx);
     const int i0 = bx * TX, j0 = by * TY;
-    const int zs = bz * g.zc;
-    const int ze = min(zs + g.zc, g.n3);
+    const int zs = bz * ZC;
     const int n3 = g.n3;
+    const boff_t s3b = (boff_t)(g.s3 * 8);
 
     const int i = i0 + lane, j = j0 + w;
     const bool c_in = (i < g.n1) && (j < g.n2);
     // every load below is unconditional from a clamped, in-grid address and
     // out-of-grid values are replaced by 0 afterwards: the body is straight-line
     // code, so the compiler's wait counts on the prefetch ring stay exact
-    const long long cxy = (long long)min(i, g.n1 - 1) + (long long)min(j, g.n2 - 1) * g.s2;
+    const boff_t cxy = (boff_t)(min(i, g.n1 - 1) + min(j, g.n2 - 1) * g.n1) * 8u;
 
     // the one extra (halo) cell this lane evaluates per plane
     int ex_i = i, ex_j = j, ex_r = 0, ex_c = 0;
@@ -218,14 +279,15 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
         ex_i = i0 + TX; ex_r = w + 1; ex_c = TX + 1; ex_act = true;
     }
     const bool ex_in = ex_act && ex_i >= 0 && ex_i < g.n1 && ex_j >= 0 && ex_j < g.n2;
-    const long long exy = (long long)min(max(ex_i, 0), g.n1 - 1) + (long long)min(max(ex_j, 0), g.n2 - 1) * g.s2;
+    const boff_t exy =
+        (boff_t)(min(max(ex_i, 0), g.n1 - 1) + min(max(ex_j, 0), g.n2 - 1) * g.n1) * 8u;
     if (!ex_act) { ex_r = w + 1; ex_c = lane + 1; }  // harmless duplicate of the centre slot
 
-    // rings: f-inputs of plane zz live in slot (zz - zs + 1) % R, update
-    // inputs of plane zz in slot (zz - zs) % R
+    // rings: f-inputs of plane zs+q-1 live in slot q % R, update inputs of
+    // plane zs+q in slot q % R
     typename P::Raw rc[R], rx[R];
     typename P::Aux ra[R];
-    auto zoff = [&](int zz) { return (long long)min(max(zz, 0), n3 - 1) * g.s3; };
+    auto zoff = [&](int zz) { return (boff_t)min(max(zz, 0), n3 - 1) * s3b; };
     auto ld_f = [&](int zz, int slot) {
         rc[slot] = p.load(cxy + zoff(zz));
         rx[slot] = p.load(exy + zoff(zz));
@@ -237,47 +299,57 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     ld_f(zs - 1, 0);
     ld_f(zs, 1 % R);
 #pragma unroll
-    for (int k = 1; k < D; k++) ld_f(zs + k, (k + 1) % R);
+    for (int k = 1; k < D && k <= ZC; k++) ld_f(zs + k, (k + 1) % R);
 #pragma unroll
-    for (int k = 0; k < D; k++) ld_a(zs + k, k);
+    for (int k = 0; k < D && k < ZC; k++) ld_a(zs + k, k);
     double fm = (c_in && zs >= 1) ? p.f(rc[0]) : 0.0;
-    double fc = c_in ? p.f(rc[1 % R]) : 0.0;
+    double fc = (c_in && zs < n3) ? p.f(rc[1 % R]) : 0.0;
     typename P::Raw cc = rc[1 % R];
-    const double fe0 = ex_in ? p.f(rx[1 % R]) : 0.0;
+    const double fe0 = (ex_in && zs < n3) ? p.f(rx[1 % R]) : 0.0;
     lds[0][w + 1][lane + 1] = fc;
     if (ex_act) lds[0][ex_r][ex_c] = fe0;
-    ld_f(zs + D, (D + 1) % R);
+    if (D <= ZC) ld_f(zs + D, (D + 1) % R);
     lds_barrier();
 
-    for (int zb = zs; zb < ze; zb += R) {
 #pragma unroll
-        for (int u = 0; u < R; u++) {
-            const int z = zb + u;
-            if (z >= ze) break;
-            const int b = (z - zs) & 1;
-            // prefetch plane z+1+D (f-inputs) and plane z+D (update inputs)
-            ld_f(z + 1 + D, (u + 1) % R);
-            ld_a(z + D, (u + D) % R);
-            // f on plane z+1 (centre and halo) into the other LDS buffer
-            const bool nxt = z + 1 < n3;
-            const typename P::Raw cn = rc[(u + 2) % R];
-            const double fn = (c_in && nxt) ? p.f(cn) : 0.0;
+    for (int u = 0; u < ZC; u++) {
+        const int z = zs + u;
+        const int b = u & 1;
+        // prefetch plane z+1+D (f-inputs) and plane z+D (update inputs)
+        if (u + 1 + D <= ZC) ld_f(z + 1 + D, (u + 1) % R);
+        if (u + D < ZC) ld_a(z + D, (u + D) % R);
+        // f on plane z+1 (centre and halo) into the other LDS buffer
+        const bool nxt = z + 1 < n3;
+        const typename P::Raw cn = rc[(u + 2) % R];
+        const double fn = (c_in && nxt) ? p.f(cn) : 0.0;
+        if (u + 1 < ZC) {
             const double fe = (ex_in && nxt) ? p.f(rx[(u + 2) % R]) : 0.0;
             lds[b ^ 1][w + 1][lane + 1] = fn;
             if (ex_act) lds[b ^ 1][ex_r][ex_c] = fe;
-            const double xm = lds[b][w + 1][lane];
-            const double xp = lds[b][w + 1][lane + 2];
-            const double ym = lds[b][w][lane + 1];
-            const double yp = lds[b][w + 2][lane + 1];
-            const double Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
-                              - p.b * xp - p.b * yp - p.b * fn;
-            if (c_in) p.finish(cxy + (long long)z * g.s3, cc, ra[u], fc, Af, acc);
-            lds_barrier();
-            fm = fc;
-            fc = fn;
-            cc = cn;
         }
+        const double xm = lds[b][w + 1][lane];
+        const double xp = lds[b][w + 1][lane + 2];
+        const double ym = lds[b][w][lane + 1];
+        const double yp = lds[b][w + 2][lane + 1];
+        const double Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
+                          - p.b * xp - p.b * yp - p.b * fn;
+        if (c_in && z < n3) p.finish(cxy + (boff_t)z * s3b, cc, ra[u % R], fc, Af, acc);
+        lds_barrier();
+        fm = fc;
+        fc = fn;
+        cc = cn;
     }
+}
+
+// Grid-stride loop over the tiles of a pass (tile order per tile_of).
+template <class P, int ZC>
+__device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
+                                          double (&lds)[2][TY + 2][TX + 2],
+                                          double (&acc)[P::ND], bool reverse = false) {
+#pragma unroll
+    for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
+    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+        march_tile<P, ZC>(p, g, tile_of(g, t, reverse), lds, acc);
 }
 
 }  // namespace sedi

@@ -52,8 +52,7 @@ class PlainSolver {
     Shape shape_{};
     double a_ = 6.0, b_ = 1.0, c_ = 1.0;
     int kind_ = 2, mode_ = 0;
-    TileGrid tg_{};
-    int nblocks_ = 0;
+    int nblocks_max_ = 0;  // partials are sized for this many workgroups
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     DevState* pinned_ = nullptr;

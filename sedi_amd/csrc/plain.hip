@@ -18,6 +18,23 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+// Prefetch distance (planes) of the march in each pass.
+#ifndef SEDI_PREFETCH_APPLY
+#define SEDI_PREFETCH_APPLY 2
+#endif
+#ifndef SEDI_PREFETCH_CG
+#define SEDI_PREFETCH_CG 2
+#endif
+#ifndef SEDI_PREFETCH_PCG
+#define SEDI_PREFETCH_PCG 1
+#endif
+#ifndef SEDI_PREFETCH_BICG
+#define SEDI_PREFETCH_BICG 2
+#endif
 
 namespace sedi {
 
@@ -65,17 +82,17 @@ __global__ void init_state_kernel(DevState* st, int kind, int maxiter, double to
 // ---------------------------------------------------------------------------
 // y = A x (Stencil.hpp:38-54).
 struct PassApply {
-    static constexpr int ND = 1, D = 2;
+    static constexpr int ND = 1, D = SEDI_PREFETCH_APPLY;
     double a, b, c;
     const double* x;
     double* y;
     struct Raw { double x; };
     struct Aux {};
-    __device__ Raw load(long long i) const { return {x[i]}; }
-    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ Raw load(boff_t i) const { return {ldb(x, i)}; }
+    __device__ Aux load_aux(boff_t) const { return {}; }
     __device__ double f(const Raw& r) const { return r.x; }
-    __device__ void finish(long long i, const Raw&, const Aux&, double, double Af, double (&)[ND]) const {
-        y[i] = Af;
+    __device__ void finish(boff_t i, const Raw&, const Aux&, double, double Af, double (&)[ND]) const {
+        stb(y, i, Af);
     }
 };
 
@@ -84,7 +101,7 @@ struct PassApply {
 // (BiCgStab.hpp:154).
 template <class PC>
 struct PassResidual {
-    static constexpr int ND = 3, D = 2;
+    static constexpr int ND = 3, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* x;
     const double* rhs;
@@ -93,13 +110,13 @@ struct PassResidual {
     PC iM;
     struct Raw { double x; };
     struct Aux { double b, im; };
-    __device__ Raw load(long long i) const { return {x[i]}; }
-    __device__ Aux load_aux(long long i) const { return {rhs[i], iM.at(i)}; }
+    __device__ Raw load(boff_t i) const { return {ldb(x, i)}; }
+    __device__ Aux load_aux(boff_t i) const { return {ldb(rhs, i), iM.at(i)}; }
     __device__ double f(const Raw& r) const { return r.x; }
-    __device__ void finish(long long i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
+    __device__ void finish(boff_t i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
         const double rv = 1.0 * q.b + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
-        r[i] = rv;
-        if (rhat) rhat[i] = rv;
+        stb(r, i, rv);
+        if (rhat) stb(rhat, i, rv);
         acc[0] += q.b * q.b;
         acc[1] += rv * rv;
         acc[2] += rv * (rv * q.im);
@@ -110,7 +127,7 @@ struct PassResidual {
 // first iteration, :144-147), then s = A p and <s,p> (:152-165).
 template <class PC>
 struct PassCgDirection {
-    static constexpr int ND = 1, D = 2;
+    static constexpr int ND = 1, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* r;
     const double* pold;
@@ -120,14 +137,14 @@ struct PassCgDirection {
     bool first;
     struct Raw { double r, p, im; };
     struct Aux {};
-    __device__ Raw load(long long i) const { return {r[i], pold[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(pold, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t) const { return {}; }
     __device__ double f(const Raw& q) const {
         const double u = q.r * q.im;
         return first ? u : 1.0 * u + beta * q.p;
     }
-    __device__ void finish(long long i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
-        pnew[i] = p;
+    __device__ void finish(boff_t i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
+        stb(pnew, i, p);
         acc[0] += s * p;
     }
 };
@@ -136,7 +153,7 @@ struct PassCgDirection {
 // <r,u>, <r,r> (Cg.hpp:168-182).
 template <class PC>
 struct PassCgUpdate {
-    static constexpr int ND = 2, D = 2;
+    static constexpr int ND = 2, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* p;
     double* x;
@@ -145,14 +162,14 @@ struct PassCgUpdate {
     double alpha;
     struct Raw { double p; };
     struct Aux { double x, r, im; };
-    __device__ Raw load(long long i) const { return {p[i]}; }
-    __device__ Aux load_aux(long long i) const { return {x[i], r[i], iM.at(i)}; }
+    __device__ Raw load(boff_t i) const { return {ldb(p, i)}; }
+    __device__ Aux load_aux(boff_t i) const { return {ldb(x, i), ldb(r, i), iM.at(i)}; }
     __device__ double f(const Raw& q) const { return q.p; }
-    __device__ void finish(long long i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
+    __device__ void finish(boff_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
         const double xv = 1.0 * q.x + alpha * pv;
         const double rv = 1.0 * q.r + (-alpha) * s;
-        x[i] = xv;
-        r[i] = rv;
+        stb(x, i, xv);
+        stb(r, i, rv);
         const double u = rv * q.im;
         acc[0] += rv * u;
         acc[1] += rv * rv;
@@ -163,7 +180,7 @@ struct PassCgUpdate {
 // (:146-149), and the first iteration's <r,u>, <w,u>, <r,r> (:155,170).
 template <class PC>
 struct PassPcgInit {
-    static constexpr int ND = 3, D = 2;
+    static constexpr int ND = 3, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* r;
     double* u;
@@ -172,13 +189,13 @@ struct PassPcgInit {
     PC iM;
     struct Raw { double r, im; };
     struct Aux {};
-    __device__ Raw load(long long i) const { return {r[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ Raw load(boff_t i) const { return {ldb(r, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t) const { return {}; }
     __device__ double f(const Raw& c) const { return c.r * c.im; }
-    __device__ void finish(long long i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
-        u[i] = uv;
-        w[i] = wv;
-        z[i] = 0.0; q[i] = 0.0; s[i] = 0.0; p[i] = 0.0;
+    __device__ void finish(boff_t i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
+        stb(u, i, uv);
+        stb(w, i, wv);
+        stb(z, i, 0.0); stb(q, i, 0.0); stb(s, i, 0.0); stb(p, i, 0.0);
         acc[0] += c.r * uv;
         acc[1] += wv * uv;
         acc[2] += c.r * c.r;
@@ -189,7 +206,7 @@ struct PassPcgInit {
 // eight recurrences in the reference's order, and the next <r,u>, <w,u>, <r,r>.
 template <class PC>
 struct PassPcgIter {
-    static constexpr int ND = 3, D = 1;
+    static constexpr int ND = 3, D = SEDI_PREFETCH_PCG;
     double a, b, c;
     const double* wold;
     double* wnew;
@@ -198,10 +215,10 @@ struct PassPcgIter {
     double alpha, beta;
     struct Raw { double w, im; };
     struct Aux { double q, s, u, p, z, r, x; };
-    __device__ Raw load(long long i) const { return {wold[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long i) const { return {q[i], s[i], u[i], p[i], z[i], r[i], x[i]}; }
+    __device__ Raw load(boff_t i) const { return {ldb(wold, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t i) const { return {ldb(q, i), ldb(s, i), ldb(u, i), ldb(p, i), ldb(z, i), ldb(r, i), ldb(x, i)}; }
     __device__ double f(const Raw& c) const { return c.w * c.im; }
-    __device__ void finish(long long i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
+    __device__ void finish(boff_t i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
         const double qv = 1.0 * m + beta * o.q;
         const double sv = 1.0 * c.w + beta * o.s;
         const double pv = 1.0 * o.u + beta * o.p;
@@ -210,8 +227,8 @@ struct PassPcgIter {
         const double xv = 1.0 * o.x + alpha * pv;
         const double uv = 1.0 * o.u + (-alpha) * qv;
         const double wv = 1.0 * c.w + (-alpha) * zv;
-        q[i] = qv; s[i] = sv; p[i] = pv; z[i] = zv;
-        r[i] = rv; x[i] = xv; u[i] = uv; wnew[i] = wv;
+        stb(q, i, qv); stb(s, i, sv); stb(p, i, pv); stb(z, i, zv);
+        stb(r, i, rv); stb(x, i, xv); stb(u, i, uv); stb(wnew, i, wv);
         acc[0] += rv * uv;
         acc[1] += wv * uv;
         acc[2] += rv * rv;
@@ -222,7 +239,7 @@ struct PassPcgIter {
 // (p = r at the first iteration), p^ = p.*iM, v = A p^, <v, r^>.
 template <class PC>
 struct PassBicgDirection {
-    static constexpr int ND = 1, D = 2;
+    static constexpr int ND = 1, D = SEDI_PREFETCH_BICG;
     double a, b, c;
     const double *r, *pold, *vold, *rhat;
     double *pnew, *vnew;
@@ -231,15 +248,15 @@ struct PassBicgDirection {
     bool first;
     struct Raw { double r, p, v, im; };
     struct Aux { double rhat; };
-    __device__ Raw load(long long i) const { return {r[i], pold[i], vold[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long i) const { return {rhat[i]}; }
+    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(pold, i), ldb(vold, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t i) const { return {ldb(rhat, i)}; }
     __device__ double pdir(const Raw& q) const {
         return first ? q.r : 1.0 * q.r + beta * q.p + cbw * q.v;  // combine3 (:169)
     }
     __device__ double f(const Raw& q) const { return pdir(q) * q.im; }
-    __device__ void finish(long long i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
-        pnew[i] = pdir(q);
-        vnew[i] = v;
+    __device__ void finish(boff_t i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
+        stb(pnew, i, pdir(q));
+        stb(vnew, i, v);
         acc[0] += v * o.rhat;
     }
 };
@@ -248,18 +265,18 @@ struct PassBicgDirection {
 // t = A s^, <s,s>, <t,s>, <t,t>; s and t are not stored.
 template <class PC>
 struct PassBicgHalf {
-    static constexpr int ND = 3, D = 2;
+    static constexpr int ND = 3, D = SEDI_PREFETCH_BICG;
     double a, b, c;
     const double *r, *v;
     PC iM;
     double alpha;
     struct Raw { double r, v, im; };
     struct Aux {};
-    __device__ Raw load(long long i) const { return {r[i], v[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long) const { return {}; }
+    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(v, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t) const { return {}; }
     __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
     __device__ double f(const Raw& q) const { return sval(q) * q.im; }
-    __device__ void finish(long long, const Raw& q, const Aux&, double, double t, double (&acc)[ND]) const {
+    __device__ void finish(boff_t, const Raw& q, const Aux&, double, double t, double (&acc)[ND]) const {
         const double sv = sval(q);
         acc[0] += sv * sv;
         acc[1] += t * sv;
@@ -272,7 +289,7 @@ struct PassBicgHalf {
 // <r,r> and the next rho = <r, r^>.
 template <class PC>
 struct PassBicgUpdate {
-    static constexpr int ND = 2, D = 2;
+    static constexpr int ND = 2, D = SEDI_PREFETCH_BICG;
     double a, b, c;
     const double *r, *v, *p, *rhat;
     double *x, *rnew;
@@ -280,17 +297,17 @@ struct PassBicgUpdate {
     double alpha, omega;
     struct Raw { double r, v, im; };
     struct Aux { double p, x, rhat; };
-    __device__ Raw load(long long i) const { return {r[i], v[i], iM.at(i)}; }
-    __device__ Aux load_aux(long long i) const { return {p[i], x[i], rhat[i]}; }
+    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(v, i), iM.at(i)}; }
+    __device__ Aux load_aux(boff_t i) const { return {ldb(p, i), ldb(x, i), ldb(rhat, i)}; }
     __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
     __device__ double f(const Raw& q) const { return sval(q) * q.im; }
-    __device__ void finish(long long i, const Raw& q, const Aux& o, double sh, double t, double (&acc)[ND]) const {
+    __device__ void finish(boff_t i, const Raw& q, const Aux& o, double sh, double t, double (&acc)[ND]) const {
         const double sv = sval(q);
         const double ph = o.p * q.im;
         const double xv = 1.0 * o.x + alpha * ph + omega * sh;  // combine3 (:217)
         const double rv = 1.0 * sv + (-omega) * t;
-        x[i] = xv;
-        rnew[i] = rv;
+        stb(x, i, xv);
+        stb(rnew, i, rv);
         acc[0] += rv * rv;
         acc[1] += rv * o.rhat;
     }
@@ -304,25 +321,18 @@ struct PassBicgUpdate {
     __shared__ double red[16];                       \
     __shared__ int lastflag
 
-template <class P>
-__device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
-                                          double (&lds)[2][TY + 2][TX + 2],
-                                          double (&acc)[P::ND]) {
-#pragma unroll
-    for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
-    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x) march_tile(p, g, tile_of(g, t), lds, acc);
-}
 
+template <int ZC>
 __global__ __launch_bounds__(NTHREADS) void stencil_apply_kernel(TileGrid g, double a, double b,
                                                                  double c, const double* x,
                                                                  double* y) {
     __shared__ double lds[2][TY + 2][TX + 2];
     PassApply P{a, b, c, x, y};
     double acc[1];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc);
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* x,
                                                             const double* rhs, double* r,
@@ -330,7 +340,7 @@ __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl,
     PASS_SHARED;
     PassResidual<PC> P{a, b, c, x, rhs, r, rhat, iM};
     double acc[3];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, false);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     double bb = acc[0];
@@ -360,7 +370,7 @@ __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl,
     if (st->maxiter < 1) cg_outcome_exhausted(st);
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
                                                                 double b, double c, const double* r,
                                                                 double* p0, double* p1, PC iM) {
@@ -370,13 +380,13 @@ __global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl 
     const int it = cst->iter;
     PassCgDirection<PC> P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
     double acc[1];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, false);
     if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->alpha = st->ru / acc[0];  // Cg.hpp:166
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                              double c, const double* p0,
                                                              const double* p1, double* x, double* r,
@@ -387,7 +397,7 @@ __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl
     const int it = cst->iter;
     PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
     double acc[2];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, true);
     if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double ru_new = acc[0];
@@ -402,7 +412,7 @@ __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl
     else st->iter = it + 1;
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* r, double* u,
                                                             double* w, double* z, double* q,
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl,
     PASS_SHARED;
     PassPcgInit<PC> P{a, b, c, r, u, w, z, q, s, p, iM};
     double acc[3];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, true);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->gamma = acc[0];  // PipelinedCg.hpp:171-173 at iter 1
@@ -423,7 +433,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl,
     st->iter = 1;
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, double* w0, double* w1,
                                                             double* x, double* r, double* u,
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl,
     PassPcgIter<PC> P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
                   cst->alpha, cst->beta};
     double acc[3];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 0);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double err = st->err_pending;  // rr before this iteration's update (:227)
@@ -459,7 +469,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl,
     st->iter = it + 1;
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
     TileGrid g, Ctl ctl, double a, double b, double c, double* const* vv, PC iM) {
     // vv: r0, r1, rhat, p0, p1, v0, v1
@@ -471,13 +481,13 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
     PassBicgDirection<PC> P{a, b, c, vv[0 + cur], vv[3 + prv], vv[5 + prv], vv[2],
                         vv[3 + cur], vv[5 + cur], iM, cst->beta, cst->cbw, it == 1};
     double acc[1];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 1);
     if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->alpha = st->rho / acc[0];  // BiCgStab.hpp:187
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl ctl, double a,
                                                                  double b, double c,
                                                                  double* const* vv, PC iM) {
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl
     const int cur = cst->iter & 1;
     PassBicgHalf<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], iM, cst->alpha};
     double acc[3];
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, (cst->iter & 1) == 0);
     if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->ss = acc[0];                               // :193-194
@@ -495,7 +505,7 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl
     else st->omega = acc[1] / acc[2];              // :213
 }
 
-template <class PC>
+template <class PC, int ZC>
 __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, Ctl ctl, double a,
                                                                    double b, double c,
                                                                    double* const* vv, double* x,
@@ -513,8 +523,9 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
         const long long n = (long long)g.n1 * g.n2 * g.n3;
         for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
              i += (long long)gridDim.x * NTHREADS) {
-            const double ph = p[i] * iM.at(i);
-            x[i] = 1.0 * x[i] + alpha * ph;
+            const boff_t bo = (boff_t)(i * 8);
+            const double ph = ldb(p, bo) * iM.at(bo);
+            stb(x, bo, 1.0 * ldb(x, bo) + alpha * ph);
         }
         acc[0] = acc[1] = 0.0;
         if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -523,7 +534,7 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
     }
     PassBicgUpdate<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], vv[3 + cur], vv[2],
                      x, vv[0 + (cur ^ 1)], iM, cst->alpha, cst->omega};
-    run_tiles(P, g, lds, acc);
+    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 1);
     if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double err = sqrt(acc[0] / st->bb);  // :223-227
@@ -544,32 +555,55 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int pick_zc(const Shape& s) {
-    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + TY - 1) / TY);
-    long long zc = (long long)s.n3 * nxy / 2048;
-    if (zc < 2) zc = 2;
-    if (zc > 64) zc = 64;
-    return (int)zc;
-}
-
-static int max_blocks() {
+static int num_cus() {
     static int cached = 0;
     if (!cached) {
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        cached = 8 * cus;
+        cached = cus;
     }
     return cached;
 }
 
+// Workgroups of 256 lanes that can be resident per CU (4 waves each; at most
+// 8 with the 32 waves a CU holds).
+constexpr int MAX_BLOCKS_PER_CU = 8;
+
+// Geometry of a pass on `s`, from the occupancy of its ZC=16 instantiation
+// (the ZC variants differ only in unrolling); cached per kernel and shape.
+template <typename K>
+static const KGeom& geom_for(K kern16, const Shape& s, int D) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
+        (void)hipGetLastError();
+        occ = 4;
+    }
+    occ = std::min(occ, MAX_BLOCKS_PER_CU);
+    return cache.emplace(key, plan_geometry(s, occ * num_cus(), D)).first->second;
+}
+
+// kernel<..., ZC> for the ZC the geometry chose
+#define SEDI_ZC_LAUNCH(name, K, geo, ...)                                                          \
+    ((geo).tg.zc == 8    ? launch(name, K<8>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)    \
+     : (geo).tg.zc == 32 ? launch(name, K<32>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)   \
+                         : launch(name, K<16>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__))
+#define SEDI_ZC_LAUNCH_PC(name, K, PCT, geo, ...)                                                   \
+    ((geo).tg.zc == 8    ? launch(name, K<PCT, 8>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__) \
+     : (geo).tg.zc == 32 ? launch(name, K<PCT, 32>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)\
+                         : launch(name, K<PCT, 16>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__))
+
 int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
                       hipStream_t stream) {
-    const TileGrid g = make_tile_grid(s, pick_zc(s));
-    const int nb = (int)std::min<long long>(g.ntiles, max_blocks());
-    SEDI_HIP(launch("stencil_apply_kernel", stencil_apply_kernel, dim3(nb), dim3(NTHREADS), stream,
-                    g, a, b, c, x, y));
+    const KGeom& k = geom_for(stencil_apply_kernel<16>, s, PassApply::D);
+    SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel, k, stream, k.tg, a, b, c, x, y));
     return SEDI_OK;
 }
 
@@ -589,14 +623,13 @@ int PlainSolver::create(const Shape& s, double a, double b, double c, int kind, 
     a_ = a; b_ = b; c_ = c;
     kind_ = kind;
     mode_ = mode;
-    tg_ = make_tile_grid(s, pick_zc(s));
-    nblocks_ = (int)std::min<long long>(tg_.ntiles, max_blocks());
+    nblocks_max_ = MAX_BLOCKS_PER_CU * num_cus();
     SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SEDI_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
     SEDI_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
     SEDI_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(DevState), hipHostMallocDefault));
     // control block: state | counter | partials
-    const size_t ctl_bytes = 512 + sizeof(double) * 3 * (size_t)nblocks_;
+    const size_t ctl_bytes = 512 + sizeof(double) * 3 * (size_t)nblocks_max_;
     SEDI_HIP(ctl_.reserve(ctl_bytes));
     SEDI_HIP(hipMemset(ctl_.p, 0, ctl_bytes));
     const int nv = kind == SEDI_CG ? 3 : (kind == SEDI_PCG ? 8 : 7);
@@ -618,18 +651,21 @@ int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
     DevState* st = reinterpret_cast<DevState*>(ctl_.p);
     Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const dim3 grid(nblocks_), block(NTHREADS);
+    const dim3 block(NTHREADS);
+    const KGeom& kr = geom_for(residual_kernel<PC, 16>, shape_, PassResidual<PC>::D);
     if (kind_ == SEDI_BICGSTAB) {
         // r_1 lives in r[1] (buffers alternate with the iteration's parity)
-        SEDI_HIP(launch("residual_kernel", residual_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)x, b, v_[1], v_[2], pc));
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, stream_, kr.tg, ctl,
+                        a_, b_, c_, (const double*)x, b, v_[1], v_[2], pc));
     } else {
-        SEDI_HIP(launch("residual_kernel", residual_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, stream_, kr.tg, ctl,
+                        a_, b_, c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
     }
-    if (kind_ == SEDI_PCG)
-        SEDI_HIP(launch("pcg_init_kernel", pcg_init_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
-                        c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
+    if (kind_ == SEDI_PCG) {
+        const KGeom& kp = geom_for(pcg_init_kernel<PC, 16>, shape_, PassPcgInit<PC>::D);
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_init_kernel", pcg_init_kernel, PC, kp, stream_, kp.tg, ctl,
+                        a_, b_, c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
+    }
     return SEDI_OK;
 }
 
@@ -638,24 +674,26 @@ int PlainSolver::launch_iterations(int count, PC pc, double* x) {
     Ctl ctl{reinterpret_cast<DevState*>(ctl_.p),
             reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const dim3 grid(nblocks_), block(NTHREADS);
+    const dim3 block(NTHREADS);
     for (int k = 0; k < count; k++) {
         if (kind_ == SEDI_CG) {
-            SEDI_HIP(launch("cg_direction_kernel", cg_direction_kernel<PC>, grid, block, stream_, tg_, ctl,
-                            a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
-            SEDI_HIP(launch("cg_update_kernel", cg_update_kernel<PC>, grid, block, stream_, tg_, ctl, a_,
-                            b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
+            const KGeom& g1 = geom_for(cg_direction_kernel<PC, 16>, shape_, PassCgDirection<PC>::D);
+            const KGeom& g2 = geom_for(cg_update_kernel<PC, 16>, shape_, PassCgUpdate<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, stream_, g2.tg,
+                            ctl, a_, b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
         } else if (kind_ == SEDI_PCG) {
-            SEDI_HIP(launch("pcg_iter_kernel", pcg_iter_kernel<PC>, grid, block, stream_, tg_, ctl, a_, b_,
-                            c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
+            const KGeom& g1 = geom_for(pcg_iter_kernel<PC, 16>, shape_, PassPcgIter<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, stream_, g1.tg,
+                            ctl, a_, b_, c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
         } else {
             double* const* vv = reinterpret_cast<double* const*>(static_cast<char*>(ctl_.p) + 384);
-            SEDI_HIP(launch("bicgstab_direction_kernel", bicgstab_direction_kernel<PC>, grid, block,
-                            stream_, tg_, ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(launch("bicgstab_half_kernel", bicgstab_half_kernel<PC>, grid, block, stream_, tg_,
-                            ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(launch("bicgstab_update_kernel", bicgstab_update_kernel<PC>, grid, block, stream_,
-                            tg_, ctl, a_, b_, c_, vv, x, pc));
+            const KGeom& g1 = geom_for(bicgstab_direction_kernel<PC, 16>, shape_, PassBicgDirection<PC>::D);
+            const KGeom& g2 = geom_for(bicgstab_half_kernel<PC, 16>, shape_, PassBicgHalf<PC>::D);
+            const KGeom& g3 = geom_for(bicgstab_update_kernel<PC, 16>, shape_, PassBicgUpdate<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_direction_kernel", bicgstab_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, vv, pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_half_kernel", bicgstab_half_kernel, PC, g2, stream_, g2.tg, ctl, a_, b_, c_, vv, pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_update_kernel", bicgstab_update_kernel, PC, g3, stream_, g3.tg, ctl, a_, b_, c_, vv, x, pc));
         }
     }
     return SEDI_OK;

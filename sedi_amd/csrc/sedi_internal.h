@@ -43,13 +43,16 @@ struct Shape {
 // stochastic-estimator.cc:102-103), which kernels instantiated with
 // PrecondScalar keep in a scalar register; sedi_solve with an explicit iM
 // vector uses PrecondVec.
+// at() takes the cell's byte offset.
 struct PrecondScalar {
     double s;
-    __device__ __forceinline__ double at(long long) const { return s; }
+    __device__ __forceinline__ double at(unsigned) const { return s; }
 };
 struct PrecondVec {
     const double* v;
-    __device__ __forceinline__ double at(long long idx) const { return v[idx]; }
+    __device__ __forceinline__ double at(unsigned bo) const {
+        return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(v) + bo);
+    }
 };
 
 }  // namespace sedi
