@@ -132,7 +132,11 @@ typedef struct sedi_estimate_args {
     int ignore_errors;      /* keep going after a failed solve             (-i) */
     uint32_t seed;          /* mt19937 seed of the probe stream                 */
     int precision;          /* SEDI_FP64 or SEDI_FP32_MIXED                     */
-    int batch_k;            /* probes solved together (1 = plain, one at a time) */
+    int batch_k;            /* probes solved together (1 = plain, one at a time);
+                               above 1 (or with SEDI_FP32_MIXED): CG only, the
+                               probes in blocks of 4 (padded with zero RHS)   */
+    int rep_every;          /* SEDI_FP32_MIXED: FP64 residual replacement
+                               every rep_every iterations (0 = 50)            */
 } sedi_estimate_args;
 
 /* Allocates everything the run needs (excluded from the timed region, as the
@@ -161,6 +165,36 @@ void sedi_estimator_destroy(sedi_estimator *est);
 /* create + run + destroy. */
 int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
                   double *errs, int *codes, double *seconds);
+
+/* ---- batched multi-RHS CG (no reference counterpart) ---------------------- */
+/* The reference solves one probe at a time (stochastic-estimator.cc:170-199);
+ * a batch solves up to K right-hand sides together with the Cg.hpp:102-208
+ * recurrences kept per right-hand side: one stencil sweep per pass serves
+ * every right-hand side, with per-RHS alpha, beta, convergence and masks.
+ *   precision SEDI_FP64: FP64 vectors, as the plain solver;
+ *   SEDI_FP32_MIXED: FP32 vectors and stencil, FP64 dot products, scalars and
+ *   solution accumulator, with an FP64 residual replacement r = b - A x every
+ *   rep_every iterations (0 = 50) and before a right-hand side is declared
+ *   converged (its returned err is then the true residual's).
+ * Results of a right-hand side do not depend on the others in its batch or on
+ * K.  K is rounded up to a multiple of 4. */
+typedef struct sedi_batch sedi_batch;
+int sedi_batch_create(const sedi_grid *grid, const sedi_stencil *stencil, int K,
+                      int precision, int rep_every, sedi_batch **out);
+/* Solves A x_k = b_k from x_k = 0 for k < nrhs <= K; b and x are [nrhs][n]
+ * FP64 (host or device).  iters/errs/codes (may be NULL) per right-hand side,
+ * with the meaning of sedi_solve's final_iter/final_err/return code. */
+int sedi_batch_solve(sedi_batch *batch, const double *b, int nrhs, double *x, int maxiter,
+                     double tol, int *iters, double *errs, int *codes);
+/* The per-iteration error of right-hand side k in the last solve; returns the
+ * number of entries (written up to cap), or a negative error code. */
+int sedi_batch_err_history(sedi_batch *batch, int k, double *out, int cap);
+void sedi_batch_destroy(sedi_batch *batch);
+
+/* Probes k0 .. k0+count-1 of the seeded stream (the estimator's probes), as
+ * [count][ncells] doubles (host or device), each generated from its own
+ * jump-ahead start: equal to sedi_rademacher_fill(seed, 2*ncells*k, ...). */
+int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double *v);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Kernel timing with HIP events recorded on the launching stream around every

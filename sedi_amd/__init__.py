@@ -10,6 +10,8 @@ libsedi_amd.so (include/sedi_c.h):
         -> (result_code, final_err, final_iter)
     rademacher_fill(seed, first, n)     DistrRngStd::fill_plusminusone
     estimate(...)                       stochastic-estimator.cc:158-216
+    Batch(grid, K, precision)           multi-RHS CG (no reference counterpart)
+    probe_batch(seed, n, k0, count)     the estimator's probes k0.., by jump-ahead
 
 Arrays are numpy float64, n1*n2*n3 cells with i1 fastest (reshape a
 (n3, n2, n1) C-ordered array to 1-D).  Everything runs in the HIP library; there
@@ -24,7 +26,8 @@ import numpy as np
 __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEGA",
            "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
            "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
-           "Estimator", "estimate", "profile_enable", "profile_get", "profile_reset"]
+           "Estimator", "estimate", "Batch", "probe_batch", "profile_enable", "profile_get",
+           "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 # SEDI_AMD_LIB selects another build of the library (e.g. a tuning variant)
@@ -54,7 +57,8 @@ class _EstArgs(ctypes.Structure):
     _fields_ = [("grid", _Grid), ("stencil", _Stencil), ("kind", ctypes.c_int),
                 ("mode", ctypes.c_int), ("samples", ctypes.c_int), ("maxiter", ctypes.c_int),
                 ("tol", ctypes.c_double), ("ignore_errors", ctypes.c_int),
-                ("seed", ctypes.c_uint32), ("precision", ctypes.c_int), ("batch_k", ctypes.c_int)]
+                ("seed", ctypes.c_uint32), ("precision", ctypes.c_int), ("batch_k", ctypes.c_int),
+                ("rep_every", ctypes.c_int)]
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
@@ -86,6 +90,14 @@ def lib():
         L.sedi_estimator_destroy.argtypes = [vp]
         L.sedi_estimator_destroy.restype = None
         L.sedi_estimate.argtypes = [ctypes.POINTER(_EstArgs), vp, ip, dp, ip, dp]
+        L.sedi_batch_create.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Stencil), ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.sedi_batch_solve.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, ip,
+                                       dp, ip]
+        L.sedi_batch_err_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int]
+        L.sedi_batch_destroy.argtypes = [vp]
+        L.sedi_batch_destroy.restype = None
+        L.sedi_probe_batch.argtypes = [ctypes.c_uint32, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
         L.sedi_profile_enable.argtypes = [ctypes.c_char_p]
         L.sedi_profile_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), dp]
         _lib = L
@@ -204,17 +216,80 @@ def rademacher_fill(seed, first_draw, ncells):
     return v
 
 
+def probe_batch(seed, ncells, k0, count):
+    """Probes k0 .. k0+count-1 of the estimator's stream, shape (count, ncells)."""
+    v = np.empty((int(count), int(ncells)))
+    _check(lib().sedi_probe_batch(int(seed) & 0xffffffff, int(ncells), int(k0), int(count), _ptr(v)))
+    return v
+
+
+class Batch:
+    """Up to K right-hand sides solved together by CG (Cg.hpp:102-208 per RHS).
+
+    precision FP64 keeps the plain solver's arithmetic; FP32_MIXED stores the
+    vectors in FP32 with FP64 reductions and an FP64 residual replacement
+    every rep_every iterations (0 = 50).
+    """
+
+    def __init__(self, grid, K, precision=FP64, rep_every=0, stencil=None):
+        self.grid = tuple(int(v) for v in grid)
+        self.stencil = stencil or Stencil()
+        g, self.n = _grid(self.grid)
+        self.K = (int(K) + 3) // 4 * 4
+        h = ctypes.c_void_p()
+        _check(lib().sedi_batch_create(ctypes.byref(g), ctypes.byref(self.stencil._c()), int(K),
+                                       int(precision), int(rep_every), ctypes.byref(h)))
+        self._h = h
+
+    def solve(self, b, maxiter=1000, tol=1e-4):
+        """b: (nrhs, n).  Returns (x, iters, errs, codes), x of shape (nrhs, n)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.ndim == 1:
+            b = b.reshape(1, -1)
+        nrhs = b.shape[0]
+        if b.shape[1] != self.n:
+            raise ValueError("expected %d cells per right-hand side, got %d" % (self.n, b.shape[1]))
+        x = np.empty_like(b)
+        iters = np.zeros(nrhs, dtype=np.int32)
+        codes = np.full(nrhs, -1, dtype=np.int32)
+        errs = np.zeros(nrhs)
+        ip = ctypes.POINTER(ctypes.c_int)
+        _check(lib().sedi_batch_solve(self._h, _ptr(b), nrhs, _ptr(x), int(maxiter), float(tol),
+                                      iters.ctypes.data_as(ip),
+                                      errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      codes.ctypes.data_as(ip)))
+        return x, iters, errs, codes
+
+    def err_history(self, k):
+        n = _check(lib().sedi_batch_err_history(self._h, int(k), None, 0))
+        out = np.zeros(n)
+        lib().sedi_batch_err_history(self._h, int(k), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Estimator:
     """The sample loop of stochastic-estimator.cc:158-216 (seeded probes)."""
 
     def __init__(self, grid, kind=CG, samples=8, maxiter=1000, tol=1e-4, seed=1234,
-                 ignore_errors=False, stencil=None, mode=0, precision=FP64, batch_k=1):
+                 ignore_errors=False, stencil=None, mode=0, precision=FP64, batch_k=1,
+                 rep_every=0):
         st = stencil or Stencil()
         g, self.n = _grid(grid)
         self.samples = int(samples)
         self.args = _EstArgs(g, st._c(), int(kind), int(mode), self.samples, int(maxiter),
                              float(tol), int(bool(ignore_errors)), int(seed) & 0xffffffff,
-                             int(precision), int(batch_k))
+                             int(precision), int(batch_k), int(rep_every))
         h = ctypes.c_void_p()
         _check(lib().sedi_estimator_create(ctypes.byref(self.args), ctypes.byref(h)))
         self._h = h

@@ -4,6 +4,9 @@
 #include <exception>
 #include <new>
 
+#include <algorithm>
+
+#include "batched.h"
 #include "estimator.h"
 #include "mt_jump.h"
 #include "profiler.h"
@@ -18,6 +21,11 @@ struct sedi_solver {
 
 struct sedi_estimator {
     sedi::Estimator impl;
+};
+
+struct sedi_batch {
+    sedi::BatchSolver impl;
+    sedi::BatchResult last;
 };
 
 namespace {
@@ -152,6 +160,105 @@ int sedi_mt_state_after(uint32_t seed, uint64_t draws, uint32_t* state624) {
         uint32_t s0[624];
         sedi::mt_seed_state(seed, s0);
         sedi::mt_jump_state(s0, draws, state624);
+        return SEDI_OK;
+    });
+}
+
+int sedi_batch_create(const sedi_grid* grid, const sedi_stencil* st, int K, int precision, int rep_every,
+                      sedi_batch** out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(out && st, SEDI_ERR_ARG, "stencil and out must not be NULL");
+        *out = nullptr;
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        SEDI_REQUIRE(K >= 1, SEDI_ERR_ARG, "batch size must be >= 1");
+        SEDI_REQUIRE(precision == SEDI_FP64 || precision == SEDI_FP32_MIXED, SEDI_ERR_ARG,
+                     "unknown precision %d", precision);
+        SEDI_REQUIRE(rep_every >= 0, SEDI_ERR_ARG, "rep_every must be >= 0");
+        if ((rc = sedi::ensure_device())) return rc;
+        sedi_batch* h = new sedi_batch();
+        rc = h->impl.create(s, st->a, st->b, st->c, (K + 3) / 4 * 4, precision == SEDI_FP32_MIXED, rep_every);
+        if (rc) {
+            delete h;
+            return rc;
+        }
+        *out = h;
+        return SEDI_OK;
+    });
+}
+
+int sedi_batch_solve(sedi_batch* h, const double* b, int nrhs, double* x, int maxiter, double tol, int* iters,
+                     double* errs, int* codes) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(h && b && x, SEDI_ERR_ARG, "batch, b and x must not be NULL");
+        SEDI_REQUIRE(nrhs >= 1 && nrhs <= h->impl.K(), SEDI_ERR_ARG, "nrhs %d outside 1..%d", nrhs,
+                     h->impl.K());
+        const size_t bytes = sizeof(double) * (size_t)h->impl.n() * nrhs;
+        hipStream_t s = h->impl.stream();
+        sedi::DevBuf bs, xs;
+        const double* bd = b;
+        double* xd = x;
+        if (!sedi::is_device_pointer(b)) {
+            SEDI_HIP(bs.reserve(bytes));
+            SEDI_HIP(hipMemcpyAsync(bs.p, b, bytes, hipMemcpyHostToDevice, s));
+            bd = bs.as<double>();
+        }
+        const bool xhost = !sedi::is_device_pointer(x);
+        if (xhost) {
+            SEDI_HIP(xs.reserve(bytes));
+            xd = xs.as<double>();
+        }
+        int rc = h->impl.load_rhs(bd, nrhs);
+        if (!rc) rc = h->impl.solve(maxiter, tol, &h->last);
+        if (!rc) rc = h->impl.store_x(xd, nrhs);
+        if (rc) return rc;
+        SEDI_HIP(hipStreamSynchronize(s));
+        if (xhost) SEDI_HIP(hipMemcpy(x, xd, bytes, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nrhs; k++) {
+            if (iters) iters[k] = h->last.iters[k];
+            if (errs) errs[k] = h->last.errs[k];
+            if (codes) codes[k] = h->last.codes[k];
+        }
+        return SEDI_OK;
+    });
+}
+
+int sedi_batch_err_history(sedi_batch* h, int k, double* out, int cap) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(h, SEDI_ERR_ARG, "batch is NULL");
+        SEDI_REQUIRE(k >= 0 && k < (int)h->last.iters.size(), SEDI_ERR_ARG, "no result for right-hand side %d", k);
+        const std::vector<double> hist = h->impl.history(k);
+        const int code = h->last.codes[k];
+        const int n = (code == 0 || code == 1) ? std::min<int>(h->last.iters[k], (int)hist.size()) : 0;
+        for (int i = 0; i < n && i < cap; i++) out[i] = hist[i];
+        return n;
+    });
+}
+
+void sedi_batch_destroy(sedi_batch* h) { delete h; }
+
+int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double* v) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(ncells >= 1 && k0 >= 0 && count >= 1 && v, SEDI_ERR_ARG, "bad probe batch arguments");
+        SEDI_REQUIRE(sedi::mt_charpoly_ok(), SEDI_ERR_STATE, "mt19937 characteristic polynomial unavailable");
+        int rc = sedi::ensure_device();
+        if (rc) return rc;
+        const int K = (count + 3) / 4 * 4;
+        sedi::DevBuf tmp, scratch, vs;
+        SEDI_HIP(tmp.reserve(sizeof(double) * (size_t)ncells * K));
+        const size_t bytes = sizeof(double) * (size_t)ncells * count;
+        const bool host = !sedi::is_device_pointer(v);
+        double* vd = v;
+        if (host) {
+            SEDI_HIP(vs.reserve(bytes));
+            vd = vs.as<double>();
+        }
+        hipStream_t s = nullptr;
+        if ((rc = sedi::fill_probe_batch(seed, ncells, k0, K, count, false, tmp.p, s, scratch))) return rc;
+        if ((rc = sedi::unbatch_rows(tmp.as<double>(), ncells, count, vd, s))) return rc;
+        SEDI_HIP(hipStreamSynchronize(s));
+        if (host) SEDI_HIP(hipMemcpy(v, vd, bytes, hipMemcpyDeviceToHost));
         return SEDI_OK;
     });
 }

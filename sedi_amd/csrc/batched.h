@@ -1,0 +1,64 @@
+// batched.h -- multi-RHS CG over blocks of 4 probes (FP64 or FP32 storage).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <vector>
+
+#include "march.h"
+#include "runtime.h"
+#include "sedi_internal.h"
+
+namespace sedi {
+
+// Per-RHS solver results of a batch.
+struct BatchResult {
+    std::vector<int> iters, codes;
+    std::vector<double> errs;
+};
+
+class BatchSolver {
+  public:
+    ~BatchSolver();
+    // K: right-hand sides per batch (a multiple of 4); fp32: FP32 vectors with
+    // FP64 reductions and residual replacement every rep_every iterations.
+    int create(const Shape& s, double a, double b, double c, int K, bool fp32, int rep_every);
+    // Solves the K systems A x_k = b_k from x = 0, where b is the batch layout
+    // [K/4][N][4] of the storage type (device).  Right-hand sides that are
+    // all zero are done at once (padding).  Synchronises the stream.
+    int solve(int maxiter, double tol, BatchResult* res);
+    // t += sum_k x_k v_k, q += sum_k v_k^2 over the first k_active probes, in
+    // probe order (the reference's accumulation, stochastic-estimator.cc:195,198).
+    int accumulate(double* t, double* q, int k_active);
+    // b: kact right-hand sides as [kact][N] FP64 (device), rounded to the
+    // storage type; slots kact..K-1 are zeroed.  Asynchronous on stream().
+    int load_rhs(const double* b, int kact);
+    // x: the first kact solutions as [kact][N] FP64 (device; x_hi + x_lo in
+    // the mixed form).  Asynchronous on stream().
+    int store_x(double* x, int kact);
+    void* rhs() const { return b_; }
+    bool fp32() const { return fp32_; }
+    int K() const { return K_; }
+    long long n() const { return shape_.n(); }
+    hipStream_t stream() const { return stream_; }
+    std::vector<double> history(int k) const;
+
+  private:
+    template <class T> int solve_t(int maxiter, double tol, BatchResult* res);
+    template <class T> int accumulate_t(double* t, double* q, int k_active);
+    template <class T> int pack_t(const double* b, int kact);
+    template <class T> int unpack_t(double* x, int kact);
+    Shape shape_{};
+    double a_ = 6.0, b_c_ = 1.0, c_ = 1.0;
+    int K_ = 0, nrb_ = 0;
+    bool fp32_ = true;
+    int rep_every_ = 50;
+    hipStream_t stream_ = nullptr;
+    DevBuf vec_, ctl_, hist_;
+    void *b_ = nullptr, *r_ = nullptr, *p_[2] = {nullptr, nullptr}, *xlo_ = nullptr;
+    double* xhi_ = nullptr;
+    int* pinned_ = nullptr;
+    int hist_cap_ = 0;
+    std::vector<double> hist_host_;
+};
+
+}  // namespace sedi

@@ -1,0 +1,715 @@
+// batched.hip -- multi-RHS CG: one stencil sweep serves a block of 4 probes.
+//
+// Layout: a batch of K right-hand sides is stored as K/4 RHS blocks, each a
+// dense grid of cells holding the 4 probes' values side by side ([K/4][N][4]).
+// A lane of the march owns one cell of one block and moves 4 values per
+// access (16 B in FP32: a wave load covers 1 KiB, as in the plain path), so
+// the stencil's neighbour reuse, the LDS plane and the halo are shared by 4
+// probes.  Workgroups are laid out as (tile, RHS block); every block keeps
+// its own scalars and reductions.
+//
+// Arithmetic ("mixed"): with FP32 storage the stencil and the
+// vector updates run in FP32 with the FP64 scalars rounded to FP32, while the
+// dot products accumulate the exact FP64 products of the FP32 values, and the
+// scalars (alpha, beta, the norms) stay FP64.  Every rep_every iterations, and
+// whenever an RHS's recursive residual reaches the tolerance, that RHS gets an
+// FP64 reliable update: x_hi += x_lo, x_lo = 0, r = b - A x_hi in FP64 (stored
+// as FP32), and beta is recomputed from the new residual; an RHS is done only
+// when its true residual meets the tolerance (Sleijpen & van der Vorst 1996).
+// With FP64 storage the recurrences are the plain solver's.
+//
+// Per-RHS state makes each probe's trajectory independent of the others in
+// its batch: an RHS that is done keeps x and r (alpha = 0), replacements are
+// applied per RHS, and each RHS's sums have a fixed order for a given grid.
+#include "batched.h"
+#include "profiler.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#ifndef SEDI_PREFETCH_BATCH
+#define SEDI_PREFETCH_BATCH 1
+#endif
+
+namespace sedi {
+
+// ---- 4-wide values ---------------------------------------------------------
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+    T v[4];
+    __device__ __forceinline__ T& operator[](int i) { return v[i]; }
+    __device__ __forceinline__ const T& operator[](int i) const { return v[i]; }
+};
+template <class T>
+__device__ __forceinline__ V4<T> operator+(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] + b[0], a[1] + b[1], a[2] + b[2], a[3] + b[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator-(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] - b[0], a[1] - b[1], a[2] - b[2], a[3] - b[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator*(T s, const V4<T>& a) {
+    return {{s * a[0], s * a[1], s * a[2], s * a[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator*(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] * b[0], a[1] * b[1], a[2] * b[2], a[3] * b[3]}};
+}
+
+// ---- device control --------------------------------------------------------
+struct BCtl {
+    double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *hist;
+    int *fiter, *code, *done, *rep, *fold;
+    int *iter, *rb_done;
+    double* partials;
+    unsigned* counters;
+    double tol;
+    int maxiter, rep_every, mixed, K;
+    long long hstride;  // history entries per RHS
+    long long pstride;  // partial sums per RHS block
+};
+
+__device__ __forceinline__ void bset_done(const BCtl& c, int k, int code, int it, double e) {
+    c.done[k] = 1;
+    c.code[k] = code;
+    c.fiter[k] = it;
+    c.ferr[k] = e;
+}
+
+__device__ __forceinline__ void brb_update(const BCtl& c, int rb) {
+    c.rb_done[rb] = c.done[4 * rb] & c.done[4 * rb + 1] & c.done[4 * rb + 2] & c.done[4 * rb + 3];
+}
+
+// ---- set-up: r = b (x0 = 0), x = 0, <b,b>, <r, r.*iM> -----------------------
+// (Cg.hpp:111-147 for x = 0: A x = 0, so r = b exactly)
+template <class T>
+__global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl c, const V4<T>* b,
+                                                              V4<T>* r, V4<T>* xlo, double* xhi, T im) {
+    __shared__ double red[4 * 8];
+    __shared__ int lastflag;
+    const int rb = blockIdx.y;
+    const V4<T>* bq = b + (long long)rb * n;
+    V4<T>* rq = r + (long long)rb * n;
+    V4<T>* xq = xlo + (long long)rb * n;
+    double acc[8];
+#pragma unroll
+    for (int d = 0; d < 8; d++) acc[d] = 0.0;
+    for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
+         i += (long long)gridDim.x * NTHREADS) {
+        const V4<T> v = bq[i];
+        rq[i] = v;
+        xq[i] = V4<T>{};
+        if (xhi)
+#pragma unroll
+            for (int q = 0; q < 4; q++) xhi[(long long)(4 * rb + q) * n + i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double bv = (double)v[q];
+            const T u = v[q] * im;
+            acc[q] += bv * bv;
+            acc[4 + q] += bv * (double)u;
+        }
+    }
+    if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        double bb = acc[q];
+        if (bb == 0.0) bb = 1.0;  // Cg.hpp:115
+        const double e = sqrt(acc[q] / bb);
+        c.bb[k] = bb;
+        c.err[k] = e;
+        c.ru[k] = acc[4 + q];
+        c.ru_old[k] = acc[4 + q];
+        c.alpha[k] = 0.0;
+        c.beta[k] = 0.0;
+        c.rep[k] = 0;
+        c.fold[k] = 0;
+        c.done[k] = 0;
+        if (e < c.tol) bset_done(c, k, 0, 0, e);  // Cg.hpp:130-134
+        else if (c.maxiter < 1) {                 // :196-206 with no iteration run
+            if (e <= c.tol) bset_done(c, k, 0, c.maxiter + 1, e);
+            else bset_done(c, k, 1, c.maxiter, e);
+        }
+    }
+    c.iter[rb] = 1;
+    brb_update(c, rb);
+}
+
+// ---- CG direction: p = u + beta p, s = A p, <s, p> (Cg.hpp:152-166,175,188) --
+template <class T>
+struct PassBDir {
+    using V = V4<T>;
+    static constexpr int ND = 4, D = SEDI_PREFETCH_BATCH;
+    T a, b, c;
+    const V* r;
+    const V* pold;
+    V* pnew;
+    T im;
+    V beta;
+    bool first;
+    struct Raw { V r, p; };
+    struct Aux {};
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
+    __device__ V f(const Raw& q) const {
+        const V u = im * q.r;
+        return first ? u : u + beta * q.p;
+    }
+    __device__ void finish(cidx_t i, const Raw&, const Aux&, const V& p, const V& s,
+                           double (&acc)[ND]) const {
+        stc(pnew, i, p);
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] += (double)s[q] * (double)p[q];
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_dir_kernel(TileGrid g, BCtl c, T a, T b, T cc,
+                                                             long long n, const V4<T>* r,
+                                                             V4<T>* p0, V4<T>* p1, T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ double red[4 * 4];
+    __shared__ int lastflag;
+    const int it = c.iter[rb];
+    const long long off = (long long)rb * n;
+    V4<T> beta;
+#pragma unroll
+    for (int q = 0; q < 4; q++) beta[q] = c.done[4 * rb + q] ? (T)0 : (T)c.beta[4 * rb + q];
+    PassBDir<T> P{a, b, cc, r + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off, im,
+                  beta, it == 1};
+    double acc[4];
+    run_tiles<PassBDir<T>, ZC>(P, g, lds, acc, false);
+    if (!grid_sum<4>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        c.alpha[k] = c.done[k] ? 0.0 : c.ru[k] / acc[q];  // Cg.hpp:166
+    }
+}
+
+// ---- CG update: x += alpha p, r -= alpha s, <r,u>, <r,r> (Cg.hpp:168-193) ---
+template <class T>
+struct PassBUpd {
+    using V = V4<T>;
+    static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
+    T a, b, c;
+    const V* p;
+    V* xlo;
+    V* r;
+    T im;
+    V alpha;
+    struct Raw { V p; };
+    struct Aux { V x, r; };
+    __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(xlo, i), ldc(r, i)}; }
+    __device__ V f(const Raw& q) const { return q.p; }
+    __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V& pv, const V& s,
+                           double (&acc)[ND]) const {
+        const V xn = o.x + alpha * pv;
+        const V rn = o.r - alpha * s;
+        stc(xlo, i, xn);
+        stc(r, i, rn);
+        const V u = im * rn;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            acc[q] += (double)rn[q] * (double)u[q];
+            acc[4 + q] += (double)rn[q] * (double)rn[q];
+        }
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c, T a, T b, T cc,
+                                                             long long n, const V4<T>* p0,
+                                                             const V4<T>* p1, V4<T>* xlo,
+                                                             V4<T>* r, T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ double red[8 * 4];
+    __shared__ int lastflag;
+    const int it = c.iter[rb];
+    const long long off = (long long)rb * n;
+    V4<T> alpha;
+#pragma unroll
+    for (int q = 0; q < 4; q++) alpha[q] = (T)c.alpha[4 * rb + q];
+    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, xlo + off, r + off, im, alpha};
+    double acc[8];
+    run_tiles<PassBUpd<T>, ZC>(P, g, lds, acc, true);
+    if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        if (c.done[k]) continue;
+        const double ru_new = acc[q];
+        c.ru_old[k] = c.ru[k];
+        c.beta[k] = ru_new / c.ru[k];  // Cg.hpp:185
+        c.ru[k] = ru_new;
+        const double e = sqrt(acc[4 + q] / c.bb[k]);  // Cg.hpp:191
+        c.err[k] = e;
+        c.hist[(long long)k * c.hstride + it - 1] = e;
+        if (c.mixed) {
+            // converged on the recursive residual, a replacement due, or the
+            // last iteration: the FP64 pass decides
+            c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter);
+        } else if (e <= c.tol) {
+            bset_done(c, k, 0, it, e);
+        } else if (it >= c.maxiter) {
+            bset_done(c, k, 1, c.maxiter, e);
+        }
+    }
+    c.iter[rb] = it + 1;
+    brb_update(c, rb);
+}
+
+// ---- FP64 reliable update: r = b - A (x_hi + x_lo) -------------------------
+// The new residual is computed in FP64 from the FP64 sum of the two parts of
+// x and stored rounded to the vector type.  x itself is read at the halo by
+// neighbouring tiles during this pass, so the fold x_hi += x_lo, x_lo = 0 is
+// left to batch_fold_kernel, which runs after it.
+template <class T>
+struct PassBRep {
+    using V = V4<double>;
+    static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
+    double a, b, c;
+    const double* xh[4];  // per-RHS FP64 parts of x of this block ([K][N] layout)
+    const V4<T>* xlo;
+    V4<T>* r;
+    const V4<T>* rhs;
+    T im;
+    int rep[4];
+    struct Raw { V xh; V4<T> xl; };
+    struct Aux { V4<T> b, r; };
+    __device__ Raw load(cidx_t i) const {
+        Raw q;
+#pragma unroll
+        for (int k = 0; k < 4; k++) q.xh[k] = ldc(xh[k], i);
+        q.xl = ldc(xlo, i);
+        return q;
+    }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(rhs, i), ldc((const V4<T>*)r, i)}; }
+    __device__ V f(const Raw& q) const {
+        V x;
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = q.xh[k] + (double)q.xl[k];
+        return x;
+    }
+    __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V&, const V& Ax,
+                           double (&acc)[ND]) const {
+        V4<T> rn = o.r;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (rep[k]) rn[k] = (T)((double)o.b[k] - Ax[k]);  // Cg.hpp:123, in FP64
+        stc(r, i, rn);
+        const V4<T> u = im * rn;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc[k] += (double)rn[k] * (double)u[k];
+            acc[4 + k] += (double)rn[k] * (double)rn[k];
+        }
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c, double a, double b,
+                                                             double cc, long long n, const double* xhi,
+                                                             const V4<T>* xlo, V4<T>* r,
+                                                             const V4<T>* rhs, T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    const int* rp = c.rep + 4 * rb;
+    if (!(rp[0] | rp[1] | rp[2] | rp[3])) return;
+    __shared__ V4<double> lds[2][TY + 2][TX + 2];
+    __shared__ double red[8 * 4];
+    __shared__ int lastflag;
+    const long long off = (long long)rb * n;
+    PassBRep<T> P;
+    P.a = a; P.b = b; P.c = cc;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        P.xh[k] = xhi + (long long)(4 * rb + k) * n;
+        P.rep[k] = rp[k];
+    }
+    P.xlo = xlo + off;
+    P.r = r + off;
+    P.rhs = rhs + off;
+    P.im = im;
+    double acc[8];
+    run_tiles<PassBRep<T>, ZC>(P, g, lds, acc, false);
+    if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    const int it = c.iter[rb] - 1;  // the iteration that requested the update
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        if (!c.rep[k]) continue;
+        c.rep[k] = 0;
+        c.fold[k] = 1;
+        c.ru[k] = acc[q];
+        c.beta[k] = acc[q] / c.ru_old[k];  // beta from the replaced residual
+        const double e = sqrt(acc[4 + q] / c.bb[k]);
+        c.err[k] = e;
+        c.hist[(long long)k * c.hstride + it - 1] = e;
+        if (e <= c.tol) bset_done(c, k, 0, it, e);
+        else if (it >= c.maxiter) bset_done(c, k, 1, c.maxiter, e);
+    }
+    brb_update(c, rb);
+}
+
+// ---- x_hi += x_lo, x_lo = 0 for the RHS just replaced ------------------------
+// Runs for a block whether or not it is done (the last update may have
+// finished it).  x_hi + x_lo is unchanged up to the one FP64 rounding of the
+// sum, which the final x = x_hi + x_lo would make anyway.
+template <class T>
+__global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl c, double* xhi,
+                                                              V4<T>* xlo) {
+    const int rb = blockIdx.y;
+    int* fp = c.fold + 4 * rb;
+    const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
+    if (!(f0 | f1 | f2 | f3)) return;
+    __shared__ double red[4];
+    __shared__ int lastflag;
+    const int fl[4] = {f0, f1, f2, f3};
+    V4<T>* xq = xlo + (long long)rb * n;
+    for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
+         i += (long long)gridDim.x * NTHREADS) {
+        V4<T> xl = xq[i];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (fl[k]) {
+                double* h = xhi + (long long)(4 * rb + k) * n + i;
+                *h = *h + (double)xl[k];
+                xl[k] = (T)0;
+            }
+        }
+        xq[i] = xl;
+    }
+    double acc[1] = {0.0};
+    if (!grid_sum<1>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    fp[0] = fp[1] = fp[2] = fp[3] = 0;
+}
+
+// ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
+template <class T>
+__global__ void batch_accum_kernel(long long n, int kact, const V4<T>* xlo, const double* xhi,
+                                   const V4<T>* b, double* t, double* q) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double tv = t[i], qv = q[i];
+        for (int k = 0; k < kact; k++) {
+            const long long cell = (long long)(k >> 2) * n + i;
+            const double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
+            const double v = (double)b[cell][k & 3];
+            tv = tv + x * v;  // stochastic-estimator.cc:195
+            qv = qv + v * v;  // :198
+        }
+        t[i] = tv;
+        q[i] = qv;
+    }
+}
+
+// ---- [K][N] FP64 <-> batch layout ------------------------------------------
+template <class T>
+__global__ void batch_pack_kernel(long long n, int kact, const double* __restrict__ src, V4<T>* dst) {
+    const int rb = blockIdx.y;
+    V4<T>* d = dst + (long long)rb * n;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        V4<T> v;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = 4 * rb + q;
+            v[q] = k < kact ? (T)src[(long long)k * n + i] : (T)0;
+        }
+        d[i] = v;
+    }
+}
+
+template <class T>
+__global__ void batch_unpack_kernel(long long n, int kact, const V4<T>* __restrict__ xlo,
+                                    const double* __restrict__ xhi, double* __restrict__ x) {
+    const int rb = blockIdx.y;
+    const V4<T>* s = xlo + (long long)rb * n;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const V4<T> v = s[i];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const long long k = 4 * rb + q;
+            if (k < kact) x[k * n + i] = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int cus() {
+    static int v = 0;
+    if (!v) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+    }
+    return v;
+}
+
+// Geometry of a batched pass, per RHS block: the workgroups resident at once
+// (a multiple of 8).  It depends on the grid and the kernel only, not on the
+// batch size, so the order of every reduction -- and with it each RHS's
+// result -- is the same whichever batch an RHS is solved in.
+template <typename K>
+static KGeom bgeom(K kern16, const Shape& s, int D) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
+        (void)hipGetLastError();
+        occ = 2;
+    }
+    occ = std::min(occ, 8);
+    const int per = std::max(8, occ * cus() / 8 * 8);
+    return cache.emplace(key, plan_geometry(s, per, D)).first->second;
+}
+
+// Workgroups per RHS block of the elementwise kernels (independent of the
+// batch size, like bgeom).
+static unsigned ew_blocks(long long n) {
+    const long long g = std::min<long long>((n + NTHREADS - 1) / NTHREADS, 2LL * cus());
+    return (unsigned)std::max<long long>(1, g);
+}
+
+#define SEDI_BZC_LAUNCH(name, K, TT, geo, grid, ...)                                             \
+    ((geo).tg.zc == 8    ? launch(name, K<TT, 8>, grid, dim3(NTHREADS), __VA_ARGS__)             \
+     : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(NTHREADS), __VA_ARGS__)            \
+                         : launch(name, K<TT, 16>, grid, dim3(NTHREADS), __VA_ARGS__))
+
+// control block layout (doubles, then ints)
+enum { CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_NDOUBLE };
+enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_NINT };
+
+BatchSolver::~BatchSolver() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int BatchSolver::create(const Shape& s, double a, double b, double c, int K, bool fp32, int rep_every) {
+    SEDI_REQUIRE(K >= 4 && K % 4 == 0, SEDI_ERR_ARG, "batch size %d must be a positive multiple of 4", K);
+    const size_t vb = (fp32 ? 4 : 8) * 4;  // bytes per cell per RHS block
+    SEDI_REQUIRE((double)s.n() * vb < 4294967296.0, SEDI_ERR_UNSUPPORTED,
+                 "grid too large for the batched path's 32-bit cell offsets");
+    shape_ = s;
+    a_ = a; b_c_ = b; c_ = c;
+    K_ = K;
+    nrb_ = K / 4;
+    fp32_ = fp32;
+    rep_every_ = rep_every > 0 ? rep_every : 50;
+    SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SEDI_HIP(hipHostMalloc((void**)&pinned_, sizeof(int) * 2 * nrb_, hipHostMallocDefault));
+    const size_t n = (size_t)s.n();
+    const size_t vec = n * vb * nrb_;
+    const size_t nvec = 5;  // b, r, p0, p1, x_lo
+    const size_t hib = fp32 ? sizeof(double) * n * K : 0;
+    SEDI_HIP(vec_.reserve(vec * nvec + hib));
+    char* base = static_cast<char*>(vec_.p);
+    b_ = base;
+    r_ = base + vec;
+    p_[0] = base + 2 * vec;
+    p_[1] = base + 3 * vec;
+    xlo_ = base + 4 * vec;
+    xhi_ = fp32 ? reinterpret_cast<double*>(base + 5 * vec) : nullptr;
+    SEDI_HIP(hipMemset(vec_.p, 0, vec * nvec + hib));
+    // control: doubles [CD_NDOUBLE][K], ints [CI_NINT][K], iter[nrb], rb_done[nrb],
+    // counters[nrb], partials[nrb][12][8*CUs]
+    const size_t pst = 12 * (size_t)(8 * cus());
+    const size_t bytes = sizeof(double) * CD_NDOUBLE * K + sizeof(int) * (CI_NINT * K + 2 * nrb_) +
+                         sizeof(unsigned) * nrb_ + 256 + sizeof(double) * pst * nrb_;
+    SEDI_HIP(ctl_.reserve(bytes));
+    SEDI_HIP(hipMemset(ctl_.p, 0, bytes));
+    return SEDI_OK;
+}
+
+template <class T>
+int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
+    SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
+    const long long n = shape_.n();
+    const int K = K_;
+    if (hist_cap_ < std::max(maxiter, 1)) {
+        SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)K * std::max(maxiter, 1)));
+        hist_cap_ = std::max(maxiter, 1);
+    }
+    BCtl c;
+    double* d = ctl_.as<double>();
+    c.bb = d + CD_BB * K; c.ru = d + CD_RU * K; c.ru_old = d + CD_RUOLD * K;
+    c.alpha = d + CD_ALPHA * K; c.beta = d + CD_BETA * K; c.err = d + CD_ERR * K; c.ferr = d + CD_FERR * K;
+    int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
+    c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
+    c.fold = ip + CI_FOLD * K;
+    c.iter = ip + CI_NINT * K;
+    c.rb_done = c.iter + nrb_;
+    c.counters = reinterpret_cast<unsigned*>(c.rb_done + nrb_);
+    char* pp = reinterpret_cast<char*>(c.counters + nrb_);
+    pp += (256 - (reinterpret_cast<uintptr_t>(pp) & 255)) & 255;
+    c.partials = reinterpret_cast<double*>(pp);
+    c.pstride = 12LL * 8 * cus();
+    c.hist = hist_.as<double>();
+    c.hstride = hist_cap_;
+    c.tol = tol;
+    c.maxiter = maxiter;
+    c.rep_every = rep_every_;
+    c.mixed = fp32_ ? 1 : 0;
+    c.K = K;
+
+    const T a = (T)a_, b = (T)b_c_, cc = (T)c_, im = (T)(1.0 / a_);
+    V4<T>* B = static_cast<V4<T>*>(b_);
+    V4<T>* R = static_cast<V4<T>*>(r_);
+    V4<T>* P0 = static_cast<V4<T>*>(p_[0]);
+    V4<T>* P1 = static_cast<V4<T>*>(p_[1]);
+    V4<T>* XL = static_cast<V4<T>*>(xlo_);
+
+    const unsigned ginit = ew_blocks(n);
+    SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ginit, nrb_), dim3(NTHREADS), stream_, n,
+                    c, (const V4<T>*)B, R, XL, xhi_, im));
+    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
+    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
+    const KGeom g3 = bgeom(batch_rep_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
+    const int chunk = 16;
+    int launched = 0, slot = 0;
+    bool pending = false;
+    hipEvent_t ev[2];
+    SEDI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    SEDI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int rc = SEDI_OK;
+    while (launched < maxiter && rc == SEDI_OK) {
+        const int cnt = std::min(chunk, maxiter - launched);
+        for (int k = 0; k < cnt; k++) {
+            hipError_t e = SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1,
+                                           dim3(g1.nblocks, nrb_), stream_, g1.tg, c, a, b, cc, n,
+                                           (const V4<T>*)R, P0, P1, im);
+            if (e == hipSuccess)
+                e = SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, nrb_),
+                                    stream_, g2.tg, c, a, b, cc, n, (const V4<T>*)P0, (const V4<T>*)P1, XL,
+                                    R, im);
+            if (e == hipSuccess && fp32_)
+                e = SEDI_BZC_LAUNCH("batch_rep_kernel", batch_rep_kernel, T, g3, dim3(g3.nblocks, nrb_),
+                                    stream_, g3.tg, c, a_, b_c_, c_, n, (const double*)xhi_,
+                                    (const V4<T>*)XL, R, (const V4<T>*)B, im);
+            if (e == hipSuccess && fp32_)
+                e = launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ginit, nrb_), dim3(NTHREADS),
+                           stream_, n, c, xhi_, XL);
+            if (e != hipSuccess) {
+                set_error("batched launch failed: %s", hipGetErrorString(e));
+                rc = SEDI_ERR_HIP;
+                break;
+            }
+        }
+        if (rc) break;
+        launched += cnt;
+        if (hipMemcpyAsync(pinned_ + slot * nrb_, c.rb_done, sizeof(int) * nrb_, hipMemcpyDeviceToHost,
+                           stream_) != hipSuccess ||
+            hipEventRecord(ev[slot], stream_) != hipSuccess) {
+            set_error("batched status copy failed");
+            rc = SEDI_ERR_HIP;
+            break;
+        }
+        if (pending) {
+            (void)hipEventSynchronize(ev[slot ^ 1]);
+            bool all = true;
+            for (int q = 0; q < nrb_; q++) all = all && pinned_[(slot ^ 1) * nrb_ + q];
+            if (all) break;
+        }
+        pending = true;
+        slot ^= 1;
+    }
+    (void)hipStreamSynchronize(stream_);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    profile_flush();
+    if (rc) return rc;
+    if (res) {
+        res->iters.assign(K, 0);
+        res->codes.assign(K, 0);
+        res->errs.assign(K, 0.0);
+        std::vector<int> done(K);
+        SEDI_HIP(hipMemcpy(res->iters.data(), c.fiter, sizeof(int) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(res->codes.data(), c.code, sizeof(int) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(res->errs.data(), c.ferr, sizeof(double) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(done.data(), c.done, sizeof(int) * K, hipMemcpyDeviceToHost));
+        for (int k = 0; k < K; k++)
+            SEDI_REQUIRE(done[k], SEDI_ERR_STATE, "batched solve stopped without a result for RHS %d", k);
+    }
+    hist_host_.resize((size_t)K * hist_cap_);
+    SEDI_HIP(hipMemcpy(hist_host_.data(), hist_.p, sizeof(double) * hist_host_.size(), hipMemcpyDeviceToHost));
+    return SEDI_OK;
+}
+
+int BatchSolver::solve(int maxiter, double tol, BatchResult* res) {
+    return fp32_ ? solve_t<float>(maxiter, tol, res) : solve_t<double>(maxiter, tol, res);
+}
+
+template <class T>
+int BatchSolver::accumulate_t(double* t, double* q, int k_active) {
+    const long long n = shape_.n();
+    long long g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n,
+                    k_active, (const V4<T>*)xlo_, (const double*)xhi_, (const V4<T>*)b_, t, q));
+    return SEDI_OK;
+}
+
+int BatchSolver::accumulate(double* t, double* q, int k_active) {
+    return fp32_ ? accumulate_t<float>(t, q, k_active) : accumulate_t<double>(t, q, k_active);
+}
+
+template <class T>
+int BatchSolver::pack_t(const double* b, int kact) {
+    const long long n = shape_.n();
+    SEDI_HIP(launch("batch_pack_kernel", batch_pack_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
+                    stream_, n, kact, b, static_cast<V4<T>*>(b_)));
+    return SEDI_OK;
+}
+
+template <class T>
+int BatchSolver::unpack_t(double* x, int kact) {
+    const long long n = shape_.n();
+    SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
+                    stream_, n, kact, static_cast<const V4<T>*>(xlo_), (const double*)xhi_, x));
+    return SEDI_OK;
+}
+
+int BatchSolver::load_rhs(const double* b, int kact) {
+    SEDI_REQUIRE(kact >= 0 && kact <= K_, SEDI_ERR_ARG, "%d right-hand sides for a batch of %d", kact, K_);
+    return fp32_ ? pack_t<float>(b, kact) : pack_t<double>(b, kact);
+}
+
+int BatchSolver::store_x(double* x, int kact) {
+    SEDI_REQUIRE(kact >= 0 && kact <= K_, SEDI_ERR_ARG, "%d solutions of a batch of %d", kact, K_);
+    return fp32_ ? unpack_t<float>(x, kact) : unpack_t<double>(x, kact);
+}
+
+std::vector<double> BatchSolver::history(int k) const {
+    std::vector<double> h;
+    if (k < 0 || k >= K_ || hist_host_.empty()) return h;
+    h.assign(hist_host_.begin() + (size_t)k * hist_cap_, hist_host_.begin() + (size_t)(k + 1) * hist_cap_);
+    return h;
+}
+
+}  // namespace sedi

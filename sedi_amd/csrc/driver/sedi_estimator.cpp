@@ -14,6 +14,7 @@
 //   -r SEED   mt19937 seed of the probe stream (default 1234)
 //   -P PREC   fp64 (default) or mixed
 //   -k K      probes solved together (default 1)
+//   -R M      mixed precision: FP64 residual replacement every M iterations (default 50)
 //   -o FILE   write D = t./q as raw little-endian float64, i1 fastest
 // Exit status: 0, 1 on usage errors, 100+code when a solve fails without -i
 // (the MPI_Abort code of stochastic-estimator.cc:183), 2 on a runtime error.
@@ -43,13 +44,14 @@ struct Options {
     uint32_t seed = 1234;
     int precision = SEDI_FP64;
     int batch = 1;
+    int rep_every = 0;
 };
 
 const char* progname = "sedi-estimator";
 
 [[noreturn]] void usage() {
     fprintf(stderr, "Usage: %s -g NxNxN -p NxNxN [-s NSAMPLES] [-e TOL] [-m MAXITER] [-i] [-f MODE] [-F]"
-                    " [-V HDF5_FILE] [-E HDF5_FILE] [-S SOLVER] [-r SEED] [-P fp64|mixed] [-k K] [-o FILE]\n",
+                    " [-V HDF5_FILE] [-E HDF5_FILE] [-S SOLVER] [-r SEED] [-P fp64|mixed] [-k K] [-R M] [-o FILE]\n",
             progname);
     fprintf(stderr, "\t-g NxNxN : size of the global 3D grid\n");
     fprintf(stderr, "\t-p NxNxN : decomposition into ranks (logical on one GPU)\n");
@@ -65,6 +67,7 @@ const char* progname = "sedi-estimator";
     fprintf(stderr, "\t-r SEED : seed of the mt19937 probe stream (default 1234)\n");
     fprintf(stderr, "\t-P PREC : fp64 (default) or mixed (FP32 vectors, FP64 reductions)\n");
     fprintf(stderr, "\t-k K : probes solved together (default 1)\n");
+    fprintf(stderr, "\t-R M : mixed precision: FP64 residual replacement every M iterations (default 50)\n");
     fprintf(stderr, "\t-o FILE : write the estimated diagonal as raw float64\n");
     exit(1);
 }
@@ -73,7 +76,7 @@ Options parse(int argc, char** argv) {
     Options o;
     const char *g = nullptr, *p = nullptr;
     int c;
-    while ((c = getopt(argc, argv, "g:p:s:e:m:if:FV:E:S:r:P:k:o:")) != -1) {
+    while ((c = getopt(argc, argv, "g:p:s:e:m:if:FV:E:S:r:P:k:R:o:")) != -1) {
         switch (c) {
         case 'g': g = optarg; break;
         case 'p': p = optarg; break;
@@ -93,6 +96,7 @@ Options parse(int argc, char** argv) {
             else usage();
             break;
         case 'k': if (sscanf(optarg, "%d", &o.batch) != 1 || o.batch < 1) usage(); break;
+        case 'R': if (sscanf(optarg, "%d", &o.rep_every) != 1 || o.rep_every < 0) usage(); break;
         case 'o': o.out = optarg; break;
         default: usage();
         }
@@ -143,6 +147,7 @@ int main(int argc, char** argv) {
     a.seed = o.seed;
     a.precision = o.precision;
     a.batch_k = o.batch;
+    a.rep_every = o.rep_every;
 
     sedi_estimator* est = nullptr;
     int rc = sedi_estimator_create(&a, &est);

@@ -3,6 +3,9 @@
 
 #include <vector>
 
+#include <memory>
+
+#include "batched.h"
 #include "plain.h"
 #include "rng.h"
 
@@ -19,6 +22,11 @@ class Estimator {
 
   private:
     void log(const char* line);
+    void log_history(int sample, const std::vector<double>& hist, int nh);
+    int run_batched(double* D, int* iters, double* errs, int* codes, double* seconds);
+    bool batched_ = false;
+    std::unique_ptr<BatchSolver> batch_;
+    DevBuf scratch_;
     sedi_estimate_args args_{};
     Shape shape_{};
     PlainSolver solver_;

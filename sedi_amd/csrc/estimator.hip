@@ -7,6 +7,7 @@
 #include "estimator.h"
 #include "profiler.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 
@@ -66,14 +67,30 @@ int Estimator::create(const sedi_estimate_args& a) {
     SEDI_REQUIRE(a.grid.n1 < (1 << 30) && a.grid.n2 < (1 << 30) && a.grid.n3 < (1 << 30), SEDI_ERR_ARG,
                  "grid dimension too large");
     SEDI_REQUIRE(a.samples >= 0 && a.maxiter >= 0, SEDI_ERR_ARG, "samples and maxiter must be >= 0");
-    SEDI_REQUIRE(a.precision == SEDI_FP64, SEDI_ERR_UNSUPPORTED,
-                 "precision %d is not available in the plain estimator", a.precision);
-    SEDI_REQUIRE(a.batch_k <= 1, SEDI_ERR_UNSUPPORTED, "batched estimation is not available here");
+    SEDI_REQUIRE(a.precision == SEDI_FP64 || a.precision == SEDI_FP32_MIXED, SEDI_ERR_ARG,
+                 "unknown precision %d", a.precision);
+    SEDI_REQUIRE(a.rep_every >= 0, SEDI_ERR_ARG, "rep_every must be >= 0");
     args_ = a;
     shape_ = Shape{(int)a.grid.n1, (int)a.grid.n2, (int)a.grid.n3};
+    const size_t n = (size_t)shape_.n();
+    batched_ = a.batch_k > 1 || a.precision == SEDI_FP32_MIXED;
+    if (batched_) {
+        SEDI_REQUIRE(a.kind == SEDI_CG, SEDI_ERR_UNSUPPORTED,
+                     "the batched and mixed-precision estimator runs CG only");
+        // blocks of 4 probes, no more than the run needs
+        int K = std::max(4, (a.batch_k + 3) / 4 * 4);
+        K = std::min(K, std::max(4, (a.samples + 3) / 4 * 4));
+        batch_.reset(new BatchSolver());
+        int rc = batch_->create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, K,
+                                a.precision == SEDI_FP32_MIXED, a.rep_every);
+        if (rc) return rc;
+        SEDI_HIP(buf_.reserve(sizeof(double) * n * 2));
+        t_ = buf_.as<double>();
+        q_ = t_ + n;
+        return SEDI_OK;
+    }
     int rc = solver_.create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, a.kind, a.mode);
     if (rc) return rc;
-    const size_t n = (size_t)shape_.n();
     SEDI_HIP(buf_.reserve(sizeof(double) * n * 5));
     v_[0] = buf_.as<double>();
     v_[1] = v_[0] + n;
@@ -92,7 +109,78 @@ void Estimator::log(const char* line) {
     if (log_fn_) log_fn_(log_user_, line);
 }
 
+void Estimator::log_history(int sample, const std::vector<double>& hist, int nh) {
+    char line[160];
+    snprintf(line, sizeof(line), "Solving for sample %d/%d...", sample + 1, args_.samples);
+    log(line);
+    for (int i = 0; i < nh && i < (int)hist.size(); i++) {
+        snprintf(line, sizeof(line), "%s: after iter=%d, err=%.10e", solver_name(args_.kind), i + 1, hist[i]);
+        log(line);
+    }
+}
+
+// Probes in batches of K (the last one padded with zero right-hand sides);
+// each batch is generated, solved together and accumulated in probe order, so
+// t and q see the same sequence of additions as the one-at-a-time loop.
+int Estimator::run_batched(double* D, int* iters, double* errs, int* codes, double* seconds) {
+    const long long n = shape_.n();
+    BatchSolver& bs = *batch_;
+    hipStream_t s = bs.stream();
+    int rc = fill_dev(t_, n, 0.0, s);
+    if (!rc) rc = fill_dev(q_, n, 0.0, s);
+    if (rc) return rc;
+    SEDI_HIP(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int K = bs.K();
+    int status = 0;
+    char line[160];
+    BatchResult res;
+    for (int k0 = 0; k0 < args_.samples && !status; k0 += K) {
+        const int kact = std::min(K, args_.samples - k0);
+        if ((rc = fill_probe_batch(args_.seed, n, k0, K, kact, bs.fp32(), bs.rhs(), s, scratch_))) return rc;
+        if ((rc = bs.solve(args_.maxiter, args_.tol, &res))) return rc;
+        int kuse = kact;
+        for (int k = 0; k < kact; k++) {
+            const int code = res.codes[k];
+            if (log_fn_) {
+                const int nh = code == 0 || code == 1 ? std::min(res.iters[k], args_.maxiter) : 0;
+                log_history(k0 + k, bs.history(k), nh);
+            }
+            if (iters) iters[k0 + k] = res.iters[k];
+            if (errs) errs[k0 + k] = res.errs[k];
+            if (codes) codes[k0 + k] = code;
+            if (!args_.ignore_errors && code != 0) {  // stochastic-estimator.cc:181-184
+                snprintf(line, sizeof(line), "Incomplete convergence in solver, errcode=%d", code);
+                log(line);
+                status = 100 + code;
+                kuse = k;
+                break;
+            }
+        }
+        if (kuse > 0 && (rc = bs.accumulate(t_, q_, kuse))) return rc;
+    }
+    if (D) {
+        double* dd = D;
+        const bool host = !is_device_pointer(D);
+        DevBuf tmp;
+        if (host) {
+            SEDI_HIP(tmp.reserve(sizeof(double) * n));
+            dd = tmp.as<double>();
+        }
+        SEDI_HIP(launch("divide_kernel", divide_kernel, ew_grid(n), dim3(256), s, (const double*)t_,
+                        (const double*)q_, dd, n));
+        SEDI_HIP(hipStreamSynchronize(s));
+        if (host) SEDI_HIP(hipMemcpy(D, dd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    SEDI_HIP(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    profile_flush();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return status;
+}
+
 int Estimator::run(double* D, int* iters, double* errs, int* codes, double* seconds) {
+    if (batched_) return run_batched(D, iters, errs, codes, seconds);
     const long long n = shape_.n();
     hipStream_t s = solver_.stream();
     // set-up outside the timed scope: t = q = 0, stream seeded, first probe

@@ -39,6 +39,7 @@
 #include "sedi_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace sedi {
 
@@ -207,30 +208,32 @@ __device__ __forceinline__ long long tile_of(const TileGrid& g, long long t, boo
     return reverse ? g.ntiles - 1 - t : t;
 }
 
-// ---- byte-offset memory access ---------------------------------------------
-// Vectors of the plain path hold fewer than 2^29 cells (checked on the host),
-// so a cell's byte offset fits in 32 bits; loads then take the vector's base
+// ---- 32-bit cell offsets ---------------------------------------------------
+// The march hands passes a cell's index within one vector as 32 bits; every
+// vector a pass touches is smaller than 4 GiB (checked on the host), so the
+// byte offset index*sizeof(T) is also 32-bit and loads take the vector's base
 // from scalar registers plus one 32-bit offset register, with no 64-bit
 // address arithmetic per access.
-using boff_t = unsigned;
+using cidx_t = unsigned;
 
 template <class T>
-__device__ __forceinline__ T ldb(const T* base, boff_t bo) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bo);
+__device__ __forceinline__ T ldc(const T* base, cidx_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)(i * (unsigned)sizeof(T)));
 }
 template <class T>
-__device__ __forceinline__ void stb(T* base, boff_t bo, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + bo) = v;
+__device__ __forceinline__ void stc(T* base, cidx_t i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(T))) = v;
 }
 
 // ---- the march -------------------------------------------------------------
-// Pass interface (all offsets are byte offsets of FP64 cells):
+// Pass interface (offsets are cell indices; the field f has type P::V, double
+// for the plain solvers and a 4-RHS vector for the batched ones):
 //   static constexpr int ND;                        number of dot products
 //   static constexpr int D;                         prefetch distance (planes)
-//   struct Raw;   Raw load(boff_t) const;           what f needs at a cell
+//   struct Raw;   Raw load(cidx_t) const;           what f needs at a cell
 //   double f(const Raw&) const;                     the stencil input field
-//   struct Aux;   Aux load_aux(boff_t) const;       other inputs of the update
-//   void finish(boff_t, const Raw&, const Aux&, double f, double Af,
+//   struct Aux;   Aux load_aux(cidx_t) const;       other inputs of the update
+//   void finish(cidx_t, const Raw&, const Aux&, V f, V Af,
 //               double (&acc)[ND]) const;           pointwise update at a cell
 // and the stencil coefficients a, b, c.
 //
@@ -240,10 +243,19 @@ __device__ __forceinline__ void stb(T* base, boff_t bo, T v) {
 // counts keep D planes of loads in flight all the way through.  A tile that
 // runs past the last plane still executes every step (clamped loads, zeroed
 // values) but does not store.
+template <class P, class = void>
+struct value_of { using type = double; };
+template <class P>
+struct value_of<P, std::void_t<typename P::V>> { using type = typename P::V; };
+template <class P>
+using value_t = typename value_of<P>::type;
+
 template <class P, int ZC>
 __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long long tile,
-                                           double (&lds)[2][TY + 2][TX + 2],
+                                           value_t<P> (&lds)[2][TY + 2][TX + 2],
                                            double (&acc)[P::ND]) {
+    using V = value_t<P>;
+    const V zero{};
     constexpr int D = P::D;
     constexpr int R = D + 1;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -255,14 +267,14 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     const int i0 = bx * TX, j0 = by * TY;
     const int zs = bz * ZC;
     const int n3 = g.n3;
-    const boff_t s3b = (boff_t)(g.s3 * 8);
+    const cidx_t s3 = (cidx_t)g.s3;
 
     const int i = i0 + lane, j = j0 + w;
     const bool c_in = (i < g.n1) && (j < g.n2);
     // every load below is unconditional from a clamped, in-grid address and
     // out-of-grid values are replaced by 0 afterwards: the body is straight-line
     // code, so the compiler's wait counts on the prefetch ring stay exact
-    const boff_t cxy = (boff_t)(min(i, g.n1 - 1) + min(j, g.n2 - 1) * g.n1) * 8u;
+    const cidx_t cxy = (cidx_t)(min(i, g.n1 - 1) + min(j, g.n2 - 1) * g.n1);
 
     // the one extra (halo) cell this lane evaluates per plane
     int ex_i = i, ex_j = j, ex_r = 0, ex_c = 0;
@@ -279,15 +291,14 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
         ex_i = i0 + TX; ex_r = w + 1; ex_c = TX + 1; ex_act = true;
     }
     const bool ex_in = ex_act && ex_i >= 0 && ex_i < g.n1 && ex_j >= 0 && ex_j < g.n2;
-    const boff_t exy =
-        (boff_t)(min(max(ex_i, 0), g.n1 - 1) + min(max(ex_j, 0), g.n2 - 1) * g.n1) * 8u;
+    const cidx_t exy = (cidx_t)(min(max(ex_i, 0), g.n1 - 1) + min(max(ex_j, 0), g.n2 - 1) * g.n1);
     if (!ex_act) { ex_r = w + 1; ex_c = lane + 1; }  // harmless duplicate of the centre slot
 
     // rings: f-inputs of plane zs+q-1 live in slot q % R, update inputs of
     // plane zs+q in slot q % R
     typename P::Raw rc[R], rx[R];
     typename P::Aux ra[R];
-    auto zoff = [&](int zz) { return (boff_t)min(max(zz, 0), n3 - 1) * s3b; };
+    auto zoff = [&](int zz) { return (cidx_t)min(max(zz, 0), n3 - 1) * s3; };
     auto ld_f = [&](int zz, int slot) {
         rc[slot] = p.load(cxy + zoff(zz));
         rx[slot] = p.load(exy + zoff(zz));
@@ -302,10 +313,10 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     for (int k = 1; k < D && k <= ZC; k++) ld_f(zs + k, (k + 1) % R);
 #pragma unroll
     for (int k = 0; k < D && k < ZC; k++) ld_a(zs + k, k);
-    double fm = (c_in && zs >= 1) ? p.f(rc[0]) : 0.0;
-    double fc = (c_in && zs < n3) ? p.f(rc[1 % R]) : 0.0;
+    V fm = (c_in && zs >= 1) ? p.f(rc[0]) : zero;
+    V fc = (c_in && zs < n3) ? p.f(rc[1 % R]) : zero;
     typename P::Raw cc = rc[1 % R];
-    const double fe0 = (ex_in && zs < n3) ? p.f(rx[1 % R]) : 0.0;
+    const V fe0 = (ex_in && zs < n3) ? p.f(rx[1 % R]) : zero;
     lds[0][w + 1][lane + 1] = fc;
     if (ex_act) lds[0][ex_r][ex_c] = fe0;
     if (D <= ZC) ld_f(zs + D, (D + 1) % R);
@@ -321,19 +332,19 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
         // f on plane z+1 (centre and halo) into the other LDS buffer
         const bool nxt = z + 1 < n3;
         const typename P::Raw cn = rc[(u + 2) % R];
-        const double fn = (c_in && nxt) ? p.f(cn) : 0.0;
+        const V fn = (c_in && nxt) ? p.f(cn) : zero;
         if (u + 1 < ZC) {
-            const double fe = (ex_in && nxt) ? p.f(rx[(u + 2) % R]) : 0.0;
+            const V fe = (ex_in && nxt) ? p.f(rx[(u + 2) % R]) : zero;
             lds[b ^ 1][w + 1][lane + 1] = fn;
             if (ex_act) lds[b ^ 1][ex_r][ex_c] = fe;
         }
-        const double xm = lds[b][w + 1][lane];
-        const double xp = lds[b][w + 1][lane + 2];
-        const double ym = lds[b][w][lane + 1];
-        const double yp = lds[b][w + 2][lane + 1];
-        const double Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
-                          - p.b * xp - p.b * yp - p.b * fn;
-        if (c_in && z < n3) p.finish(cxy + (boff_t)z * s3b, cc, ra[u % R], fc, Af, acc);
+        const V xm = lds[b][w + 1][lane];
+        const V xp = lds[b][w + 1][lane + 2];
+        const V ym = lds[b][w][lane + 1];
+        const V yp = lds[b][w + 2][lane + 1];
+        const V Af = p.a * fc - p.c * xm - p.c * ym - p.c * fm
+                     - p.b * xp - p.b * yp - p.b * fn;
+        if (c_in && z < n3) p.finish(cxy + (cidx_t)z * s3, cc, ra[u % R], fc, Af, acc);
         lds_barrier();
         fm = fc;
         fc = fn;
@@ -344,7 +355,7 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
 // Grid-stride loop over the tiles of a pass (tile order per tile_of).
 template <class P, int ZC>
 __device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
-                                          double (&lds)[2][TY + 2][TX + 2],
+                                          value_t<P> (&lds)[2][TY + 2][TX + 2],
                                           double (&acc)[P::ND], bool reverse = false) {
 #pragma unroll
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;

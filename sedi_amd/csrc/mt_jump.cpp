@@ -140,6 +140,33 @@ void square_mod(Poly& p, const Poly& phi) {
     p.swap(sq);
 }
 
+// carry-less 64x64 -> 128 multiply (PCLMULQDQ on the host)
+__attribute__((target("pclmul,sse2"))) inline void clmul64(uint64_t a, uint64_t b, uint64_t* lo,
+                                                           uint64_t* hi) {
+    typedef long long v2di __attribute__((vector_size(16)));
+    v2di va = {(long long)a, 0}, vb = {(long long)b, 0};
+    v2di r = __builtin_ia32_pclmulqdq128(va, vb, 0x00);
+    *lo = (uint64_t)r[0];
+    *hi = (uint64_t)r[1];
+}
+
+// r = a * b mod phi
+Poly mul_mod(const Poly& a, const Poly& b, const Poly& phi) {
+    Poly prod(2 * W + 2, 0);
+    for (int i = 0; i < W; i++) {
+        if (!a[i]) continue;
+        for (int j = 0; j < W; j++) {
+            if (!b[j]) continue;
+            uint64_t lo, hi;
+            clmul64(a[i], b[j], &lo, &hi);
+            prod[i + j] ^= lo;
+            prod[i + j + 1] ^= hi;
+        }
+    }
+    reduce(prod, phi);
+    return prod;
+}
+
 void mulx_mod(Poly& p, const Poly& phi) {
     uint64_t carry = 0;
     for (int k = 0; k < W; k++) {
@@ -173,6 +200,35 @@ const std::vector<uint64_t>& mt_jump_poly(unsigned long long J) {
         if ((J >> b) & 1ull) mulx_mod(r, phi);
     }
     return cache.emplace(J, std::move(r)).first->second;
+}
+
+const std::vector<uint64_t>& mt_probe_poly(long long ncells, int k) {
+    static std::mutex mu;
+    static std::map<std::pair<long long, int>, Poly> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find({ncells, k});
+        if (it != cache.end()) return it->second;
+    }
+    Poly r;
+    if (k == 0) {
+        r.assign(W, 0);
+        r[0] = 1;
+    } else {
+        const Poly& step = mt_jump_poly(2ull * (unsigned long long)ncells);
+        const Poly& prev = mt_probe_poly(ncells, k - 1);
+        r = mul_mod(prev, step, charpoly());
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    return cache.emplace(std::make_pair(ncells, k), std::move(r)).first->second;
+}
+
+int mt_poly_words() { return W; }
+
+void mt_word_sequence(const uint32_t* s, uint32_t* w, int count) {
+    std::vector<uint32_t> v;
+    word_sequence(s, v, (size_t)count);
+    std::memcpy(w, v.data(), sizeof(uint32_t) * count);
 }
 
 void mt_jump_state(const uint32_t* s, unsigned long long J, uint32_t* out) {

@@ -13,5 +13,15 @@ const std::vector<uint64_t>& mt_jump_poly(unsigned long long J);
 // out = the state J outputs after s (both oldest word first; out may not alias s).
 void mt_jump_state(const uint32_t* s, unsigned long long J, uint32_t* out);
 bool mt_charpoly_ok();
+// x^(2*ncells*k) mod phi: the jump from a stream's start to probe k of a grid
+// of ncells cells (two outputs per cell).  Cached; independent of the seed.
+const std::vector<uint64_t>& mt_probe_poly(long long ncells, int k);
+// Words per cached polynomial (19937 bits rounded up, plus one).
+int mt_poly_words();
+// Untempered word sequence w_0 .. w_{count-1} from state s (w_0..w_623 = s).
+void mt_word_sequence(const uint32_t* s, uint32_t* w, int count);
+// Degree of the characteristic polynomial and the words of one jump window.
+constexpr int MT_DEG = 19937;
+constexpr int MT_WINDOW = MT_DEG + 624;
 
 }  // namespace sedi

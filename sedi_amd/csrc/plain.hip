@@ -88,11 +88,11 @@ struct PassApply {
     double* y;
     struct Raw { double x; };
     struct Aux {};
-    __device__ Raw load(boff_t i) const { return {ldb(x, i)}; }
-    __device__ Aux load_aux(boff_t) const { return {}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(x, i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
     __device__ double f(const Raw& r) const { return r.x; }
-    __device__ void finish(boff_t i, const Raw&, const Aux&, double, double Af, double (&)[ND]) const {
-        stb(y, i, Af);
+    __device__ void finish(cidx_t i, const Raw&, const Aux&, double, double Af, double (&)[ND]) const {
+        stc(y, i, Af);
     }
 };
 
@@ -110,13 +110,13 @@ struct PassResidual {
     PC iM;
     struct Raw { double x; };
     struct Aux { double b, im; };
-    __device__ Raw load(boff_t i) const { return {ldb(x, i)}; }
-    __device__ Aux load_aux(boff_t i) const { return {ldb(rhs, i), iM.at(i)}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(x, i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(rhs, i), iM.at(i)}; }
     __device__ double f(const Raw& r) const { return r.x; }
-    __device__ void finish(boff_t i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
+    __device__ void finish(cidx_t i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
         const double rv = 1.0 * q.b + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
-        stb(r, i, rv);
-        if (rhat) stb(rhat, i, rv);
+        stc(r, i, rv);
+        if (rhat) stc(rhat, i, rv);
         acc[0] += q.b * q.b;
         acc[1] += rv * rv;
         acc[2] += rv * (rv * q.im);
@@ -137,14 +137,14 @@ struct PassCgDirection {
     bool first;
     struct Raw { double r, p, im; };
     struct Aux {};
-    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(pold, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t) const { return {}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
     __device__ double f(const Raw& q) const {
         const double u = q.r * q.im;
         return first ? u : 1.0 * u + beta * q.p;
     }
-    __device__ void finish(boff_t i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
-        stb(pnew, i, p);
+    __device__ void finish(cidx_t i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
+        stc(pnew, i, p);
         acc[0] += s * p;
     }
 };
@@ -162,14 +162,14 @@ struct PassCgUpdate {
     double alpha;
     struct Raw { double p; };
     struct Aux { double x, r, im; };
-    __device__ Raw load(boff_t i) const { return {ldb(p, i)}; }
-    __device__ Aux load_aux(boff_t i) const { return {ldb(x, i), ldb(r, i), iM.at(i)}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(x, i), ldc(r, i), iM.at(i)}; }
     __device__ double f(const Raw& q) const { return q.p; }
-    __device__ void finish(boff_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
+    __device__ void finish(cidx_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
         const double xv = 1.0 * q.x + alpha * pv;
         const double rv = 1.0 * q.r + (-alpha) * s;
-        stb(x, i, xv);
-        stb(r, i, rv);
+        stc(x, i, xv);
+        stc(r, i, rv);
         const double u = rv * q.im;
         acc[0] += rv * u;
         acc[1] += rv * rv;
@@ -189,13 +189,13 @@ struct PassPcgInit {
     PC iM;
     struct Raw { double r, im; };
     struct Aux {};
-    __device__ Raw load(boff_t i) const { return {ldb(r, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t) const { return {}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
     __device__ double f(const Raw& c) const { return c.r * c.im; }
-    __device__ void finish(boff_t i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
-        stb(u, i, uv);
-        stb(w, i, wv);
-        stb(z, i, 0.0); stb(q, i, 0.0); stb(s, i, 0.0); stb(p, i, 0.0);
+    __device__ void finish(cidx_t i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
+        stc(u, i, uv);
+        stc(w, i, wv);
+        stc(z, i, 0.0); stc(q, i, 0.0); stc(s, i, 0.0); stc(p, i, 0.0);
         acc[0] += c.r * uv;
         acc[1] += wv * uv;
         acc[2] += c.r * c.r;
@@ -215,10 +215,10 @@ struct PassPcgIter {
     double alpha, beta;
     struct Raw { double w, im; };
     struct Aux { double q, s, u, p, z, r, x; };
-    __device__ Raw load(boff_t i) const { return {ldb(wold, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t i) const { return {ldb(q, i), ldb(s, i), ldb(u, i), ldb(p, i), ldb(z, i), ldb(r, i), ldb(x, i)}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(wold, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(q, i), ldc(s, i), ldc(u, i), ldc(p, i), ldc(z, i), ldc(r, i), ldc(x, i)}; }
     __device__ double f(const Raw& c) const { return c.w * c.im; }
-    __device__ void finish(boff_t i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
+    __device__ void finish(cidx_t i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
         const double qv = 1.0 * m + beta * o.q;
         const double sv = 1.0 * c.w + beta * o.s;
         const double pv = 1.0 * o.u + beta * o.p;
@@ -227,8 +227,8 @@ struct PassPcgIter {
         const double xv = 1.0 * o.x + alpha * pv;
         const double uv = 1.0 * o.u + (-alpha) * qv;
         const double wv = 1.0 * c.w + (-alpha) * zv;
-        stb(q, i, qv); stb(s, i, sv); stb(p, i, pv); stb(z, i, zv);
-        stb(r, i, rv); stb(x, i, xv); stb(u, i, uv); stb(wnew, i, wv);
+        stc(q, i, qv); stc(s, i, sv); stc(p, i, pv); stc(z, i, zv);
+        stc(r, i, rv); stc(x, i, xv); stc(u, i, uv); stc(wnew, i, wv);
         acc[0] += rv * uv;
         acc[1] += wv * uv;
         acc[2] += rv * rv;
@@ -248,15 +248,15 @@ struct PassBicgDirection {
     bool first;
     struct Raw { double r, p, v, im; };
     struct Aux { double rhat; };
-    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(pold, i), ldb(vold, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t i) const { return {ldb(rhat, i)}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i), ldc(vold, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(rhat, i)}; }
     __device__ double pdir(const Raw& q) const {
         return first ? q.r : 1.0 * q.r + beta * q.p + cbw * q.v;  // combine3 (:169)
     }
     __device__ double f(const Raw& q) const { return pdir(q) * q.im; }
-    __device__ void finish(boff_t i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
-        stb(pnew, i, pdir(q));
-        stb(vnew, i, v);
+    __device__ void finish(cidx_t i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
+        stc(pnew, i, pdir(q));
+        stc(vnew, i, v);
         acc[0] += v * o.rhat;
     }
 };
@@ -272,11 +272,11 @@ struct PassBicgHalf {
     double alpha;
     struct Raw { double r, v, im; };
     struct Aux {};
-    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(v, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t) const { return {}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
     __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
     __device__ double f(const Raw& q) const { return sval(q) * q.im; }
-    __device__ void finish(boff_t, const Raw& q, const Aux&, double, double t, double (&acc)[ND]) const {
+    __device__ void finish(cidx_t, const Raw& q, const Aux&, double, double t, double (&acc)[ND]) const {
         const double sv = sval(q);
         acc[0] += sv * sv;
         acc[1] += t * sv;
@@ -297,17 +297,17 @@ struct PassBicgUpdate {
     double alpha, omega;
     struct Raw { double r, v, im; };
     struct Aux { double p, x, rhat; };
-    __device__ Raw load(boff_t i) const { return {ldb(r, i), ldb(v, i), iM.at(i)}; }
-    __device__ Aux load_aux(boff_t i) const { return {ldb(p, i), ldb(x, i), ldb(rhat, i)}; }
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(p, i), ldc(x, i), ldc(rhat, i)}; }
     __device__ double sval(const Raw& q) const { return 1.0 * q.r + (-alpha) * q.v; }
     __device__ double f(const Raw& q) const { return sval(q) * q.im; }
-    __device__ void finish(boff_t i, const Raw& q, const Aux& o, double sh, double t, double (&acc)[ND]) const {
+    __device__ void finish(cidx_t i, const Raw& q, const Aux& o, double sh, double t, double (&acc)[ND]) const {
         const double sv = sval(q);
         const double ph = o.p * q.im;
         const double xv = 1.0 * o.x + alpha * ph + omega * sh;  // combine3 (:217)
         const double rv = 1.0 * sv + (-omega) * t;
-        stb(x, i, xv);
-        stb(rnew, i, rv);
+        stc(x, i, xv);
+        stc(rnew, i, rv);
         acc[0] += rv * rv;
         acc[1] += rv * o.rhat;
     }
@@ -523,9 +523,9 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
         const long long n = (long long)g.n1 * g.n2 * g.n3;
         for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
              i += (long long)gridDim.x * NTHREADS) {
-            const boff_t bo = (boff_t)(i * 8);
-            const double ph = ldb(p, bo) * iM.at(bo);
-            stb(x, bo, 1.0 * ldb(x, bo) + alpha * ph);
+            const cidx_t c = (cidx_t)i;
+            const double ph = ldc(p, c) * iM.at(c);
+            stc(x, c, 1.0 * ldc(x, c) + alpha * ph);
         }
         acc[0] = acc[1] = 0.0;
         if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -651,7 +651,6 @@ int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
     DevState* st = reinterpret_cast<DevState*>(ctl_.p);
     Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const dim3 block(NTHREADS);
     const KGeom& kr = geom_for(residual_kernel<PC, 16>, shape_, PassResidual<PC>::D);
     if (kind_ == SEDI_BICGSTAB) {
         // r_1 lives in r[1] (buffers alternate with the iteration's parity)
@@ -674,7 +673,6 @@ int PlainSolver::launch_iterations(int count, PC pc, double* x) {
     Ctl ctl{reinterpret_cast<DevState*>(ctl_.p),
             reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const dim3 block(NTHREADS);
     for (int k = 0; k < count; k++) {
         if (kind_ == SEDI_CG) {
             const KGeom& g1 = geom_for(cg_direction_kernel<PC, 16>, shape_, PassCgDirection<PC>::D);

@@ -29,4 +29,14 @@ class MtStream {
     DevBuf state_;
 };
 
+// Probes k0 .. k0+kact-1 of the stream mt19937(seed) on a grid of ncells
+// cells, written as +-1 into the batch layout [K/4][ncells][4] (float or
+// double); slots kact .. K-1 are zeroed.  K is a multiple of 4.  Uses
+// `scratch` for the uploaded jump data; synchronises `stream`.
+int fill_probe_batch(uint32_t seed, long long ncells, int k0, int K, int kact, bool fp32, void* out,
+                     hipStream_t stream, DevBuf& scratch);
+
+// [K/4][ncells][4] FP64 batch layout -> [count][ncells] rows (count <= K).
+int unbatch_rows(const double* batch, long long ncells, int count, double* rows, hipStream_t stream);
+
 }  // namespace sedi

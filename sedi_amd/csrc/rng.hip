@@ -20,6 +20,9 @@
 #include "mt_jump.h"
 #include "profiler.h"
 
+#include <cstring>
+#include <vector>
+
 namespace sedi {
 
 namespace {
@@ -145,6 +148,133 @@ __global__ __launch_bounds__(256) void mt_fill_kernel(MtState* s, double* out, l
         s->carry_valid = carry_valid;
         s->carry = carry;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched probes: workgroup k produces probe k0+k of the stream.  Its start
+// state is the jump of the stream's start state by 2N(k0+k) outputs: with
+// q = x^(2N(k0+k)) mod phi (cached on the host per grid size) the jumped
+// state's word j is the XOR of w_{i+j} over the set bits i of q, where w is
+// the stream's untempered word sequence from its start (uploaded once per
+// seed).  The workgroup then regenerates and tempers like mt_fill_kernel and
+// writes its probe into lane k%4 of RHS block k/4 of `out` ([K/4][N][4] of T).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void mt_batch_kernel(const uint32_t* __restrict__ wseq,
+                                                       const uint64_t* __restrict__ polys, int pw,
+                                                       long long ncells, int kact,
+                                                       T* __restrict__ out) {
+    __shared__ uint32_t S[2][MT_N];
+    __shared__ uint32_t O[2][MT_N];
+    const int t = threadIdx.x;
+    const int k = blockIdx.x;
+    if (k >= kact) {  // padding: a zero right-hand side
+        T* z = out + (long long)(k >> 2) * ncells * 4 + (k & 3);
+        for (long long i = t; i < ncells; i += 256) z[i * 4] = (T)0;
+        return;
+    }
+    const uint64_t* q = polys + (long long)k * pw;
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    for (int wi = 0; wi < pw; wi++) {
+        uint64_t bits = q[wi];
+        while (bits) {
+            const int b = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            const uint32_t* src = wseq + wi * 64 + b;
+            a0 ^= src[t];
+            a1 ^= src[t + 256];
+            if (t < MT_N - 512) a2 ^= src[t + 512];
+        }
+    }
+    S[0][t] = a0;
+    S[0][t + 256] = a1;
+    if (t < MT_N - 512) S[0][t + 512] = a2;
+    __syncthreads();
+
+    T* dst = out + (long long)(k >> 2) * ncells * 4 + (k & 3);
+    int cur = 0;
+    long long done = 0;
+    while (done < ncells) {
+        // regenerate (as mt_fill_kernel)
+        const int nxt = cur ^ 1;
+        if (t < 227) {
+            const uint32_t v = twist(S[cur][t], S[cur][t + 1], S[cur][t + MT_M]);
+            S[nxt][t] = v;
+            O[nxt][t] = temper(v);
+        }
+        __syncthreads();
+        if (t < 227) {
+            const int i = t + 227;
+            const uint32_t v = twist(S[cur][i], S[cur][i + 1], S[nxt][i - 227]);
+            S[nxt][i] = v;
+            O[nxt][i] = temper(v);
+        }
+        __syncthreads();
+        if (t < 170) {
+            const int i = t + 454;
+            const uint32_t nx = (i + 1 < MT_N) ? S[cur][i + 1] : S[nxt][0];
+            const uint32_t v = twist(S[cur][i], nx, S[nxt][i - 227]);
+            S[nxt][i] = v;
+            O[nxt][i] = temper(v);
+        }
+        __syncthreads();
+        cur = nxt;
+        // 312 cells per block: the probe starts at an even output, so pairs
+        // never straddle two blocks
+        long long pairs = MT_N / 2;
+        if (pairs > ncells - done) pairs = ncells - done;
+        for (int p = t; p < pairs; p += 256)
+            dst[(done + p) * 4] = (T)rademacher(O[cur][2 * p], O[cur][2 * p + 1]);
+        done += pairs;
+    }
+}
+
+int fill_probe_batch(uint32_t seed, long long ncells, int k0, int K, int kact, bool fp32, void* out,
+                     hipStream_t stream, DevBuf& scratch) {
+    SEDI_REQUIRE(K > 0 && K % 4 == 0 && kact >= 0 && kact <= K && k0 >= 0, SEDI_ERR_ARG,
+                 "bad probe batch (k0 %d, K %d, active %d)", k0, K, kact);
+    const int pw = mt_poly_words();
+    const size_t wbytes = sizeof(uint32_t) * MT_WINDOW;
+    const size_t pbytes = sizeof(uint64_t) * (size_t)pw * K;
+    std::vector<uint8_t> host(wbytes + pbytes);
+    uint32_t s0[624];
+    mt_seed_state(seed, s0);
+    mt_word_sequence(s0, reinterpret_cast<uint32_t*>(host.data()), MT_WINDOW);
+    for (int k = 0; k < kact; k++) {
+        const std::vector<uint64_t>& q = mt_probe_poly(ncells, k0 + k);
+        std::memcpy(host.data() + wbytes + sizeof(uint64_t) * (size_t)pw * k, q.data(), sizeof(uint64_t) * pw);
+    }
+    SEDI_HIP(scratch.reserve(host.size()));
+    SEDI_HIP(hipMemcpyAsync(scratch.p, host.data(), host.size(), hipMemcpyHostToDevice, stream));
+    const uint32_t* wseq = scratch.as<uint32_t>();
+    const uint64_t* polys = reinterpret_cast<const uint64_t*>(static_cast<char*>(scratch.p) + wbytes);
+    if (fp32)
+        SEDI_HIP(launch("mt_batch_kernel", mt_batch_kernel<float>, dim3(K), dim3(256), stream, wseq, polys,
+                        pw, ncells, kact, static_cast<float*>(out)));
+    else
+        SEDI_HIP(launch("mt_batch_kernel", mt_batch_kernel<double>, dim3(K), dim3(256), stream, wseq, polys,
+                        pw, ncells, kact, static_cast<double*>(out)));
+    // the host staging buffer must outlive the asynchronous copy
+    SEDI_HIP(hipStreamSynchronize(stream));
+    return SEDI_OK;
+}
+
+__global__ void unbatch_rows_kernel(const double* __restrict__ batch, long long ncells, int count,
+                                    double* __restrict__ rows) {
+    const long long total = ncells * count;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long k = e / ncells, i = e - k * ncells;
+        rows[e] = batch[(k >> 2) * ncells * 4 + i * 4 + (k & 3)];
+    }
+}
+
+int unbatch_rows(const double* batch, long long ncells, int count, double* rows, hipStream_t stream) {
+    long long g = (ncells * count + 255) / 256;
+    if (g > 8192) g = 8192;
+    SEDI_HIP(launch("unbatch_rows_kernel", unbatch_rows_kernel, dim3((unsigned)g), dim3(256), stream, batch,
+                    ncells, count, rows));
+    return SEDI_OK;
 }
 
 int MtStream::init() {

@@ -43,15 +43,15 @@ struct Shape {
 // stochastic-estimator.cc:102-103), which kernels instantiated with
 // PrecondScalar keep in a scalar register; sedi_solve with an explicit iM
 // vector uses PrecondVec.
-// at() takes the cell's byte offset.
+// at() takes the cell's index.
 struct PrecondScalar {
     double s;
     __device__ __forceinline__ double at(unsigned) const { return s; }
 };
 struct PrecondVec {
     const double* v;
-    __device__ __forceinline__ double at(unsigned bo) const {
-        return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(v) + bo);
+    __device__ __forceinline__ double at(unsigned i) const {
+        return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(v) + (unsigned)(i * 8u));
     }
 };
 
