@@ -1,18 +1,29 @@
-"""Benchmark: probe-solves/s of the estimator on one MI355X.
+"""Benchmark: probe-solves/s of the stochastic diag(A^-1) estimator on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver cg|pcg|bicgstab]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch K] [--mode headline|plain]
 
-Workload (one step): the plain single-probe FP64 drop-in on a 256x256x256 grid
-(16,777,216 unknowns), 16 Rademacher probes from mt19937 seed 7, tol 1e-9,
-maxiter 2000: one sedi_estimator_run (probe generation + 16 solves +
-accumulation, the reference's TIMER_GLOBAL scope).  The probes are generated
-on the device, so inputs are resident in HBM when the timed region starts.
+Headline workload (one step): the batched mixed-precision CG estimator on a
+256x256x256 grid (16,777,216 unknowns): 64 Rademacher probes from mt19937
+seed 11 solved as one interleaved block of k = 64 right-hand sides (FP32
+vectors and stencil, FP64 dot products and scalars, FP64 residual replacement
+every 50 iterations), relative residual 1e-8, maxiter 2000: one
+sedi_estimator_run (probe generation + the block solve + accumulation, the
+reference's TIMER_GLOBAL scope).  Probes are generated on the device, so the
+inputs are resident in HBM when the timed region starts.
+
+Beside it, on the same grid, stream and tolerance: the plain FP64
+one-probe-at-a-time CG (the drop-in) on the first 16 probes, the speed-up of
+the batched form over it, and the max / RMS relative error of the mixed
+form's diagonal against the FP64 one on those 16 probes.
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (its algorithmic bytes per launch over its average duration, measured with HIP
 events on the solver's stream during the timed steps) and the CPU baseline:
 the reference's own solver code (oracle/_ref/sedi_ref, built from the
 reference's headers) on the host cores under mpirun, on a bounded sample.
+
+--mode plain measures the plain FP64 drop-in alone (256^3, 16 probes, seed 7,
+tol 1e-9, --solver cg|pcg|bicgstab).
 """
 import argparse
 import json
@@ -27,14 +38,19 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 GRID = (256, 256, 256)
-SAMPLES = 16
-SEED = 7
-TOL = 1e-9
 MAXITER = 2000
 NAMES = {"cg": 2, "pcg": 3, "bicgstab": 1}
 SOLVER_LABEL = {2: "CG", 3: "PipelinedCG", 1: "BiCGStab"}
-# dominant kernel per solver and the vectors it streams per launch (FP64 N cells)
+# plain FP64 drop-in workload (--mode plain)
+PLAIN = dict(samples=16, seed=7, tol=1e-9)
+# headline batched mixed workload and its FP64 comparison
+BATCH = dict(samples=64, seed=11, tol=1e-8, rep_every=50)
+CMP_SAMPLES = 16
+# dominant kernel per solver and the FP64 vectors it streams per launch
 DOMINANT = {2: ("cg_update_kernel", 5), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
+# batched mixed CG: batch_upd_kernel streams p, x_lo, r (read) and x_lo, r
+# (written): 5 FP32 values per right-hand side and cell
+BATCH_KERNEL, BATCH_BYTES_PER_RHS_CELL = "batch_upd_kernel", 5 * 4
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sedi_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
 
@@ -42,31 +58,39 @@ MPIRUN = "/opt/conda/bin/mpirun"
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--solver", default="cg", choices=sorted(NAMES))
+    p.add_argument("--mode", default="headline", choices=["headline", "plain"])
+    p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
+    p.add_argument("--solver", default="cg", choices=sorted(NAMES), help="--mode plain only")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-compare", action="store_true",
+                   help="headline: skip the plain FP64 comparison run")
     p.add_argument("--cpu-iters", type=int, default=300,
                    help="iterations of the bounded CPU-baseline sample")
     p.add_argument("--stencil", action="store_true",
                    help="time the standalone 7-point stencil kernel (y = A x, FP64) at 256^3 and exit")
     p.add_argument("--pmc-workload", action="store_true",
-                   help="run a short fixed workload for rocprofv3 --pmc (1 probe per solver, "
-                        "10 iterations each, at 256^3) and exit")
+                   help="run a short fixed workload for rocprofv3 --pmc (1 probe per plain solver "
+                        "and one batch of 64 mixed CG probes, 10 iterations each, at 256^3) and exit")
     return p.parse_args()
 
 
 def pmc_workload():
-    """One probe per solver, capped at 10 iterations (-m 10 -i), on the 256^3
-    grid: a few dozen dispatches of every kernel, including the estimator's
-    streaming accumulate_kernel (reads x, v, t, q; writes t, q), whose known
-    traffic calibrates the counters."""
+    """One probe per plain solver and one mixed batch of 64 CG probes, capped
+    at 10 iterations (-m 10 -i), on the 256^3 grid: a few dozen dispatches of
+    every kernel, including the estimator's streaming accumulate_kernel (reads
+    x, v, t, q; writes t, q), whose known traffic calibrates the counters."""
     import sedi_amd
     for kind in (2, 3, 1):
-        est = sedi_amd.Estimator(GRID, kind=kind, samples=1, maxiter=10, tol=TOL, seed=SEED,
+        est = sedi_amd.Estimator(GRID, kind=kind, samples=1, maxiter=10, tol=1e-9, seed=7,
                                  ignore_errors=True)
         est.run(want_D=False)
         est.close()
+    est = sedi_amd.Estimator(GRID, kind=2, samples=64, maxiter=10, tol=1e-30, seed=11,
+                             ignore_errors=True, precision=sedi_amd.FP32_MIXED, batch_k=64)
+    est.run(want_D=False)
+    est.close()
     print("pmc workload done", flush=True)
 
 
@@ -88,17 +112,19 @@ def stencil_only(reps=20):
     launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
     avg = ms / 1e3 / launches
     gbs = 2 * n * 8 / avg / 1e9
-    print(json.dumps({"kernel": "stencil_apply_kernel", "grid": list(GRID), "launches": launches,
-                      "avg_launch_us": round(avg * 1e6, 2), "achieved_GBs": round(gbs, 1),
-                      "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+    return {"kernel": "stencil_apply_kernel", "grid": list(GRID), "launches": launches,
+            "avg_launch_us": round(avg * 1e6, 2), "achieved_GBs": round(gbs, 1),
+            "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def traffic_ratio_from_profiles(kernel):
+    """Measured HBM bytes over algorithmic bytes of a launch, from the committed
+    rocprofv3 --pmc summary of the kernel (calibrated FETCH_SIZE + WRITE_SIZE
+    of fully active launches), if there is one."""
     path = os.path.join(ROOT, "profiles", "%s_256cube_pmc.txt" % kernel)
     if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    m = re.search(r"^traffic_ratio:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -125,9 +151,9 @@ def factor3(p, n):
     return best
 
 
-def cpu_baseline(kind, gpu_iters_mean, iters_cap):
-    """The reference's solver on the host cores: 1 probe (seed 7, probe 0) of
-    the same 256^3 workload capped at `iters_cap` iterations (-m cap -i),
+def cpu_baseline(kind, seed, tol, gpu_iters_mean, iters_cap):
+    """The reference's solver on the host cores: 1 probe (probe 0 of `seed`)
+    of the same 256^3 workload capped at `iters_cap` iterations (-m cap -i),
     scaled to a full probe-solve with the GPU run's mean iteration count."""
     if not os.path.exists(REF_BIN):
         return None
@@ -136,7 +162,7 @@ def cpu_baseline(kind, gpu_iters_mean, iters_cap):
         p -= 1
     dec = factor3(p, GRID[0])
     cmd = [REF_BIN, "-g", "x".join(map(str, GRID)), "-p", "x".join(map(str, dec)), "-s", "1",
-           "-e", repr(TOL), "-m", str(iters_cap), "-i", "-S", str(kind), "-r", str(SEED)]
+           "-e", repr(tol), "-m", str(iters_cap), "-i", "-S", str(kind), "-r", str(seed)]
     if p > 1:
         if not os.path.exists(MPIRUN):
             return None
@@ -151,34 +177,55 @@ def cpu_baseline(kind, gpu_iters_mean, iters_cap):
     secs, iters = float(m.group(1)), int(m.group(2))
     per_iter = secs / max(iters, 1)
     probe_secs = per_iter * gpu_iters_mean
-    return {"value": 1.0 / probe_secs, "unit": "probe-solves/s", "cores": p, "kind": "reference",
-            "sample": ("1 probe of the same workload (256^3, seed 7, probe 0), %s, capped at %d "
-                       "iterations (-m %d -i) under mpirun -n %d -p %s: %.2f s = %.2f ms/iteration, "
-                       "scaled to the GPU run's mean of %.1f iterations per probe-solve"
-                       % (SOLVER_LABEL[kind], iters, iters, p, "x".join(map(str, dec)), secs,
-                          per_iter * 1e3, gpu_iters_mean))}
+    return {"value": round(1.0 / probe_secs, 5), "unit": "probe-solves/s", "cores": p,
+            "kind": "reference",
+            "sample": ("1 probe of the same workload (256^3, seed %d, probe 0), plain FP64 %s, "
+                       "capped at %d iterations (-m %d -i) under mpirun -n %d -p %s: %.2f s = "
+                       "%.2f ms/iteration, scaled to the GPU run's mean of %.1f iterations per "
+                       "probe-solve" % (seed, SOLVER_LABEL[kind], iters, iters, p,
+                                        "x".join(map(str, dec)), secs, per_iter * 1e3,
+                                        gpu_iters_mean))}
 
 
-def main():
-    args = parse()
-    if args.pmc_workload:
-        pmc_workload()
-        return
-    if args.stencil:
-        stencil_only()
-        return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
-              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def base_line(args, value, wall, dtype, config):
+    return {
+        "metric": "probe-solves/s at 256^3 (batched mixed vs plain FP64); % HBM peak; p99 job latency",
+        "value": round(value, 4),
+        "unit": "probe-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (device-generated Rademacher probes, mt19937 seed %d)" % config["seed"],
+        "config": config,
+    }
+
+
+def roofline(kernel, launches, kms, bytes_total):
+    avg_s = kms / 1e3 / max(launches, 1)
+    per_launch = bytes_total / max(launches, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    ratio = traffic_ratio_from_profiles(kernel)
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if ratio is None else int(ratio * per_launch),
+            "traffic_over_algorithmic": ratio,
+            "algorithmic_bytes_per_launch": int(per_launch), "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches": launches}
+
+
+def run_plain(args):
     import numpy as np
     import sedi_amd
-
     kind = NAMES[args.solver]
     kname, nvec = DOMINANT[kind]
     n = GRID[0] * GRID[1] * GRID[2]
-    est = sedi_amd.Estimator(GRID, kind=kind, samples=SAMPLES, maxiter=MAXITER, tol=TOL,
-                             seed=SEED, ignore_errors=True)
+    est = sedi_amd.Estimator(GRID, kind=kind, samples=PLAIN["samples"], maxiter=MAXITER,
+                             tol=PLAIN["tol"], seed=PLAIN["seed"], ignore_errors=True)
     for _ in range(args.warmup):
         est.run(want_D=False)
     sedi_amd.profile_reset()
@@ -190,42 +237,111 @@ def main():
     launches, kms = sedi_amd.profile_get(kname)
     est.close()
     wall = t1 - t0
-    loop = sum(r["seconds"] for r in runs)
     iters = np.concatenate([r["iters"] for r in runs])
-    solves = SAMPLES * args.steps
-    value = solves / wall
-    avg_s = kms / 1e3 / max(launches, 1)
-    bytes_per_launch = nvec * n * 8
-    achieved = bytes_per_launch / avg_s / 1e9
-    line = {
-        "metric": "probe-solves/s at 256^3 (batched mixed vs plain FP64); % HBM peak; p99 job latency",
-        "value": round(value, 4),
-        "unit": "probe-solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (device-generated Rademacher probes, mt19937 seed 7)",
-        "config": {"workload": "plain single-probe FP64 %s estimator, 256^3 grid, 16 probes "
-                               "(seed 7), tol 1e-9, maxiter 2000" % SOLVER_LABEL[kind],
-                   "grid": list(GRID), "samples": SAMPLES, "solver": SOLVER_LABEL[kind],
-                   "precision": "fp64", "tol": TOL, "maxiter": MAXITER, "seed": SEED},
-        "iterations": {"min": int(iters.min()), "max": int(iters.max()),
-                       "mean": round(float(iters.mean()), 2)},
-        "estimator_loop_s": round(loop, 4),
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(kname),
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": round(avg_s * 1e6, 2), "launches": launches},
-        "cpu_baseline": None,
-    }
+    config = {"workload": "plain single-probe FP64 %s estimator, 256^3 grid, 16 probes (seed 7), "
+                          "tol 1e-9, maxiter 2000" % SOLVER_LABEL[kind],
+              "grid": list(GRID), "samples": PLAIN["samples"], "solver": SOLVER_LABEL[kind],
+              "precision": "fp64", "tol": PLAIN["tol"], "maxiter": MAXITER, "seed": PLAIN["seed"]}
+    line = base_line(args, PLAIN["samples"] * args.steps / wall, wall, "f64", config)
+    line["iterations"] = {"min": int(iters.min()), "max": int(iters.max()),
+                          "mean": round(float(iters.mean()), 2)}
+    line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
+    line["roofline"] = roofline(kname, launches, kms, launches * nvec * n * 8)
+    line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(kind, float(iters.mean()), args.cpu_iters)
+        line["cpu_baseline"] = cpu_baseline(kind, PLAIN["seed"], PLAIN["tol"], float(iters.mean()),
+                                            args.cpu_iters)
+    return line
+
+
+def run_headline(args):
+    import numpy as np
+    import sedi_amd
+    n = GRID[0] * GRID[1] * GRID[2]
+    K = max(4, (args.batch + 3) // 4 * 4)
+    est = sedi_amd.Estimator(GRID, kind=2, samples=BATCH["samples"], maxiter=MAXITER,
+                             tol=BATCH["tol"], seed=BATCH["seed"], ignore_errors=True,
+                             precision=sedi_amd.FP32_MIXED, batch_k=K,
+                             rep_every=BATCH["rep_every"])
+    for _ in range(args.warmup):
+        est.run(want_D=False)
+    sedi_amd.profile_reset()
+    sedi_amd.profile_enable(BATCH_KERNEL)
+    t0 = time.perf_counter()
+    runs = [est.run(want_D=False) for _ in range(args.steps)]
+    t1 = time.perf_counter()
+    sedi_amd.profile_enable(None)
+    launches, kms = sedi_amd.profile_get(BATCH_KERNEL)
+    est.close()
+    wall = t1 - t0
+    iters = np.concatenate([r["iters"] for r in runs])
+    codes = np.concatenate([r["codes"] for r in runs])
+    # algorithmic bytes: each right-hand side is credited with its own
+    # iterations only (a converged one that still rides in an active block of
+    # 4 is not counted)
+    rhs_iters = int(iters.sum())
+    config = {"workload": "batched mixed-precision CG estimator, 256^3 grid, 64 probes (seed 11) "
+                          "as one block of k=%d, FP32 vectors / FP64 reductions, residual "
+                          "replacement every 50, rel. residual 1e-8, maxiter 2000" % K,
+              "grid": list(GRID), "samples": BATCH["samples"], "solver": "CG", "batch_k": K,
+              "precision": "fp32 storage + fp64 reductions", "tol": BATCH["tol"],
+              "maxiter": MAXITER, "seed": BATCH["seed"], "rep_every": BATCH["rep_every"]}
+    value = BATCH["samples"] * args.steps / wall
+    line = base_line(args, value, wall, "f32+f64", config)
+    line["iterations"] = {"min": int(iters.min()), "max": int(iters.max()),
+                          "mean": round(float(iters.mean()), 2),
+                          "unconverged": int(np.count_nonzero(codes))}
+    line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
+    line["roofline"] = roofline(BATCH_KERNEL, launches, kms,
+                                rhs_iters * n * BATCH_BYTES_PER_RHS_CELL)
+    if not args.no_compare:
+        # plain FP64 drop-in and the mixed form on the first 16 probes of the
+        # same stream: the speed-up and the diagonal error of the mixed form
+        common = dict(kind=2, samples=CMP_SAMPLES, maxiter=MAXITER, tol=BATCH["tol"],
+                      seed=BATCH["seed"], ignore_errors=True)
+        warm = sedi_amd.Estimator(GRID, **dict(common, samples=1, maxiter=10))
+        warm.run(want_D=False)
+        warm.close()
+        plain = sedi_amd.Estimator(GRID, **common)
+        t0 = time.perf_counter()
+        rp = plain.run()
+        tp = time.perf_counter() - t0
+        plain.close()
+        mixed = sedi_amd.Estimator(GRID, precision=sedi_amd.FP32_MIXED, batch_k=CMP_SAMPLES,
+                                   rep_every=BATCH["rep_every"], **common)
+        rm = mixed.run()
+        mixed.close()
+        rel = np.abs(rm["D"] - rp["D"]) / np.abs(rp["D"])
+        pv = CMP_SAMPLES / tp
+        line["plain_fp64"] = {"value": round(pv, 4), "unit": "probe-solves/s",
+                              "workload": "plain FP64 CG, one probe at a time, probes 0-15 of "
+                                          "the same stream, tol 1e-8, maxiter 2000",
+                              "iterations_mean": round(float(rp["iters"].mean()), 2)}
+        line["speedup_vs_plain_fp64"] = round(value / pv, 3)
+        line["mixed_vs_fp64_D"] = {"max_rel": float(rel.max()),
+                                   "rms_rel": float(np.sqrt(np.mean(rel * rel))),
+                                   "probes": CMP_SAMPLES,
+                                   "iterations_mean_mixed": round(float(rm["iters"].mean()), 2)}
+    line["cpu_baseline"] = None
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(2, BATCH["seed"], BATCH["tol"], float(iters.mean()),
+                                            args.cpu_iters)
+    return line
+
+
+def main():
+    args = parse()
+    if args.pmc_workload:
+        pmc_workload()
+        return
+    if args.stencil:
+        print(json.dumps(stencil_only()), flush=True)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
+              "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    line = run_plain(args) if args.mode == "plain" else run_headline(args)
     print(json.dumps(line), flush=True)
 
 

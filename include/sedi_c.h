@@ -136,7 +136,8 @@ typedef struct sedi_estimate_args {
                                above 1 (or with SEDI_FP32_MIXED): CG only, the
                                probes in blocks of 4 (padded with zero RHS)   */
     int rep_every;          /* SEDI_FP32_MIXED: FP64 residual replacement
-                               every rep_every iterations (0 = 50)            */
+                               every rep_every iterations (0 = 50), besides
+                               the tenfold-decrease and convergence triggers  */
 } sedi_estimate_args;
 
 /* Allocates everything the run needs (excluded from the timed region, as the
@@ -160,6 +161,25 @@ int sedi_estimator_run(sedi_estimator *est, double *D, int *iters, double *errs,
 typedef void (*sedi_log_fn)(void *user, const char *line);
 int sedi_estimator_set_log(sedi_estimator *est, sedi_log_fn fn, void *user);
 
+/* Solver validation mode (stochastic-estimator.cc:162-171, ValidateSolver::
+ * fill_with_ref_rhs, ValidateSolver.hpp:93-96): every sample solves with this
+ * right-hand side (n cells, i1 fastest, host or device; copied) instead of a
+ * probe.  NULL returns to the probes. */
+int sedi_estimator_set_rhs(sedi_estimator *est, const double *v);
+
+/* Per-sample observer for the validation checks (ValidateSolver::
+ * compare_to_ref_solution after each solve, stochastic-estimator.cc:189-191;
+ * ValidateEstimator::compare_to_ref_inv_diag on D = t./q after each
+ * accumulation, :200-205).  Called from sedi_estimator_run after sample k with
+ * host copies (valid during the call) of the requested arrays, NULL for the
+ * others; the copies cost a device synchronisation per sample. */
+#define SEDI_HOOK_X 1 /* the sample's solution x_k */
+#define SEDI_HOOK_V 2 /* its right-hand side v_k   */
+#define SEDI_HOOK_D 4 /* D = t./q so far           */
+typedef void (*sedi_sample_fn)(void *user, int k, const double *x, const double *v,
+                               const double *D);
+int sedi_estimator_set_sample_hook(sedi_estimator *est, sedi_sample_fn fn, int want, void *user);
+
 void sedi_estimator_destroy(sedi_estimator *est);
 
 /* create + run + destroy. */
@@ -174,8 +194,9 @@ int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
  *   precision SEDI_FP64: FP64 vectors, as the plain solver;
  *   SEDI_FP32_MIXED: FP32 vectors and stencil, FP64 dot products, scalars and
  *   solution accumulator, with an FP64 residual replacement r = b - A x every
- *   rep_every iterations (0 = 50) and before a right-hand side is declared
- *   converged (its returned err is then the true residual's).
+ *   rep_every iterations (0 = 50), whenever the residual has dropped tenfold
+ *   since the last replacement (reliable updates), and before a right-hand
+ *   side is declared converged (its returned err is then the true residual's).
  * Results of a right-hand side do not depend on the others in its batch or on
  * K.  K is rounded up to a multiple of 4. */
 typedef struct sedi_batch sedi_batch;
@@ -195,6 +216,20 @@ void sedi_batch_destroy(sedi_batch *batch);
  * [count][ncells] doubles (host or device), each generated from its own
  * jump-ahead start: equal to sedi_rademacher_fill(seed, 2*ncells*k, ...). */
 int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double *v);
+
+/* ---- validation archives (HDF5) ------------------------------------------- */
+/* The reference's validation archives (ValidateSolver.hpp:30-85: datasets
+ * "x_k" and "v_k"; ValidateEstimator.hpp:27-62: "diag_inv") hold FP64 arrays
+ * of shape [n1][n2][n3] with i3 fastest (Slice3d::pack_into, Slice3d.hpp:42-51).
+ * These calls take and return i1-fastest arrays (host memory) and transpose.
+ * libhdf5 is loaded at run time; without it they return SEDI_ERR_UNSUPPORTED.
+ * They do not touch the GPU. */
+int sedi_h5_available(void);
+int sedi_h5_read(const char *path, const char *dataset, const sedi_grid *grid, double *out);
+/* Creates (truncates) path with `count` datasets; the reference never writes
+ * archives, this lets its -V / -E modes read the GPU's results. */
+int sedi_h5_write(const char *path, int count, const char *const *datasets,
+                  const double *const *arrays, const sedi_grid *grid);
 
 /* ---- measurement ---------------------------------------------------------- */
 /* Kernel timing with HIP events recorded on the launching stream around every

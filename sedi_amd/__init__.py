@@ -26,8 +26,8 @@ import numpy as np
 __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEGA",
            "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
            "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
-           "Estimator", "estimate", "Batch", "probe_batch", "profile_enable", "profile_get",
-           "profile_reset"]
+           "Estimator", "estimate", "Batch", "probe_batch", "h5_available", "h5_read", "h5_write",
+           "HOOK_X", "HOOK_V", "HOOK_D", "profile_enable", "profile_get", "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 # SEDI_AMD_LIB selects another build of the library (e.g. a tuning variant)
@@ -62,6 +62,9 @@ class _EstArgs(ctypes.Structure):
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
+_DP = ctypes.POINTER(ctypes.c_double)
+SAMPLE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, _DP, _DP, _DP)
+HOOK_X, HOOK_V, HOOK_D = 1, 2, 4
 
 _lib = None
 
@@ -98,6 +101,11 @@ def lib():
         L.sedi_batch_destroy.argtypes = [vp]
         L.sedi_batch_destroy.restype = None
         L.sedi_probe_batch.argtypes = [ctypes.c_uint32, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
+        L.sedi_estimator_set_rhs.argtypes = [vp, vp]
+        L.sedi_estimator_set_sample_hook.argtypes = [vp, SAMPLE_FN, ctypes.c_int, vp]
+        L.sedi_h5_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_Grid), vp]
+        L.sedi_h5_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                    ctypes.POINTER(vp), ctypes.POINTER(_Grid)]
         L.sedi_profile_enable.argtypes = [ctypes.c_char_p]
         L.sedi_profile_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), dp]
         _lib = L
@@ -304,6 +312,28 @@ class Estimator:
         self._log = LOG_FN(lambda user, line: fn(line.decode()))
         _check(lib().sedi_estimator_set_log(self._h, self._log, None))
 
+    def set_rhs(self, v):
+        """Validation mode: every sample solves with right-hand side v (None: probes)."""
+        if v is None:
+            _check(lib().sedi_estimator_set_rhs(self._h, None))
+            return
+        v = _f64(v, self.n)
+        _check(lib().sedi_estimator_set_rhs(self._h, _ptr(v)))
+
+    def set_sample_hook(self, fn, want=HOOK_X | HOOK_V | HOOK_D):
+        """fn(k, x, v, D) after each sample, numpy copies (None when not requested)."""
+        if fn is None:
+            self._hook = None
+            _check(lib().sedi_estimator_set_sample_hook(self._h, SAMPLE_FN(), 0, None))
+            return
+        n = self.n
+
+        def cb(user, k, x, v, D):
+            cp = lambda p: None if not p else np.ctypeslib.as_array(p, shape=(n,)).copy()
+            fn(k, cp(x), cp(v), cp(D))
+        self._hook = SAMPLE_FN(cb)
+        _check(lib().sedi_estimator_set_sample_hook(self._h, self._hook, int(want), None))
+
     def run(self, want_D=True):
         """Returns dict(status, D, iters, errs, codes, seconds)."""
         D = np.zeros(self.n) if want_D else None
@@ -338,6 +368,29 @@ def estimate(grid, kind=CG, samples=8, maxiter=1000, tol=1e-4, seed=1234, ignore
         return est.run()
     finally:
         est.close()
+
+
+def h5_available():
+    """True if libhdf5 can be loaded (validation archives)."""
+    return lib().sedi_h5_available() == 1
+
+
+def h5_read(path, dataset, grid):
+    """A [n1][n2][n3] (i3 fastest) FP64 dataset as an i1-fastest vector."""
+    g, n = _grid(grid)
+    out = np.empty(n)
+    _check(lib().sedi_h5_read(str(path).encode(), dataset.encode(), ctypes.byref(g), _ptr(out)))
+    return out
+
+
+def h5_write(path, grid, **datasets):
+    """Writes i1-fastest vectors as the reference's archive datasets."""
+    g, n = _grid(grid)
+    names = list(datasets)
+    arrs = [_f64(datasets[k], n) for k in names]
+    cn = (ctypes.c_char_p * len(names))(*[k.encode() for k in names])
+    ca = (ctypes.c_void_p * len(names))(*[a.ctypes.data for a in arrs])
+    _check(lib().sedi_h5_write(str(path).encode(), len(names), cn, ca, ctypes.byref(g)))
 
 
 def profile_enable(kernel_name):

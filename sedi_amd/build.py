@@ -29,7 +29,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
 # Packed-FP32 instructions are switched off for every device compile.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "plain.hip", "rng.hip", "batched.hip", "estimator.hip"]
+LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "h5io.cpp", "plain.hip", "rng.hip", "batched.hip",
+               "estimator.hip"]
 DRIVER_SOURCE = os.path.join(CSRC, "driver", "sedi_estimator.cpp")
 
 
@@ -64,7 +65,7 @@ def build(verbose=True, extra=(), lib=LIB, build_dir=BUILD, driver=True):
         if warn and verbose:
             print(warn, file=sys.stderr)
     tmp = lib + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-ldl"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, lib)
     if not driver:

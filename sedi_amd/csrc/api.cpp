@@ -8,6 +8,7 @@
 
 #include "batched.h"
 #include "estimator.h"
+#include "h5io.h"
 #include "mt_jump.h"
 #include "profiler.h"
 #include "plain.h"
@@ -211,7 +212,7 @@ int sedi_batch_solve(sedi_batch* h, const double* b, int nrhs, double* x, int ma
         }
         int rc = h->impl.load_rhs(bd, nrhs);
         if (!rc) rc = h->impl.solve(maxiter, tol, &h->last);
-        if (!rc) rc = h->impl.store_x(xd, nrhs);
+        if (!rc) rc = h->impl.store_x(xd, 0, nrhs);
         if (rc) return rc;
         SEDI_HIP(hipStreamSynchronize(s));
         if (xhost) SEDI_HIP(hipMemcpy(x, xd, bytes, hipMemcpyDeviceToHost));
@@ -260,6 +261,51 @@ int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double* v
         SEDI_HIP(hipStreamSynchronize(s));
         if (host) SEDI_HIP(hipMemcpy(v, vd, bytes, hipMemcpyDeviceToHost));
         return SEDI_OK;
+    });
+}
+
+int sedi_estimator_set_rhs(sedi_estimator* est, const double* v) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(est, SEDI_ERR_ARG, "estimator is NULL");
+        return est->impl.set_rhs(v);
+    });
+}
+
+int sedi_estimator_set_sample_hook(sedi_estimator* est, sedi_sample_fn fn, int want, void* user) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(est, SEDI_ERR_ARG, "estimator is NULL");
+        SEDI_REQUIRE((want & ~(SEDI_HOOK_X | SEDI_HOOK_V | SEDI_HOOK_D)) == 0, SEDI_ERR_ARG, "bad hook mask %d",
+                     want);
+        est->impl.set_sample_hook(fn, want, user);
+        return SEDI_OK;
+    });
+}
+
+int sedi_h5_available(void) {
+    return guarded([&]() -> int { return sedi::h5_available() ? 1 : 0; });
+}
+
+int sedi_h5_read(const char* path, const char* dataset, const sedi_grid* grid, double* out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(path && dataset && out, SEDI_ERR_ARG, "path, dataset and out must not be NULL");
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        return sedi::h5_read(path, dataset, s, out);
+    });
+}
+
+int sedi_h5_write(const char* path, int count, const char* const* datasets, const double* const* arrays,
+                  const sedi_grid* grid) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(path && count >= 0 && (count == 0 || (datasets && arrays)), SEDI_ERR_ARG,
+                     "bad archive arguments");
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        for (int k = 0; k < count; k++)
+            SEDI_REQUIRE(datasets[k] && arrays[k], SEDI_ERR_ARG, "dataset %d is NULL", k);
+        return sedi::h5_write(path, count, datasets, arrays, s);
     });
 }
 

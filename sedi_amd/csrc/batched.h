@@ -26,15 +26,18 @@ class BatchSolver {
     // [K/4][N][4] of the storage type (device).  Right-hand sides that are
     // all zero are done at once (padding).  Synchronises the stream.
     int solve(int maxiter, double tol, BatchResult* res);
-    // t += sum_k x_k v_k, q += sum_k v_k^2 over the first k_active probes, in
-    // probe order (the reference's accumulation, stochastic-estimator.cc:195,198).
-    int accumulate(double* t, double* q, int k_active);
+    // t += x_k v_k, q += v_k^2 for k = k0 .. k1-1 in probe order (the
+    // reference's accumulation, stochastic-estimator.cc:195,198).
+    int accumulate(double* t, double* q, int k0, int k1);
     // b: kact right-hand sides as [kact][N] FP64 (device), rounded to the
-    // storage type; slots kact..K-1 are zeroed.  Asynchronous on stream().
-    int load_rhs(const double* b, int kact);
-    // x: the first kact solutions as [kact][N] FP64 (device; x_hi + x_lo in
-    // the mixed form).  Asynchronous on stream().
-    int store_x(double* x, int kact);
+    // storage type; slots kact..K-1 are zeroed.  With `replicate`, b is one
+    // right-hand side used for all kact slots.  Asynchronous on stream().
+    int load_rhs(const double* b, int kact, bool replicate = false);
+    // x: solutions k0 .. k1-1 as [k1-k0][N] FP64 (device; x_hi + x_lo in the
+    // mixed form).  Asynchronous on stream().
+    int store_x(double* x, int k0, int k1);
+    // v: right-hand sides k0 .. k1-1 as stored (rounded to the storage type).
+    int store_rhs(double* v, int k0, int k1);
     void* rhs() const { return b_; }
     bool fp32() const { return fp32_; }
     int K() const { return K_; }
@@ -44,9 +47,9 @@ class BatchSolver {
 
   private:
     template <class T> int solve_t(int maxiter, double tol, BatchResult* res);
-    template <class T> int accumulate_t(double* t, double* q, int k_active);
-    template <class T> int pack_t(const double* b, int kact);
-    template <class T> int unpack_t(double* x, int kact);
+    template <class T> int accumulate_t(double* t, double* q, int k0, int k1);
+    template <class T> int pack_t(const double* b, int kact, long long row_stride);
+    template <class T> int unpack_t(double* x, int k0, int k1, bool rhs);
     Shape shape_{};
     double a_ = 6.0, b_c_ = 1.0, c_ = 1.0;
     int K_ = 0, nrb_ = 0;

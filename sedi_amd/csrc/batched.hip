@@ -8,15 +8,25 @@
 // probes.  Workgroups are laid out as (tile, RHS block); every block keeps
 // its own scalars and reductions.
 //
-// Arithmetic ("mixed"): with FP32 storage the stencil and the
-// vector updates run in FP32 with the FP64 scalars rounded to FP32, while the
-// dot products accumulate the exact FP64 products of the FP32 values, and the
-// scalars (alpha, beta, the norms) stay FP64.  Every rep_every iterations, and
-// whenever an RHS's recursive residual reaches the tolerance, that RHS gets an
-// FP64 reliable update: x_hi += x_lo, x_lo = 0, r = b - A x_hi in FP64 (stored
-// as FP32), and beta is recomputed from the new residual; an RHS is done only
-// when its true residual meets the tolerance (Sleijpen & van der Vorst 1996).
-// With FP64 storage the recurrences are the plain solver's.
+// Arithmetic, mixed form: with FP32 storage the stencil and the vector updates
+// run in FP32 with the FP64 scalars rounded to FP32, while the dot products
+// accumulate the exact FP64 products of the FP32 values, and the scalars
+// (alpha, beta, the norms) stay FP64.  An RHS gets an FP64 residual
+// replacement r = b - A (x_hi + x_lo) (computed in FP64, stored as FP32;
+// then x_hi += x_lo, x_lo = 0) every rep_every iterations, when its recursive
+// residual has dropped tenfold since the last replacement, when it reaches the
+// tolerance, and at maxiter.  The tenfold rule is the reliable-update
+// criterion (Sleijpen & van der Vorst, "Reliable updated residuals in hybrid
+// Bi-CG methods", Computing 56, 1996; delta = 0.1 as in Clark et al., "Solving
+// lattice QCD systems of equations using mixed precision solvers on GPUs",
+// Comput. Phys. Commun. 181, 2010): the FP32 recursion drifts from the true
+// residual by about eps32 * cond(A) * |r at the last replacement|, so
+// replacing at every decade keeps the drift at a percent of |r| for
+// cond(A) up to ~1e5 (512^3).  After a replacement beta is recomputed from the
+// new residual; if the true residual exceeds twice the recursive one the
+// recursion had drifted and the directions restart (beta = 0).  An RHS is done
+// only when its true residual meets the tolerance.  With FP64 storage the
+// recurrences are the plain solver's.
 //
 // Per-RHS state makes each probe's trajectory independent of the others in
 // its batch: an RHS that is done keeps x and r (alpha = 0), replacements are
@@ -60,14 +70,30 @@ __device__ __forceinline__ V4<T> operator*(const V4<T>& a, const V4<T>& b) {
     return {{a[0] * b[0], a[1] * b[1], a[2] * b[2], a[3] * b[3]}};
 }
 
+// Store of a 4-RHS value.  SEDI_BATCH_NT selects non-temporal stores (the
+// vectors written by a pass are not read again until the next pass).
+template <class T>
+__device__ __forceinline__ void stv(V4<T>* base, cidx_t i, const V4<T>& v) {
+#ifdef SEDI_BATCH_NT
+    typedef T vec4 __attribute__((ext_vector_type(4)));
+    const vec4 w = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(
+        w, reinterpret_cast<vec4*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(V4<T>))));
+#else
+    stc(base, i, v);
+#endif
+}
+
 // ---- device control --------------------------------------------------------
 struct BCtl {
-    double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *hist;
+    double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *elast, *hist;
     int *fiter, *code, *done, *rep, *fold;
     int *iter, *rb_done;
     double* partials;
     unsigned* counters;
     double tol;
+    double delta;    // replacement when the residual dropped by this factor
+    double restart;  // restart when true > restart * recursive residual
     int maxiter, rep_every, mixed, K;
     long long hstride;  // history entries per RHS
     long long pstride;  // partial sums per RHS block
@@ -131,6 +157,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         c.rep[k] = 0;
         c.fold[k] = 0;
         c.done[k] = 0;
+        c.elast[k] = e;
         if (e < c.tol) bset_done(c, k, 0, 0, e);  // Cg.hpp:130-134
         else if (c.maxiter < 1) {                 // :196-206 with no iteration run
             if (e <= c.tol) bset_done(c, k, 0, c.maxiter + 1, e);
@@ -163,7 +190,7 @@ struct PassBDir {
     }
     __device__ void finish(cidx_t i, const Raw&, const Aux&, const V& p, const V& s,
                            double (&acc)[ND]) const {
-        stc(pnew, i, p);
+        stv(pnew, i, p);
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q] += (double)s[q] * (double)p[q];
     }
@@ -216,8 +243,8 @@ struct PassBUpd {
                            double (&acc)[ND]) const {
         const V xn = o.x + alpha * pv;
         const V rn = o.r - alpha * s;
-        stc(xlo, i, xn);
-        stc(r, i, rn);
+        stv(xlo, i, xn);
+        stv(r, i, rn);
         const V u = im * rn;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -261,7 +288,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c,
         if (c.mixed) {
             // converged on the recursive residual, a replacement due, or the
             // last iteration: the FP64 pass decides
-            c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter);
+            c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter) ||
+                       (e < c.delta * c.elast[k]);
         } else if (e <= c.tol) {
             bset_done(c, k, 0, it, e);
         } else if (it >= c.maxiter) {
@@ -356,9 +384,11 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
         c.rep[k] = 0;
         c.fold[k] = 1;
         c.ru[k] = acc[q];
-        c.beta[k] = acc[q] / c.ru_old[k];  // beta from the replaced residual
         const double e = sqrt(acc[4 + q] / c.bb[k]);
+        // beta from the replaced residual, or a restart if the recursion drifted
+        c.beta[k] = e > c.restart * c.err[k] ? 0.0 : acc[q] / c.ru_old[k];
         c.err[k] = e;
+        c.elast[k] = e;
         c.hist[(long long)k * c.hstride + it - 1] = e;
         if (e <= c.tol) bset_done(c, k, 0, it, e);
         else if (it >= c.maxiter) bset_done(c, k, 1, c.maxiter, e);
@@ -403,12 +433,12 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
 
 // ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
 template <class T>
-__global__ void batch_accum_kernel(long long n, int kact, const V4<T>* xlo, const double* xhi,
+__global__ void batch_accum_kernel(long long n, int k0, int k1, const V4<T>* xlo, const double* xhi,
                                    const V4<T>* b, double* t, double* q) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         double tv = t[i], qv = q[i];
-        for (int k = 0; k < kact; k++) {
+        for (int k = k0; k < k1; k++) {
             const long long cell = (long long)(k >> 2) * n + i;
             const double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
             const double v = (double)b[cell][k & 3];
@@ -422,7 +452,8 @@ __global__ void batch_accum_kernel(long long n, int kact, const V4<T>* xlo, cons
 
 // ---- [K][N] FP64 <-> batch layout ------------------------------------------
 template <class T>
-__global__ void batch_pack_kernel(long long n, int kact, const double* __restrict__ src, V4<T>* dst) {
+__global__ void batch_pack_kernel(long long n, int kact, long long row_stride, const double* __restrict__ src,
+                                  V4<T>* dst) {
     const int rb = blockIdx.y;
     V4<T>* d = dst + (long long)rb * n;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -431,24 +462,26 @@ __global__ void batch_pack_kernel(long long n, int kact, const double* __restric
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int k = 4 * rb + q;
-            v[q] = k < kact ? (T)src[(long long)k * n + i] : (T)0;
+            v[q] = k < kact ? (T)src[(long long)k * row_stride + i] : (T)0;
         }
         d[i] = v;
     }
 }
 
+// Rows k0 .. k1-1 of the batch (x = x_hi + x_lo, or the right-hand sides when
+// xhi is null and `src` is b) into [k1-k0][n] FP64.  Grid y = RHS blocks.
 template <class T>
-__global__ void batch_unpack_kernel(long long n, int kact, const V4<T>* __restrict__ xlo,
+__global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __restrict__ src,
                                     const double* __restrict__ xhi, double* __restrict__ x) {
     const int rb = blockIdx.y;
-    const V4<T>* s = xlo + (long long)rb * n;
+    const V4<T>* s = src + (long long)rb * n;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const V4<T> v = s[i];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const long long k = 4 * rb + q;
-            if (k < kact) x[k * n + i] = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
+            if (k >= k0 && k < k1) x[(k - k0) * n + i] = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
         }
     }
 }
@@ -484,9 +517,19 @@ static KGeom bgeom(K kern16, const Shape& s, int D) {
         (void)hipGetLastError();
         occ = 2;
     }
+#ifdef SEDI_BATCH_MAXOCC
+    occ = std::min(occ, SEDI_BATCH_MAXOCC);
+#endif
     occ = std::min(occ, 8);
     const int per = std::max(8, occ * cus() / 8 * 8);
+#ifdef SEDI_BATCH_ZC
+    KGeom k;
+    k.tg = make_tile_grid(s, SEDI_BATCH_ZC);
+    k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
+    return cache.emplace(key, k).first->second;
+#else
     return cache.emplace(key, plan_geometry(s, per, D)).first->second;
+#endif
 }
 
 // Workgroups per RHS block of the elementwise kernels (independent of the
@@ -502,7 +545,7 @@ static unsigned ew_blocks(long long n) {
                          : launch(name, K<TT, 16>, grid, dim3(NTHREADS), __VA_ARGS__))
 
 // control block layout (doubles, then ints)
-enum { CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_NDOUBLE };
+enum { CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST, CD_NDOUBLE };
 enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_NINT };
 
 BatchSolver::~BatchSolver() {
@@ -560,6 +603,7 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     double* d = ctl_.as<double>();
     c.bb = d + CD_BB * K; c.ru = d + CD_RU * K; c.ru_old = d + CD_RUOLD * K;
     c.alpha = d + CD_ALPHA * K; c.beta = d + CD_BETA * K; c.err = d + CD_ERR * K; c.ferr = d + CD_FERR * K;
+    c.elast = d + CD_ELAST * K;
     int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
     c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
     c.fold = ip + CI_FOLD * K;
@@ -575,6 +619,8 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     c.tol = tol;
     c.maxiter = maxiter;
     c.rep_every = rep_every_;
+    c.delta = 0.1;
+    c.restart = 2.0;
     c.mixed = fp32_ ? 1 : 0;
     c.K = K;
 
@@ -666,43 +712,57 @@ int BatchSolver::solve(int maxiter, double tol, BatchResult* res) {
 }
 
 template <class T>
-int BatchSolver::accumulate_t(double* t, double* q, int k_active) {
+int BatchSolver::accumulate_t(double* t, double* q, int k0, int k1) {
     const long long n = shape_.n();
     long long g = (n + 255) / 256;
     if (g > 8192) g = 8192;
     SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n,
-                    k_active, (const V4<T>*)xlo_, (const double*)xhi_, (const V4<T>*)b_, t, q));
+                    k0, k1, (const V4<T>*)xlo_, (const double*)xhi_, (const V4<T>*)b_, t, q));
     return SEDI_OK;
 }
 
-int BatchSolver::accumulate(double* t, double* q, int k_active) {
-    return fp32_ ? accumulate_t<float>(t, q, k_active) : accumulate_t<double>(t, q, k_active);
+int BatchSolver::accumulate(double* t, double* q, int k0, int k1) {
+    SEDI_REQUIRE(0 <= k0 && k0 <= k1 && k1 <= K_, SEDI_ERR_ARG, "probe range %d..%d of a batch of %d", k0, k1, K_);
+    if (k0 == k1) return SEDI_OK;
+    return fp32_ ? accumulate_t<float>(t, q, k0, k1) : accumulate_t<double>(t, q, k0, k1);
 }
 
 template <class T>
-int BatchSolver::pack_t(const double* b, int kact) {
+int BatchSolver::pack_t(const double* b, int kact, long long row_stride) {
     const long long n = shape_.n();
     SEDI_HIP(launch("batch_pack_kernel", batch_pack_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
-                    stream_, n, kact, b, static_cast<V4<T>*>(b_)));
+                    stream_, n, kact, row_stride, b, static_cast<V4<T>*>(b_)));
     return SEDI_OK;
 }
 
 template <class T>
-int BatchSolver::unpack_t(double* x, int kact) {
+int BatchSolver::unpack_t(double* x, int k0, int k1, bool rhs) {
     const long long n = shape_.n();
-    SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
-                    stream_, n, kact, static_cast<const V4<T>*>(xlo_), (const double*)xhi_, x));
+    const int rb0 = k0 / 4, rb1 = (k1 + 3) / 4;
+    const V4<T>* src = static_cast<const V4<T>*>(rhs ? b_ : xlo_) + (long long)rb0 * n;
+    const double* xhi = rhs ? nullptr : xhi_;
+    // the kernel indexes RHS blocks from rb0: shift the row range to match
+    SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), rb1 - rb0),
+                    dim3(NTHREADS), stream_, n, k0 - 4 * rb0, k1 - 4 * rb0, src,
+                    xhi ? xhi + (long long)4 * rb0 * n : nullptr, x));
     return SEDI_OK;
 }
 
-int BatchSolver::load_rhs(const double* b, int kact) {
+int BatchSolver::load_rhs(const double* b, int kact, bool replicate) {
     SEDI_REQUIRE(kact >= 0 && kact <= K_, SEDI_ERR_ARG, "%d right-hand sides for a batch of %d", kact, K_);
-    return fp32_ ? pack_t<float>(b, kact) : pack_t<double>(b, kact);
+    const long long stride = replicate ? 0 : shape_.n();
+    return fp32_ ? pack_t<float>(b, kact, stride) : pack_t<double>(b, kact, stride);
 }
 
-int BatchSolver::store_x(double* x, int kact) {
-    SEDI_REQUIRE(kact >= 0 && kact <= K_, SEDI_ERR_ARG, "%d solutions of a batch of %d", kact, K_);
-    return fp32_ ? unpack_t<float>(x, kact) : unpack_t<double>(x, kact);
+int BatchSolver::store_x(double* x, int k0, int k1) {
+    SEDI_REQUIRE(0 <= k0 && k0 < k1 && k1 <= K_, SEDI_ERR_ARG, "solutions %d..%d of a batch of %d", k0, k1, K_);
+    return fp32_ ? unpack_t<float>(x, k0, k1, false) : unpack_t<double>(x, k0, k1, false);
+}
+
+int BatchSolver::store_rhs(double* v, int k0, int k1) {
+    SEDI_REQUIRE(0 <= k0 && k0 < k1 && k1 <= K_, SEDI_ERR_ARG, "right-hand sides %d..%d of a batch of %d", k0,
+                 k1, K_);
+    return fp32_ ? unpack_t<float>(v, k0, k1, true) : unpack_t<double>(v, k0, k1, true);
 }
 
 std::vector<double> BatchSolver::history(int k) const {

@@ -8,7 +8,8 @@
 //   -m N      maxiter (1000)                          DomainDecomposition.hpp:34-36)
 //   -i        ignore solver errors          -f MODE   solver mode bits (accepted)
 //   -F        no FP exceptions (accepted; the GPU does not trap)
-//   -V FILE / -E FILE  HDF5 validation archives
+//   -V FILE / -E FILE  HDF5 validation archives (ValidateSolver.hpp,
+//             ValidateEstimator.hpp; libhdf5 is loaded at run time)
 // Additions (the reference fixes these at compile time or not at all):
 //   -S K      solver: 1 BiCGStab, 2 CG, 3 PipelinedCg (the SOLVER numbers; default 2)
 //   -r SEED   mt19937 seed of the probe stream (default 1234)
@@ -16,11 +17,14 @@
 //   -k K      probes solved together (default 1)
 //   -R M      mixed precision: FP64 residual replacement every M iterations (default 50)
 //   -o FILE   write D = t./q as raw little-endian float64, i1 fastest
+//   -X FILE   write an HDF5 validation archive of this run: x_k and v_k of the
+//             last sample, diag_inv = D (the datasets -V and -E read)
 // Exit status: 0, 1 on usage errors, 100+code when a solve fails without -i
 // (the MPI_Abort code of stochastic-estimator.cc:183), 2 on a runtime error.
 #include <sys/resource.h>
 #include <unistd.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -45,13 +49,14 @@ struct Options {
     int precision = SEDI_FP64;
     int batch = 1;
     int rep_every = 0;
+    const char* xfile = nullptr;
 };
 
 const char* progname = "sedi-estimator";
 
 [[noreturn]] void usage() {
     fprintf(stderr, "Usage: %s -g NxNxN -p NxNxN [-s NSAMPLES] [-e TOL] [-m MAXITER] [-i] [-f MODE] [-F]"
-                    " [-V HDF5_FILE] [-E HDF5_FILE] [-S SOLVER] [-r SEED] [-P fp64|mixed] [-k K] [-R M] [-o FILE]\n",
+                    " [-V HDF5_FILE] [-E HDF5_FILE] [-S SOLVER] [-r SEED] [-P fp64|mixed] [-k K] [-R M] [-o FILE] [-X HDF5_FILE]\n",
             progname);
     fprintf(stderr, "\t-g NxNxN : size of the global 3D grid\n");
     fprintf(stderr, "\t-p NxNxN : decomposition into ranks (logical on one GPU)\n");
@@ -68,6 +73,8 @@ const char* progname = "sedi-estimator";
     fprintf(stderr, "\t-P PREC : fp64 (default) or mixed (FP32 vectors, FP64 reductions)\n");
     fprintf(stderr, "\t-k K : probes solved together (default 1)\n");
     fprintf(stderr, "\t-R M : mixed precision: FP64 residual replacement every M iterations (default 50)\n");
+    fprintf(stderr, "\t-o FILE : write the diagonal estimate as raw float64\n");
+    fprintf(stderr, "\t-X HDF5_FILE : write a validation archive (x_k, v_k of the last sample, diag_inv)\n");
     fprintf(stderr, "\t-o FILE : write the estimated diagonal as raw float64\n");
     exit(1);
 }
@@ -76,7 +83,7 @@ Options parse(int argc, char** argv) {
     Options o;
     const char *g = nullptr, *p = nullptr;
     int c;
-    while ((c = getopt(argc, argv, "g:p:s:e:m:if:FV:E:S:r:P:k:R:o:")) != -1) {
+    while ((c = getopt(argc, argv, "g:p:s:e:m:if:FV:E:S:r:P:k:R:o:X:")) != -1) {
         switch (c) {
         case 'g': g = optarg; break;
         case 'p': p = optarg; break;
@@ -98,6 +105,7 @@ Options parse(int argc, char** argv) {
         case 'k': if (sscanf(optarg, "%d", &o.batch) != 1 || o.batch < 1) usage(); break;
         case 'R': if (sscanf(optarg, "%d", &o.rep_every) != 1 || o.rep_every < 0) usage(); break;
         case 'o': o.out = optarg; break;
+        case 'X': o.xfile = optarg; break;
         default: usage();
         }
     }
@@ -110,9 +118,64 @@ const char* solver_name(int k) {
     return k == SEDI_CG ? "Cg" : (k == SEDI_PCG ? "PipelinedCg" : "BiCgStab");
 }
 
-void print_line(void*, const char* line) {
+// ---- validation (ValidateSolver.hpp:30-113, ValidateEstimator.hpp:27-87) ----
+struct Validation {
+    long n = 0;
+    bool solver = false, estimator = false, archive = false;
+    std::vector<double> ref_x, ref_v, ref_D;  // i1 fastest
+    double l2_sol = 0.0, linf_sol = 0.0;
+    std::vector<double> last_x, last_v;
+};
+
+// Norms of the reference solution (ValidateSolver.hpp:62-73).
+void solution_norms(Validation& v) {
+    double l2 = 0.0, linf = 0.0;
+    for (long i = 0; i < v.n; i++) {
+        const double x = v.ref_x[i];
+        l2 += x * x;
+        if (fabs(x) > linf) linf = fabs(x);
+    }
+    v.l2_sol = sqrt(l2 / v.n);
+    v.linf_sol = linf;
+}
+
+// sqrt(sum delta^2 / n) and max |delta| (ValidateSolver.hpp:99-111,
+// ValidateEstimator.hpp:72-84; one rank, so the allreduces are the identity).
+void errors(const double* a, const double* b, long n, double* l2, double* linf) {
+    double e2 = 0.0, ei = 0.0;
+    for (long i = 0; i < n; i++) {
+        const double d = a[i] - b[i];
+        e2 += d * d;
+        if (fabs(d) > ei) ei = fabs(d);
+    }
+    *l2 = sqrt(e2 / n);
+    *linf = ei;
+}
+
+void on_sample(void* user, int, const double* x, const double* v, const double* D) {
+    Validation& val = *static_cast<Validation*>(user);
+    double l2, linf;
+    if (val.solver && x) {
+        printf("VALIDATION: comparing to ref solution from validation dataset\n");
+        errors(x, val.ref_x.data(), val.n, &l2, &linf);
+        printf("VALIDATION: rel L2 error=%.4e, rel Linf error=%.4e\n", l2 / val.l2_sol, linf / val.linf_sol);
+    }
+    if (val.estimator && D) {
+        errors(D, val.ref_D.data(), val.n, &l2, &linf);
+        printf("VALIDATION_ESTIMATOR: L2 error=%.4e, Linf error=%.4e\n", l2, linf);
+    }
+    if (val.archive) {
+        if (x) val.last_x.assign(x, x + val.n);
+        if (v) val.last_v.assign(v, v + val.n);
+    }
+}
+
+void print_line(void* user, const char* line) {
     fputs(line, stdout);
     fputc('\n', stdout);
+    const Validation* val = static_cast<const Validation*>(user);
+    if (val && val->solver && !strncmp(line, "Solving for sample", 18))
+        printf("VALIDATION: filling RHS from validation dataset\n");  // ValidateSolver.hpp:94
 }
 
 }  // namespace
@@ -129,10 +192,41 @@ int main(int argc, char** argv) {
     printf("Global grid shape is (%ld, %ld, %ld)\n", o.ng1, o.ng2, o.ng3);
     printf("Domain decomposition across 1 GPU (logical -p %dx%dx%d)\n", o.np1, o.np2, o.np3);
     printf("Process block shape is (%ld, %ld, %ld)\n", o.ng1 / o.np1, o.ng2 / o.np2, o.ng3 / o.np3);
-    if (o.vfile || o.efile) {
+    const long n = o.ng1 * o.ng2 * o.ng3;
+    const sedi_grid grid{o.ng1, o.ng2, o.ng3};
+    Validation val;
+    val.n = n;
+    if ((o.vfile || o.efile || o.xfile) && sedi_h5_available() != 1) {
+        // stochastic-estimator.cc:119-124
         printf("WARNING: validation mode selected, but compiled without HDF5 support: disabling!\n");
-        o.vfile = o.efile = nullptr;
+        o.vfile = o.efile = o.xfile = nullptr;
     }
+    if (o.vfile) {  // ValidateSolver.hpp:30-85
+        val.ref_x.resize(n);
+        val.ref_v.resize(n);
+        if (sedi_h5_read(o.vfile, "x_k", &grid, val.ref_x.data()) < 0 ||
+            sedi_h5_read(o.vfile, "v_k", &grid, val.ref_v.data()) < 0) {
+            fprintf(stderr, "%s: %s\n", progname, sedi_last_error());
+            return 2;
+        }
+        solution_norms(val);
+        printf("VALIDATION: rank %d read arrays from %s\n", 0, o.vfile);
+        val.solver = true;
+    }
+    if (o.efile) {  // ValidateEstimator.hpp:27-62
+        val.ref_D.resize(n);
+        if (sedi_h5_read(o.efile, "diag_inv", &grid, val.ref_D.data()) < 0) {
+            fprintf(stderr, "%s: %s\n", progname, sedi_last_error());
+            return 2;
+        }
+        printf("VALIDATION_ESTIMATOR: rank %d read arrays from %s\n", 0, o.efile);
+        val.estimator = true;
+    }
+    if (val.solver && val.estimator) {  // stochastic-estimator.cc:127-131
+        printf("WARNING: cannot enable both solver and estimator validation mode: disabling both!\n");
+        val.solver = val.estimator = false;
+    }
+    val.archive = o.xfile != nullptr;
 
     sedi_estimate_args a;
     memset(&a, 0, sizeof(a));
@@ -157,13 +251,22 @@ int main(int argc, char** argv) {
     }
     printf("Initialized %s solver, sync_dotprod=%d, sync_stencil=%d\n", solver_name(o.solver),
            (o.mode & 2) != 0, (o.mode & 1) != 0);
-    sedi_estimator_set_log(est, print_line, nullptr);
-    const long n = o.ng1 * o.ng2 * o.ng3;
-    std::vector<double> D(o.out ? n : 0);
+    sedi_estimator_set_log(est, print_line, &val);
+    if (val.solver) rc = sedi_estimator_set_rhs(est, val.ref_v.data());
+    const int want = (val.solver ? SEDI_HOOK_X : 0) | (val.estimator ? SEDI_HOOK_D : 0) |
+                     (val.archive ? SEDI_HOOK_X | SEDI_HOOK_V : 0);
+    if (rc >= 0 && want) rc = sedi_estimator_set_sample_hook(est, on_sample, want, &val);
+    if (rc < 0) {
+        fprintf(stderr, "%s: %s\n", progname, sedi_last_error());
+        sedi_estimator_destroy(est);
+        return 2;
+    }
+    const bool want_D = o.out || val.archive;
+    std::vector<double> D(want_D ? n : 0);
     std::vector<int> iters(o.samples), codes(o.samples, -1);
     std::vector<double> errs(o.samples);
     double secs = 0.0;
-    rc = sedi_estimator_run(est, o.out ? D.data() : nullptr, iters.data(), errs.data(), codes.data(), &secs);
+    rc = sedi_estimator_run(est, want_D ? D.data() : nullptr, iters.data(), errs.data(), codes.data(), &secs);
     if (rc < 0) {
         fprintf(stderr, "%s: %s\n", progname, sedi_last_error());
         sedi_estimator_destroy(est);
@@ -197,6 +300,15 @@ int main(int argc, char** argv) {
             return 2;
         }
         fclose(f);
+    }
+    if (val.archive && o.samples > 0) {
+        const char* names[3] = {"x_k", "v_k", "diag_inv"};
+        const double* arrays[3] = {val.last_x.data(), val.last_v.data(), D.data()};
+        if (sedi_h5_write(o.xfile, 3, names, arrays, &grid) < 0) {
+            fprintf(stderr, "%s: %s\n", progname, sedi_last_error());
+            sedi_estimator_destroy(est);
+            return 2;
+        }
     }
     sedi_estimator_destroy(est);
     return 0;

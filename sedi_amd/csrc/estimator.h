@@ -19,11 +19,29 @@ class Estimator {
     int create(const sedi_estimate_args& a);
     int run(double* D, int* iters, double* errs, int* codes, double* seconds);
     void set_log(sedi_log_fn fn, void* user) { log_fn_ = fn; log_user_ = user; }
+    // Validation mode (stochastic-estimator.cc:162-171): every sample solves
+    // with this right-hand side (n cells, host or device) instead of a probe;
+    // NULL returns to the probes.
+    int set_rhs(const double* v);
+    void set_sample_hook(sedi_sample_fn fn, int want, void* user) {
+        hook_ = fn;
+        hook_want_ = want;
+        hook_user_ = user;
+    }
 
   private:
     void log(const char* line);
     void log_history(int sample, const std::vector<double>& hist, int nh);
     int run_batched(double* D, int* iters, double* errs, int* codes, double* seconds);
+    // host copies for the sample hook, then the call
+    int call_hook(int k, const double* x, const double* v, hipStream_t s);
+    int call_hook_batched(int k, int slot, hipStream_t s);
+    bool fixed_rhs_ = false;
+    DevBuf rhs_, hook_buf_, tmp_xv_;
+    sedi_sample_fn hook_ = nullptr;
+    int hook_want_ = 0;
+    void* hook_user_ = nullptr;
+    std::vector<double> hx_, hv_, hd_;
     bool batched_ = false;
     std::unique_ptr<BatchSolver> batch_;
     DevBuf scratch_;

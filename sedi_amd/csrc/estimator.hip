@@ -109,6 +109,58 @@ void Estimator::log(const char* line) {
     if (log_fn_) log_fn_(log_user_, line);
 }
 
+int Estimator::set_rhs(const double* v) {
+    if (!v) {
+        fixed_rhs_ = false;
+        return SEDI_OK;
+    }
+    const size_t bytes = sizeof(double) * (size_t)shape_.n();
+    SEDI_HIP(rhs_.reserve(bytes));
+    SEDI_HIP(hipMemcpy(rhs_.p, v, bytes, is_device_pointer(v) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    fixed_rhs_ = true;
+    return SEDI_OK;
+}
+
+// x, v: device vectors of the sample; D = t./q so far (the reference computes
+// it per sample in estimator validation mode, stochastic-estimator.cc:200-205).
+int Estimator::call_hook(int k, const double* x, const double* v, hipStream_t s) {
+    const long long n = shape_.n();
+    const size_t bytes = sizeof(double) * (size_t)n;
+    const bool wx = hook_want_ & SEDI_HOOK_X, wv = hook_want_ & SEDI_HOOK_V, wd = hook_want_ & SEDI_HOOK_D;
+    if (wd) {
+        SEDI_HIP(hook_buf_.reserve(bytes));
+        SEDI_HIP(launch("divide_kernel", divide_kernel, ew_grid(n), dim3(256), s, (const double*)t_,
+                        (const double*)q_, hook_buf_.as<double>(), n));
+    }
+    SEDI_HIP(hipStreamSynchronize(s));
+    if (wx) {
+        hx_.resize(n);
+        SEDI_HIP(hipMemcpy(hx_.data(), x, bytes, hipMemcpyDeviceToHost));
+    }
+    if (wv) {
+        hv_.resize(n);
+        SEDI_HIP(hipMemcpy(hv_.data(), v, bytes, hipMemcpyDeviceToHost));
+    }
+    if (wd) {
+        hd_.resize(n);
+        SEDI_HIP(hipMemcpy(hd_.data(), hook_buf_.p, bytes, hipMemcpyDeviceToHost));
+    }
+    hook_(hook_user_, k, wx ? hx_.data() : nullptr, wv ? hv_.data() : nullptr, wd ? hd_.data() : nullptr);
+    return SEDI_OK;
+}
+
+int Estimator::call_hook_batched(int k, int slot, hipStream_t s) {
+    const long long n = shape_.n();
+    const size_t bytes = sizeof(double) * (size_t)n;
+    SEDI_HIP(tmp_xv_.reserve(2 * bytes));
+    double* x = tmp_xv_.as<double>();
+    double* v = x + n;
+    int rc = batch_->store_x(x, slot, slot + 1);
+    if (!rc) rc = batch_->store_rhs(v, slot, slot + 1);
+    if (rc) return rc;
+    return call_hook(k, x, v, s);
+}
+
 void Estimator::log_history(int sample, const std::vector<double>& hist, int nh) {
     char line[160];
     snprintf(line, sizeof(line), "Solving for sample %d/%d...", sample + 1, args_.samples);
@@ -137,8 +189,9 @@ int Estimator::run_batched(double* D, int* iters, double* errs, int* codes, doub
     BatchResult res;
     for (int k0 = 0; k0 < args_.samples && !status; k0 += K) {
         const int kact = std::min(K, args_.samples - k0);
-        if ((rc = fill_probe_batch(args_.seed, n, k0, K, kact, bs.fp32(), bs.rhs(), s, scratch_))) return rc;
-        if ((rc = bs.solve(args_.maxiter, args_.tol, &res))) return rc;
+        if (fixed_rhs_) rc = bs.load_rhs(rhs_.as<double>(), kact, true);
+        else rc = fill_probe_batch(args_.seed, n, k0, K, kact, bs.fp32(), bs.rhs(), s, scratch_);
+        if (rc || (rc = bs.solve(args_.maxiter, args_.tol, &res))) return rc;
         int kuse = kact;
         for (int k = 0; k < kact; k++) {
             const int code = res.codes[k];
@@ -156,8 +209,11 @@ int Estimator::run_batched(double* D, int* iters, double* errs, int* codes, doub
                 kuse = k;
                 break;
             }
+            if (hook_) {  // one sample at a time, as the reference's loop
+                if ((rc = bs.accumulate(t_, q_, k, k + 1)) || (rc = call_hook_batched(k0 + k, k, s))) return rc;
+            }
         }
-        if (kuse > 0 && (rc = bs.accumulate(t_, q_, kuse))) return rc;
+        if (!hook_ && (rc = bs.accumulate(t_, q_, 0, kuse))) return rc;
     }
     if (D) {
         double* dd = D;
@@ -189,12 +245,15 @@ int Estimator::run(double* D, int* iters, double* errs, int* codes, double* seco
     if (rc) return rc;
     SEDI_HIP(hipEventRecord(ev_free_[0], s));
     SEDI_HIP(hipEventRecord(ev_free_[1], s));
-    if ((rc = rng_.seed(args_.seed, rng_stream_))) return rc;
-    if (args_.samples > 0) {
-        if ((rc = rng_.fill(v_[0], n, rng_stream_))) return rc;
-        SEDI_HIP(hipEventRecord(ev_v_[0], rng_stream_));
+    if (!fixed_rhs_) {
+        if ((rc = rng_.seed(args_.seed, rng_stream_))) return rc;
+        if (args_.samples > 0) {
+            if ((rc = rng_.fill(v_[0], n, rng_stream_))) return rc;
+            SEDI_HIP(hipEventRecord(ev_v_[0], rng_stream_));
+        }
     }
     SEDI_HIP(hipStreamSynchronize(s));
+    SEDI_HIP(hipStreamSynchronize(rng_stream_));
 
     const auto t0 = std::chrono::steady_clock::now();
     int status = 0;
@@ -204,16 +263,19 @@ int Estimator::run(double* D, int* iters, double* errs, int* codes, double* seco
         const int cur = k & 1, nxt = cur ^ 1;
         snprintf(line, sizeof(line), "Solving for sample %d/%d...", k + 1, args_.samples);
         log(line);
-        if (k + 1 < args_.samples) {
-            SEDI_HIP(hipStreamWaitEvent(rng_stream_, ev_free_[nxt], 0));
-            if ((rc = rng_.fill(v_[nxt], n, rng_stream_))) return rc;
-            SEDI_HIP(hipEventRecord(ev_v_[nxt], rng_stream_));
+        const double* vk = fixed_rhs_ ? rhs_.as<double>() : v_[cur];
+        if (!fixed_rhs_) {
+            if (k + 1 < args_.samples) {
+                SEDI_HIP(hipStreamWaitEvent(rng_stream_, ev_free_[nxt], 0));
+                if ((rc = rng_.fill(v_[nxt], n, rng_stream_))) return rc;
+                SEDI_HIP(hipEventRecord(ev_v_[nxt], rng_stream_));
+            }
+            SEDI_HIP(hipStreamWaitEvent(s, ev_v_[cur], 0));
         }
-        SEDI_HIP(hipStreamWaitEvent(s, ev_v_[cur], 0));
         if ((rc = fill_dev(x_, n, 0.0, s))) return rc;  // stochastic-estimator.cc:175
         double fe = 0.0;
         int fi = 0;
-        const int code = solver_.solve_dev(v_[cur], x_, nullptr, args_.maxiter, args_.tol, &fe, &fi);
+        const int code = solver_.solve_dev(vk, x_, nullptr, args_.maxiter, args_.tol, &fe, &fi);
         if (code < 0) return code;
         if (log_fn_) {
             const int nh = solver_.err_history(nullptr, 0);
@@ -235,7 +297,8 @@ int Estimator::run(double* D, int* iters, double* errs, int* codes, double* seco
             break;
         }
         SEDI_HIP(launch("accumulate_kernel", accumulate_kernel, ew_grid(n), dim3(256), s,
-                        (const double*)x_, (const double*)v_[cur], t_, q_, n));
+                        (const double*)x_, vk, t_, q_, n));
+        if (hook_ && (rc = call_hook(k, x_, vk, s))) return rc;
         SEDI_HIP(hipEventRecord(ev_free_[cur], s));
     }
     if (D) {

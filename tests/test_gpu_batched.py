@@ -114,21 +114,26 @@ def test_batch_eigenvector_one_iteration():
 
 
 # ------------------------------------------------------------------ mixed ----
-@pytest.mark.parametrize("tol,rep", [(1e-8, 0), (1e-9, 0), (1e-9, 10)])
-def test_batch_mixed_converges_on_true_residual(tol, rep):
-    b = probes(1234, S32, 0, 8)
-    x, iters, errs, codes = sedi.Batch(S32, 8, sedi.FP32_MIXED, rep_every=rep).solve(
+@pytest.mark.parametrize("shape,tol,rep", [(S32, 1e-8, 0), (S32, 1e-9, 0), (S32, 1e-9, 10), (S32, 1e-12, 0),
+                                           ((12, 10, 8), 1e-8, 0), ((12, 10, 8), 1e-11, 0),
+                                           ((12, 10, 8), 1e-13, 0)])
+def test_batch_mixed_converges_on_true_residual(shape, tol, rep):
+    # tolerances far below FP32's precision and grids that converge before
+    # the first periodic replacement: the tenfold rule keeps the FP32
+    # recursion close to the true residual
+    b = probes(1234, shape, 0, 8)
+    x, iters, errs, codes = sedi.Batch(shape, 8, sedi.FP32_MIXED, rep_every=rep).solve(
         b, maxiter=1000, tol=tol)
-    xd, itd, _, _ = sedi.Batch(S32, 8, sedi.FP64).solve(b, maxiter=1000, tol=tol)
-    assert np.all(codes == 0)
+    xd, itd, _, _ = sedi.Batch(shape, 8, sedi.FP64).solve(b, maxiter=1000, tol=tol)
+    assert np.all(codes == 0), (iters, errs)
     for k in range(8):
         # the reported error is the FP64 replacement's true residual
-        t = true_rel_residual(S32, b[k], x[k])
+        t = true_rel_residual(shape, b[k], x[k])
         assert errs[k] <= tol and t <= 1.05 * tol, (k, errs[k], t)
         assert abs(t - errs[k]) <= 0.05 * tol
         # a few more iterations than FP64 at most
         assert iters[k] <= itd[k] + max(5, itd[k] // 10), (iters[k], itd[k])
-        assert normwise(x[k], xd[k]) < 1e-6
+        assert normwise(x[k], xd[k]) < max(1e-6, 1e3 * tol)
 
 
 def test_batch_mixed_history_ends_with_true_residual():
