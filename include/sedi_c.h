@@ -133,8 +133,9 @@ typedef struct sedi_estimate_args {
     uint32_t seed;          /* mt19937 seed of the probe stream                 */
     int precision;          /* SEDI_FP64 or SEDI_FP32_MIXED                     */
     int batch_k;            /* probes solved together (1 = plain, one at a time);
-                               above 1 (or with SEDI_FP32_MIXED): CG only, the
-                               probes in blocks of 4 (padded with zero RHS)   */
+                               above 1 (or with SEDI_FP32_MIXED): the batched
+                               solvers, probes in blocks of 4 (padded with
+                               zero right-hand sides)                         */
     int rep_every;          /* SEDI_FP32_MIXED: FP64 residual replacement
                                every rep_every iterations (0 = 50), besides
                                the tenfold-decrease and convergence triggers  */
@@ -186,11 +187,12 @@ void sedi_estimator_destroy(sedi_estimator *est);
 int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
                   double *errs, int *codes, double *seconds);
 
-/* ---- batched multi-RHS CG (no reference counterpart) ---------------------- */
+/* ---- batched multi-RHS solvers (no reference counterpart) ------------------ */
 /* The reference solves one probe at a time (stochastic-estimator.cc:170-199);
- * a batch solves up to K right-hand sides together with the Cg.hpp:102-208
- * recurrences kept per right-hand side: one stencil sweep per pass serves
- * every right-hand side, with per-RHS alpha, beta, convergence and masks.
+ * a batch solves up to K right-hand sides together with the recurrences of
+ * Cg.hpp:102-208, PipelinedCg.hpp:113-243 or BiCgStab.hpp:117-264 (kind)
+ * kept per right-hand side: one stencil sweep per pass serves every
+ * right-hand side, with per-RHS scalars, convergence and masks.
  *   precision SEDI_FP64: FP64 vectors, as the plain solver;
  *   SEDI_FP32_MIXED: FP32 vectors and stencil, FP64 dot products, scalars and
  *   solution accumulator, with an FP64 residual replacement r = b - A x every
@@ -200,7 +202,7 @@ int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
  * Results of a right-hand side do not depend on the others in its batch or on
  * K.  K is rounded up to a multiple of 4. */
 typedef struct sedi_batch sedi_batch;
-int sedi_batch_create(const sedi_grid *grid, const sedi_stencil *stencil, int K,
+int sedi_batch_create(const sedi_grid *grid, const sedi_stencil *stencil, int kind, int K,
                       int precision, int rep_every, sedi_batch **out);
 /* Solves A x_k = b_k from x_k = 0 for k < nrhs <= K; b and x are [nrhs][n]
  * FP64 (host or device).  iters/errs/codes (may be NULL) per right-hand side,

@@ -94,7 +94,7 @@ def lib():
         L.sedi_estimator_destroy.restype = None
         L.sedi_estimate.argtypes = [ctypes.POINTER(_EstArgs), vp, ip, dp, ip, dp]
         L.sedi_batch_create.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(_Stencil), ctypes.c_int,
-                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.sedi_batch_solve.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, ip,
                                        dp, ip]
         L.sedi_batch_err_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int]
@@ -232,21 +232,23 @@ def probe_batch(seed, ncells, k0, count):
 
 
 class Batch:
-    """Up to K right-hand sides solved together by CG (Cg.hpp:102-208 per RHS).
+    """Up to K right-hand sides solved together by CG, pipelined CG or
+    BiCGStab (kind; the recurrences of Cg.hpp / PipelinedCg.hpp / BiCgStab.hpp
+    per right-hand side).
 
     precision FP64 keeps the plain solver's arithmetic; FP32_MIXED stores the
     vectors in FP32 with FP64 reductions and an FP64 residual replacement
-    every rep_every iterations (0 = 50).
+    every rep_every iterations (0 = 50) and at every tenfold decrease.
     """
 
-    def __init__(self, grid, K, precision=FP64, rep_every=0, stencil=None):
+    def __init__(self, grid, K, precision=FP64, rep_every=0, stencil=None, kind=CG):
         self.grid = tuple(int(v) for v in grid)
         self.stencil = stencil or Stencil()
         g, self.n = _grid(self.grid)
         self.K = (int(K) + 3) // 4 * 4
         h = ctypes.c_void_p()
-        _check(lib().sedi_batch_create(ctypes.byref(g), ctypes.byref(self.stencil._c()), int(K),
-                                       int(precision), int(rep_every), ctypes.byref(h)))
+        _check(lib().sedi_batch_create(ctypes.byref(g), ctypes.byref(self.stencil._c()), int(kind),
+                                       int(K), int(precision), int(rep_every), ctypes.byref(h)))
         self._h = h
 
     def solve(self, b, maxiter=1000, tol=1e-4):

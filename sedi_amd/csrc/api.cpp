@@ -165,8 +165,8 @@ int sedi_mt_state_after(uint32_t seed, uint64_t draws, uint32_t* state624) {
     });
 }
 
-int sedi_batch_create(const sedi_grid* grid, const sedi_stencil* st, int K, int precision, int rep_every,
-                      sedi_batch** out) {
+int sedi_batch_create(const sedi_grid* grid, const sedi_stencil* st, int kind, int K, int precision,
+                      int rep_every, sedi_batch** out) {
     return guarded([&]() -> int {
         SEDI_REQUIRE(out && st, SEDI_ERR_ARG, "stencil and out must not be NULL");
         *out = nullptr;
@@ -179,7 +179,8 @@ int sedi_batch_create(const sedi_grid* grid, const sedi_stencil* st, int K, int 
         SEDI_REQUIRE(rep_every >= 0, SEDI_ERR_ARG, "rep_every must be >= 0");
         if ((rc = sedi::ensure_device())) return rc;
         sedi_batch* h = new sedi_batch();
-        rc = h->impl.create(s, st->a, st->b, st->c, (K + 3) / 4 * 4, precision == SEDI_FP32_MIXED, rep_every);
+        rc = h->impl.create(s, st->a, st->b, st->c, kind, (K + 3) / 4 * 4, precision == SEDI_FP32_MIXED,
+                            rep_every);
         if (rc) {
             delete h;
             return rc;

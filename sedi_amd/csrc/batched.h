@@ -1,4 +1,5 @@
-// batched.h -- multi-RHS CG over blocks of 4 probes (FP64 or FP32 storage).
+// batched.h -- multi-RHS Krylov solvers over blocks of 4 probes (FP64 or
+// FP32 storage): CG, pipelined CG and BiCGStab with per-RHS scalars.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,9 +20,10 @@ struct BatchResult {
 class BatchSolver {
   public:
     ~BatchSolver();
-    // K: right-hand sides per batch (a multiple of 4); fp32: FP32 vectors with
-    // FP64 reductions and residual replacement every rep_every iterations.
-    int create(const Shape& s, double a, double b, double c, int K, bool fp32, int rep_every);
+    // kind: SEDI_CG, SEDI_PCG or SEDI_BICGSTAB.  K: right-hand sides per batch
+    // (a multiple of 4); fp32: FP32 vectors with FP64 reductions and residual
+    // replacement every rep_every iterations.
+    int create(const Shape& s, double a, double b, double c, int kind, int K, bool fp32, int rep_every);
     // Solves the K systems A x_k = b_k from x = 0, where b is the batch layout
     // [K/4][N][4] of the storage type (device).  Right-hand sides that are
     // all zero are done at once (padding).  Synchronises the stream.
@@ -38,12 +40,15 @@ class BatchSolver {
     int store_x(double* x, int k0, int k1);
     // v: right-hand sides k0 .. k1-1 as stored (rounded to the storage type).
     int store_rhs(double* v, int k0, int k1);
-    void* rhs() const { return b_; }
+    void* rhs() const { return v_[0]; }
     bool fp32() const { return fp32_; }
     int K() const { return K_; }
+    int kind() const { return kind_; }
     long long n() const { return shape_.n(); }
     hipStream_t stream() const { return stream_; }
     std::vector<double> history(int k) const;
+    // Device bytes of the vectors of a batch of K right-hand sides on grid s.
+    static double bytes_needed(const Shape& s, int kind, int K, bool fp32);
 
   private:
     template <class T> int solve_t(int maxiter, double tol, BatchResult* res);
@@ -52,12 +57,12 @@ class BatchSolver {
     template <class T> int unpack_t(double* x, int k0, int k1, bool rhs);
     Shape shape_{};
     double a_ = 6.0, b_c_ = 1.0, c_ = 1.0;
-    int K_ = 0, nrb_ = 0;
+    int kind_ = 0, K_ = 0, nrb_ = 0, nvec_ = 0;
     bool fp32_ = true;
     int rep_every_ = 50;
     hipStream_t stream_ = nullptr;
     DevBuf vec_, ctl_, hist_;
-    void *b_ = nullptr, *r_ = nullptr, *p_[2] = {nullptr, nullptr}, *xlo_ = nullptr;
+    void* v_[12] = {};
     double* xhi_ = nullptr;
     int* pinned_ = nullptr;
     int hist_cap_ = 0;

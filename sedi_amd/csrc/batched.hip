@@ -1,4 +1,7 @@
-// batched.hip -- multi-RHS CG: one stencil sweep serves a block of 4 probes.
+// batched.hip -- multi-RHS Krylov solvers: one stencil sweep serves a block of
+// 4 probes.  This file: the layout, the common kernels (set-up, replacement,
+// fold, accumulation, packing), the CG passes and the host driver; the
+// pipelined CG and BiCGStab passes are in batched_pcg.hip / batched_bicg.hip.
 //
 // Layout: a batch of K right-hand sides is stored as K/4 RHS blocks, each a
 // dense grid of cells holding the 4 probes' values side by side ([K/4][N][4]).
@@ -26,95 +29,26 @@
 // new residual; if the true residual exceeds twice the recursive one the
 // recursion had drifted and the directions restart (beta = 0).  An RHS is done
 // only when its true residual meets the tolerance.  With FP64 storage the
-// recurrences are the plain solver's.
+// recurrences are the plain solvers'.
 //
 // Per-RHS state makes each probe's trajectory independent of the others in
-// its batch: an RHS that is done keeps x and r (alpha = 0), replacements are
-// applied per RHS, and each RHS's sums have a fixed order for a given grid.
-#include "batched.h"
-#include "profiler.h"
+// its batch: an RHS that is done keeps x and r (alpha = beta = 0),
+// replacements are applied per RHS, and each RHS's sums have a fixed order
+// for a given grid.
+#include "batched_common.h"
 
 #include <cmath>
 #include <cstring>
-#include <map>
-#include <mutex>
-#include <tuple>
-
-#ifndef SEDI_PREFETCH_BATCH
-#define SEDI_PREFETCH_BATCH 1
-#endif
 
 namespace sedi {
 
-// ---- 4-wide values ---------------------------------------------------------
-template <class T>
-struct alignas(4 * sizeof(T)) V4 {
-    T v[4];
-    __device__ __forceinline__ T& operator[](int i) { return v[i]; }
-    __device__ __forceinline__ const T& operator[](int i) const { return v[i]; }
-};
-template <class T>
-__device__ __forceinline__ V4<T> operator+(const V4<T>& a, const V4<T>& b) {
-    return {{a[0] + b[0], a[1] + b[1], a[2] + b[2], a[3] + b[3]}};
-}
-template <class T>
-__device__ __forceinline__ V4<T> operator-(const V4<T>& a, const V4<T>& b) {
-    return {{a[0] - b[0], a[1] - b[1], a[2] - b[2], a[3] - b[3]}};
-}
-template <class T>
-__device__ __forceinline__ V4<T> operator*(T s, const V4<T>& a) {
-    return {{s * a[0], s * a[1], s * a[2], s * a[3]}};
-}
-template <class T>
-__device__ __forceinline__ V4<T> operator*(const V4<T>& a, const V4<T>& b) {
-    return {{a[0] * b[0], a[1] * b[1], a[2] * b[2], a[3] * b[3]}};
-}
-
-// Store of a 4-RHS value.  SEDI_BATCH_NT selects non-temporal stores (the
-// vectors written by a pass are not read again until the next pass).
-template <class T>
-__device__ __forceinline__ void stv(V4<T>* base, cidx_t i, const V4<T>& v) {
-#ifdef SEDI_BATCH_NT
-    typedef T vec4 __attribute__((ext_vector_type(4)));
-    const vec4 w = {v[0], v[1], v[2], v[3]};
-    __builtin_nontemporal_store(
-        w, reinterpret_cast<vec4*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(V4<T>))));
-#else
-    stc(base, i, v);
-#endif
-}
-
-// ---- device control --------------------------------------------------------
-struct BCtl {
-    double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *elast, *hist;
-    int *fiter, *code, *done, *rep, *fold;
-    int *iter, *rb_done;
-    double* partials;
-    unsigned* counters;
-    double tol;
-    double delta;    // replacement when the residual dropped by this factor
-    double restart;  // restart when true > restart * recursive residual
-    int maxiter, rep_every, mixed, K;
-    long long hstride;  // history entries per RHS
-    long long pstride;  // partial sums per RHS block
-};
-
-__device__ __forceinline__ void bset_done(const BCtl& c, int k, int code, int it, double e) {
-    c.done[k] = 1;
-    c.code[k] = code;
-    c.fiter[k] = it;
-    c.ferr[k] = e;
-}
-
-__device__ __forceinline__ void brb_update(const BCtl& c, int rb) {
-    c.rb_done[rb] = c.done[4 * rb] & c.done[4 * rb + 1] & c.done[4 * rb + 2] & c.done[4 * rb + 3];
-}
-
 // ---- set-up: r = b (x0 = 0), x = 0, <b,b>, <r, r.*iM> -----------------------
-// (Cg.hpp:111-147 for x = 0: A x = 0, so r = b exactly)
+// Cg.hpp:111-147, PipelinedCg.hpp:113-136 and BiCgStab.hpp:117-163 for x = 0:
+// A x = 0, so r = b exactly; BiCGStab also takes r^ = r (:154).
 template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl c, const V4<T>* b,
-                                                              V4<T>* r, V4<T>* xlo, double* xhi, T im) {
+                                                              V4<T>* r, V4<T>* rhat, V4<T>* xlo,
+                                                              double* xhi, T im) {
     __shared__ double red[4 * 8];
     __shared__ int lastflag;
     const int rb = blockIdx.y;
@@ -128,6 +62,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
          i += (long long)gridDim.x * NTHREADS) {
         const V4<T> v = bq[i];
         rq[i] = v;
+        if (rhat) rhat[(long long)rb * n + i] = v;
         xq[i] = V4<T>{};
         if (xhi)
 #pragma unroll
@@ -150,18 +85,29 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         const double e = sqrt(acc[q] / bb);
         c.bb[k] = bb;
         c.err[k] = e;
+        c.elast[k] = e;
         c.ru[k] = acc[4 + q];
         c.ru_old[k] = acc[4 + q];
         c.alpha[k] = 0.0;
         c.beta[k] = 0.0;
         c.rep[k] = 0;
         c.fold[k] = 0;
+        c.sexit[k] = 0;
         c.done[k] = 0;
-        c.elast[k] = e;
-        if (e < c.tol) bset_done(c, k, 0, 0, e);  // Cg.hpp:130-134
-        else if (c.maxiter < 1) {                 // :196-206 with no iteration run
-            if (e <= c.tol) bset_done(c, k, 0, c.maxiter + 1, e);
-            else bset_done(c, k, 1, c.maxiter, e);
+        if (c.kind == SEDI_BICGSTAB) {
+            c.rho[k] = acc[q];  // <r, r^> with r^ = r (:160)
+            c.rho_prev[k] = 0.0;
+            c.omega[k] = 1.0;
+            c.ss[k] = 0.0;
+            c.cbw[k] = 0.0;
+        }
+        if (e < c.tol) {  // Cg.hpp:130-134 (strict <)
+            bset_done(c, k, 0, 0, e);
+        } else if (c.kind == SEDI_BICGSTAB) {
+            if (c.maxiter < 1) bbicg_outcome(c, k, c.maxiter + 1);
+            else if (c.rho[k] == 0.0) bbicg_outcome(c, k, 1);  // :163
+        } else if (c.maxiter < 1) {
+            bcg_exhausted(c, k, e);
         }
     }
     c.iter[rb] = 1;
@@ -282,14 +228,14 @@ __global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c,
         c.ru_old[k] = c.ru[k];
         c.beta[k] = ru_new / c.ru[k];  // Cg.hpp:185
         c.ru[k] = ru_new;
-        const double e = sqrt(acc[4 + q] / c.bb[k]);  // Cg.hpp:191
+        const double e = sqrt(acc[4 + q] / c.bb[k]);  // :191
         c.err[k] = e;
         c.hist[(long long)k * c.hstride + it - 1] = e;
         if (c.mixed) {
             // converged on the recursive residual, a replacement due, or the
             // last iteration: the FP64 pass decides
             c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter) ||
-                       (e < c.delta * c.elast[k]);
+                       (e < c.drop * c.elast[k]);
         } else if (e <= c.tol) {
             bset_done(c, k, 0, it, e);
         } else if (it >= c.maxiter) {
@@ -300,12 +246,13 @@ __global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c,
     brb_update(c, rb);
 }
 
-// ---- FP64 reliable update: r = b - A (x_hi + x_lo) -------------------------
+// ---- FP64 replacement: r = b - A (x_hi + x_lo) -------------------------------
 // The new residual is computed in FP64 from the FP64 sum of the two parts of
 // x and stored rounded to the vector type.  x itself is read at the halo by
 // neighbouring tiles during this pass, so the fold x_hi += x_lo, x_lo = 0 is
-// left to batch_fold_kernel, which runs after it.
-template <class T>
+// left to batch_fold_kernel, which runs after it.  The second dot is <r, u>
+// (CG) or <r, r^> (BiCGStab, RHAT).
+template <class T, bool RHAT>
 struct PassBRep {
     using V = V4<double>;
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
@@ -314,10 +261,11 @@ struct PassBRep {
     const V4<T>* xlo;
     V4<T>* r;
     const V4<T>* rhs;
+    const V4<T>* rhat;
     T im;
     int rep[4];
     struct Raw { V xh; V4<T> xl; };
-    struct Aux { V4<T> b, r; };
+    struct Aux { V4<T> b, r, rh; };
     __device__ Raw load(cidx_t i) const {
         Raw q;
 #pragma unroll
@@ -325,7 +273,13 @@ struct PassBRep {
         q.xl = ldc(xlo, i);
         return q;
     }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(rhs, i), ldc((const V4<T>*)r, i)}; }
+    __device__ Aux load_aux(cidx_t i) const {
+        Aux o;
+        o.b = ldc(rhs, i);
+        o.r = ldc((const V4<T>*)r, i);
+        if (RHAT) o.rh = ldc(rhat, i);
+        return o;
+    }
     __device__ V f(const Raw& q) const {
         V x;
 #pragma unroll
@@ -338,8 +292,8 @@ struct PassBRep {
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (rep[k]) rn[k] = (T)((double)o.b[k] - Ax[k]);  // Cg.hpp:123, in FP64
-        stc(r, i, rn);
-        const V4<T> u = im * rn;
+        stv(r, i, rn);
+        const V4<T> u = RHAT ? o.rh : im * rn;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             acc[k] += (double)rn[k] * (double)u[k];
@@ -348,11 +302,12 @@ struct PassBRep {
     }
 };
 
-template <class T, int ZC>
+template <class T, bool RHAT, int ZC>
 __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c, double a, double b,
                                                              double cc, long long n, const double* xhi,
-                                                             const V4<T>* xlo, V4<T>* r,
-                                                             const V4<T>* rhs, T im) {
+                                                             const V4<T>* xlo, V4<T>* r0, V4<T>* r1,
+                                                             const V4<T>* rhs, const V4<T>* rhat,
+                                                             T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     const int* rp = c.rep + 4 * rb;
@@ -361,7 +316,10 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     __shared__ double red[8 * 4];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
-    PassBRep<T> P;
+    // the residual the next iteration reads: CG has one, BiCGStab alternates
+    // with the parity of the iteration (r of iteration it+1 = iter[rb])
+    V4<T>* r = (r1 && (c.iter[rb] & 1)) ? r1 : r0;
+    PassBRep<T, RHAT> P;
     P.a = a; P.b = b; P.c = cc;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -371,9 +329,10 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     P.xlo = xlo + off;
     P.r = r + off;
     P.rhs = rhs + off;
+    P.rhat = RHAT ? rhat + off : nullptr;
     P.im = im;
     double acc[8];
-    run_tiles<PassBRep<T>, ZC>(P, g, lds, acc, false);
+    run_tiles<PassBRep<T, RHAT>, ZC>(P, g, lds, acc, false);
     if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
@@ -383,15 +342,28 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
         if (!c.rep[k]) continue;
         c.rep[k] = 0;
         c.fold[k] = 1;
-        c.ru[k] = acc[q];
         const double e = sqrt(acc[4 + q] / c.bb[k]);
-        // beta from the replaced residual, or a restart if the recursion drifted
-        c.beta[k] = e > c.restart * c.err[k] ? 0.0 : acc[q] / c.ru_old[k];
+        const bool drifted = e > c.restart * c.err[k];
         c.err[k] = e;
         c.elast[k] = e;
         c.hist[(long long)k * c.hstride + it - 1] = e;
-        if (e <= c.tol) bset_done(c, k, 0, it, e);
-        else if (it >= c.maxiter) bset_done(c, k, 1, c.maxiter, e);
+        if (!RHAT) {
+            c.ru[k] = acc[q];
+            // beta from the replaced residual, or a restart if the recursion drifted
+            c.beta[k] = drifted ? 0.0 : acc[q] / c.ru_old[k];
+            if (e <= c.tol) bset_done(c, k, 0, it, e);
+            else if (it >= c.maxiter) bset_done(c, k, 1, c.maxiter, e);
+        } else {
+            // BiCgStab.hpp:229-232 and :160-169 of the next iteration on the
+            // replaced residual
+            c.rho[k] = acc[q];
+            if (e <= c.tol) { bbicg_outcome(c, k, it); continue; }
+            if (it >= c.maxiter) { bbicg_outcome(c, k, c.maxiter + 1); continue; }
+            if (c.rho[k] == 0.0) { bbicg_outcome(c, k, it + 1); continue; }
+            const double beta = drifted ? 0.0 : (c.rho[k] / c.rho_prev[k]) * (c.alpha[k] / c.omega[k]);
+            c.beta[k] = beta;
+            c.cbw[k] = -beta * c.omega[k];
+        }
     }
     brb_update(c, rb);
 }
@@ -489,64 +461,78 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int cus() {
-    static int v = 0;
-    if (!v) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
-            v = 256;
+template <class T, bool RHAT>
+static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, const V4<T>* rhat) {
+    const KGeom g = bgeom(batch_rep_kernel<T, RHAT, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const dim3 grid(g.nblocks, r.nrb);
+    const T im = (T)r.im;
+    const double* xhi = r.xhi;
+    const V4<T>* xlo = static_cast<const V4<T>*>(r.v[SLOT_XLO]);
+    const V4<T>* b = static_cast<const V4<T>*>(r.v[SLOT_B]);
+    if (g.tg.zc == 8)
+        return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 8>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
+                      r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+    if (g.tg.zc == 32)
+        return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 32>, grid, dim3(NTHREADS), r.stream, g.tg,
+                      r.c, r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+    return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 16>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
+                  r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+}
+
+template <class T>
+int fold_only(const BRun& r) {
+    SEDI_HIP(launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS),
+                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO])));
+    return SEDI_OK;
+}
+
+// CG: res is r and rhat is null.  BiCGStab: res is r0 and rhat r^; the kernel
+// picks r0 / r1 by the parity of the next iteration.
+template <class T>
+int replace_and_fold(const BRun& r, V4<T>* res, const V4<T>* rhat) {
+    hipError_t e;
+    if (rhat) e = launch_rep<T, true>(r, res, static_cast<V4<T>*>(r.v[BI_R1]), rhat);
+    else e = launch_rep<T, false>(r, res, nullptr, nullptr);
+    if (e != hipSuccess) {
+        set_error("replacement launch failed: %s", hipGetErrorString(e));
+        return SEDI_ERR_HIP;
     }
-    return v;
+    return fold_only<T>(r);
 }
 
-// Geometry of a batched pass, per RHS block: the workgroups resident at once
-// (a multiple of 8).  It depends on the grid and the kernel only, not on the
-// batch size, so the order of every reduction -- and with it each RHS's
-// result -- is the same whichever batch an RHS is solved in.
-template <typename K>
-static KGeom bgeom(K kern16, const Shape& s, int D) {
-    static std::mutex mu;
-    static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
-    const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
-        (void)hipGetLastError();
-        occ = 2;
-    }
-#ifdef SEDI_BATCH_MAXOCC
-    occ = std::min(occ, SEDI_BATCH_MAXOCC);
-#endif
-    occ = std::min(occ, 8);
-    const int per = std::max(8, occ * cus() / 8 * 8);
-#ifdef SEDI_BATCH_ZC
-    KGeom k;
-    k.tg = make_tile_grid(s, SEDI_BATCH_ZC);
-    k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
-    return cache.emplace(key, k).first->second;
-#else
-    return cache.emplace(key, plan_geometry(s, per, D)).first->second;
-#endif
+template <class T>
+static int cg_iteration(const BRun& r) {
+    const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
+    V4<T>* R = static_cast<V4<T>*>(r.v[CG_R]);
+    V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
+    V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
+    V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
+    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
+                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, im));
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, r.nrb), r.stream,
+                             g2.tg, r.c, a, b, cc, r.n, (const V4<T>*)P0, (const V4<T>*)P1, XL, R, im));
+    if (r.mixed) return replace_and_fold<T>(r, R, nullptr);
+    return SEDI_OK;
 }
-
-// Workgroups per RHS block of the elementwise kernels (independent of the
-// batch size, like bgeom).
-static unsigned ew_blocks(long long n) {
-    const long long g = std::min<long long>((n + NTHREADS - 1) / NTHREADS, 2LL * cus());
-    return (unsigned)std::max<long long>(1, g);
-}
-
-#define SEDI_BZC_LAUNCH(name, K, TT, geo, grid, ...)                                             \
-    ((geo).tg.zc == 8    ? launch(name, K<TT, 8>, grid, dim3(NTHREADS), __VA_ARGS__)             \
-     : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(NTHREADS), __VA_ARGS__)            \
-                         : launch(name, K<TT, 16>, grid, dim3(NTHREADS), __VA_ARGS__))
 
 // control block layout (doubles, then ints)
-enum { CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST, CD_NDOUBLE };
-enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_NINT };
+enum {
+    CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
+    CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW,
+    CD_NDOUBLE
+};
+enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_NINT };
+
+static int nvec_of(int kind) {
+    return kind == SEDI_CG ? CG_NVEC : (kind == SEDI_PCG ? PCG_NVEC : BI_NVEC);
+}
+
+double BatchSolver::bytes_needed(const Shape& s, int kind, int K, bool fp32) {
+    const double n = (double)s.n();
+    return n * K * ((fp32 ? 4.0 : 8.0) * nvec_of(kind) + (fp32 ? 8.0 : 0.0));
+}
 
 BatchSolver::~BatchSolver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
@@ -554,35 +540,35 @@ BatchSolver::~BatchSolver() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-int BatchSolver::create(const Shape& s, double a, double b, double c, int K, bool fp32, int rep_every) {
+int BatchSolver::create(const Shape& s, double a, double b, double c, int kind, int K, bool fp32,
+                        int rep_every) {
+    SEDI_REQUIRE(kind == SEDI_CG || kind == SEDI_PCG || kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
+                 "unknown solver kind %d", kind);
     SEDI_REQUIRE(K >= 4 && K % 4 == 0, SEDI_ERR_ARG, "batch size %d must be a positive multiple of 4", K);
     const size_t vb = (fp32 ? 4 : 8) * 4;  // bytes per cell per RHS block
     SEDI_REQUIRE((double)s.n() * vb < 4294967296.0, SEDI_ERR_UNSUPPORTED,
                  "grid too large for the batched path's 32-bit cell offsets");
     shape_ = s;
     a_ = a; b_c_ = b; c_ = c;
+    kind_ = kind;
     K_ = K;
     nrb_ = K / 4;
     fp32_ = fp32;
     rep_every_ = rep_every > 0 ? rep_every : 50;
+    nvec_ = nvec_of(kind);
     SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SEDI_HIP(hipHostMalloc((void**)&pinned_, sizeof(int) * 2 * nrb_, hipHostMallocDefault));
     const size_t n = (size_t)s.n();
     const size_t vec = n * vb * nrb_;
-    const size_t nvec = 5;  // b, r, p0, p1, x_lo
     const size_t hib = fp32 ? sizeof(double) * n * K : 0;
-    SEDI_HIP(vec_.reserve(vec * nvec + hib));
+    SEDI_HIP(vec_.reserve(vec * nvec_ + hib));
     char* base = static_cast<char*>(vec_.p);
-    b_ = base;
-    r_ = base + vec;
-    p_[0] = base + 2 * vec;
-    p_[1] = base + 3 * vec;
-    xlo_ = base + 4 * vec;
-    xhi_ = fp32 ? reinterpret_cast<double*>(base + 5 * vec) : nullptr;
-    SEDI_HIP(hipMemset(vec_.p, 0, vec * nvec + hib));
+    for (int k = 0; k < nvec_; k++) v_[k] = base + k * vec;
+    xhi_ = fp32 ? reinterpret_cast<double*>(base + nvec_ * vec) : nullptr;
+    SEDI_HIP(hipMemset(vec_.p, 0, vec * nvec_ + hib));
     // control: doubles [CD_NDOUBLE][K], ints [CI_NINT][K], iter[nrb], rb_done[nrb],
     // counters[nrb], partials[nrb][12][8*CUs]
-    const size_t pst = 12 * (size_t)(8 * cus());
+    const size_t pst = 12 * (size_t)(8 * batch_cus());
     const size_t bytes = sizeof(double) * CD_NDOUBLE * K + sizeof(int) * (CI_NINT * K + 2 * nrb_) +
                          sizeof(unsigned) * nrb_ + 256 + sizeof(double) * pst * nrb_;
     SEDI_HIP(ctl_.reserve(bytes));
@@ -599,44 +585,57 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
         SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)K * std::max(maxiter, 1)));
         hist_cap_ = std::max(maxiter, 1);
     }
-    BCtl c;
+    BRun run;
+    BCtl& c = run.c;
     double* d = ctl_.as<double>();
     c.bb = d + CD_BB * K; c.ru = d + CD_RU * K; c.ru_old = d + CD_RUOLD * K;
     c.alpha = d + CD_ALPHA * K; c.beta = d + CD_BETA * K; c.err = d + CD_ERR * K; c.ferr = d + CD_FERR * K;
     c.elast = d + CD_ELAST * K;
+    c.gamma = d + CD_GAMMA * K; c.delta = d + CD_DELTA * K; c.errp = d + CD_ERRP * K;
+    c.gamma_prev = d + CD_GAMMAP * K; c.alpha_prev = d + CD_ALPHAP * K;
+    c.rho = d + CD_RHO * K; c.rho_prev = d + CD_RHOP * K; c.omega = d + CD_OMEGA * K; c.ss = d + CD_SS * K;
+    c.cbw = d + CD_CBW * K;
     int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
     c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
-    c.fold = ip + CI_FOLD * K;
+    c.fold = ip + CI_FOLD * K; c.sexit = ip + CI_SEXIT * K;
     c.iter = ip + CI_NINT * K;
     c.rb_done = c.iter + nrb_;
     c.counters = reinterpret_cast<unsigned*>(c.rb_done + nrb_);
     char* pp = reinterpret_cast<char*>(c.counters + nrb_);
     pp += (256 - (reinterpret_cast<uintptr_t>(pp) & 255)) & 255;
     c.partials = reinterpret_cast<double*>(pp);
-    c.pstride = 12LL * 8 * cus();
+    c.pstride = 12LL * 8 * batch_cus();
     c.hist = hist_.as<double>();
     c.hstride = hist_cap_;
     c.tol = tol;
     c.maxiter = maxiter;
     c.rep_every = rep_every_;
-    c.delta = 0.1;
+    c.drop = 0.1;
     c.restart = 2.0;
     c.mixed = fp32_ ? 1 : 0;
     c.K = K;
+    c.kind = kind_;
+    run.shape = shape_;
+    run.n = n;
+    run.nrb = nrb_;
+    run.a = a_; run.b = b_c_; run.cc = c_; run.im = 1.0 / a_;
+    run.stream = stream_;
+    for (int k = 0; k < 12; k++) run.v[k] = v_[k];
+    run.xhi = xhi_;
+    run.mixed = fp32_;
 
-    const T a = (T)a_, b = (T)b_c_, cc = (T)c_, im = (T)(1.0 / a_);
-    V4<T>* B = static_cast<V4<T>*>(b_);
-    V4<T>* R = static_cast<V4<T>*>(r_);
-    V4<T>* P0 = static_cast<V4<T>*>(p_[0]);
-    V4<T>* P1 = static_cast<V4<T>*>(p_[1]);
-    V4<T>* XL = static_cast<V4<T>*>(xlo_);
-
-    const unsigned ginit = ew_blocks(n);
-    SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ginit, nrb_), dim3(NTHREADS), stream_, n,
-                    c, (const V4<T>*)B, R, XL, xhi_, im));
-    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
-    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
-    const KGeom g3 = bgeom(batch_rep_kernel<T, 16>, shape_, SEDI_PREFETCH_BATCH);
+    const T im = (T)(1.0 / a_);
+    V4<T>* B = static_cast<V4<T>*>(v_[SLOT_B]);
+    V4<T>* XL = static_cast<V4<T>*>(v_[SLOT_XLO]);
+    // the first residual: r (CG, pipelined CG) or r1, read by iteration 1 (BiCGStab)
+    V4<T>* R = static_cast<V4<T>*>(kind_ == SEDI_BICGSTAB ? v_[BI_R1] : v_[CG_R]);
+    V4<T>* RH = kind_ == SEDI_BICGSTAB ? static_cast<V4<T>*>(v_[BI_RH]) : nullptr;
+    SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
+                    stream_, n, c, (const V4<T>*)B, R, RH, XL, xhi_, im));
+    if (kind_ == SEDI_PCG) {
+        const int rc = pcg_setup<T>(run);
+        if (rc) return rc;
+    }
     const int chunk = 16;
     int launched = 0, slot = 0;
     bool pending = false;
@@ -646,26 +645,10 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     int rc = SEDI_OK;
     while (launched < maxiter && rc == SEDI_OK) {
         const int cnt = std::min(chunk, maxiter - launched);
-        for (int k = 0; k < cnt; k++) {
-            hipError_t e = SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1,
-                                           dim3(g1.nblocks, nrb_), stream_, g1.tg, c, a, b, cc, n,
-                                           (const V4<T>*)R, P0, P1, im);
-            if (e == hipSuccess)
-                e = SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, nrb_),
-                                    stream_, g2.tg, c, a, b, cc, n, (const V4<T>*)P0, (const V4<T>*)P1, XL,
-                                    R, im);
-            if (e == hipSuccess && fp32_)
-                e = SEDI_BZC_LAUNCH("batch_rep_kernel", batch_rep_kernel, T, g3, dim3(g3.nblocks, nrb_),
-                                    stream_, g3.tg, c, a_, b_c_, c_, n, (const double*)xhi_,
-                                    (const V4<T>*)XL, R, (const V4<T>*)B, im);
-            if (e == hipSuccess && fp32_)
-                e = launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ginit, nrb_), dim3(NTHREADS),
-                           stream_, n, c, xhi_, XL);
-            if (e != hipSuccess) {
-                set_error("batched launch failed: %s", hipGetErrorString(e));
-                rc = SEDI_ERR_HIP;
-                break;
-            }
+        for (int k = 0; k < cnt && rc == SEDI_OK; k++) {
+            if (kind_ == SEDI_CG) rc = cg_iteration<T>(run);
+            else if (kind_ == SEDI_PCG) rc = pcg_iteration<T>(run);
+            else rc = bicg_iteration<T>(run);
         }
         if (rc) break;
         launched += cnt;
@@ -717,7 +700,7 @@ int BatchSolver::accumulate_t(double* t, double* q, int k0, int k1) {
     long long g = (n + 255) / 256;
     if (g > 8192) g = 8192;
     SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n,
-                    k0, k1, (const V4<T>*)xlo_, (const double*)xhi_, (const V4<T>*)b_, t, q));
+                    k0, k1, (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B], t, q));
     return SEDI_OK;
 }
 
@@ -731,7 +714,7 @@ template <class T>
 int BatchSolver::pack_t(const double* b, int kact, long long row_stride) {
     const long long n = shape_.n();
     SEDI_HIP(launch("batch_pack_kernel", batch_pack_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
-                    stream_, n, kact, row_stride, b, static_cast<V4<T>*>(b_)));
+                    stream_, n, kact, row_stride, b, static_cast<V4<T>*>(v_[SLOT_B])));
     return SEDI_OK;
 }
 
@@ -739,7 +722,7 @@ template <class T>
 int BatchSolver::unpack_t(double* x, int k0, int k1, bool rhs) {
     const long long n = shape_.n();
     const int rb0 = k0 / 4, rb1 = (k1 + 3) / 4;
-    const V4<T>* src = static_cast<const V4<T>*>(rhs ? b_ : xlo_) + (long long)rb0 * n;
+    const V4<T>* src = static_cast<const V4<T>*>(rhs ? v_[SLOT_B] : v_[SLOT_XLO]) + (long long)rb0 * n;
     const double* xhi = rhs ? nullptr : xhi_;
     // the kernel indexes RHS blocks from rb0: shift the row range to match
     SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), rb1 - rb0),
@@ -771,5 +754,10 @@ std::vector<double> BatchSolver::history(int k) const {
     h.assign(hist_host_.begin() + (size_t)k * hist_cap_, hist_host_.begin() + (size_t)(k + 1) * hist_cap_);
     return h;
 }
+
+template int replace_and_fold<float>(const BRun&, V4<float>*, const V4<float>*);
+template int replace_and_fold<double>(const BRun&, V4<double>*, const V4<double>*);
+template int fold_only<float>(const BRun&);
+template int fold_only<double>(const BRun&);
 
 }  // namespace sedi

@@ -1,0 +1,261 @@
+// batched_bicg.hip -- batched BiCGStab: per RHS the recurrences, breakdown
+// codes and absolute s-test of BiCgStab.hpp:117-264, in the three fused
+// passes of the plain solver (direction | half step | update).
+//
+// Mixed form: the FP64 replacement r = b - A x (batched.hip) runs after the
+// update when due; the next iteration's rho = <r, r^> and beta are then taken
+// from the replaced residual (Sleijpen & van der Vorst 1996, the reliable
+// updates BiCGStab was analysed with).  The vectors r, p, v alternate between
+// two buffers with the parity of the iteration, because each pass reads them
+// at the halo while writing their successors.
+#include "batched_common.h"
+
+#include <cmath>
+
+namespace sedi {
+
+// ---- direction (BiCgStab.hpp:165-187): p = r + beta p - beta omega v --------
+// (p = r at the first iteration), p^ = p.*iM, v = A p^, <v, r^>
+template <class T>
+struct PassBBicgDir {
+    using V = V4<T>;
+    static constexpr int ND = 4, D = SEDI_PREFETCH_BATCH;
+    T a, b, c;
+    const V *r, *pold, *vold, *rhat;
+    V *pnew, *vnew;
+    T im;
+    V beta, cbw;
+    bool first;
+    struct Raw { V r, p, v; };
+    struct Aux { V rhat; };
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i), ldc(vold, i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(rhat, i)}; }
+    __device__ V pdir(const Raw& q) const { return first ? q.r : q.r + beta * q.p + cbw * q.v; }  // :169
+    __device__ V f(const Raw& q) const { return im * pdir(q); }
+    __device__ void finish(cidx_t i, const Raw& q, const Aux& o, const V&, const V& v,
+                           double (&acc)[ND]) const {
+        stv(pnew, i, pdir(q));
+        stv(vnew, i, v);
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += (double)v[k] * (double)o.rhat[k];
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_dir_kernel(TileGrid g, BCtl c, T a, T b, T cc,
+                                                                  long long n, const V4<T>* r0,
+                                                                  const V4<T>* r1, V4<T>* p0, V4<T>* p1,
+                                                                  V4<T>* v0, V4<T>* v1, const V4<T>* rhat,
+                                                                  T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ double red[4 * 4];
+    __shared__ int lastflag;
+    const int it = c.iter[rb];
+    const int cur = it & 1;
+    const long long off = (long long)rb * n;
+    V4<T> beta, cbw;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool d = c.done[4 * rb + k];
+        beta[k] = d ? (T)0 : (T)c.beta[4 * rb + k];
+        cbw[k] = d ? (T)0 : (T)c.cbw[4 * rb + k];
+    }
+    PassBBicgDir<T> P{a, b, cc, (cur ? r1 : r0) + off, (cur ? p0 : p1) + off, (cur ? v0 : v1) + off,
+                      rhat + off, (cur ? p1 : p0) + off, (cur ? v1 : v0) + off, im, beta, cbw, it == 1};
+    double acc[4];
+    run_tiles<PassBBicgDir<T>, ZC>(P, g, lds, acc, cur == 1);
+    if (!grid_sum<4>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        c.alpha[k] = c.done[k] ? 0.0 : c.rho[k] / acc[q];  // BiCgStab.hpp:187
+    }
+}
+
+// ---- half step (BiCgStab.hpp:190-213): s = r - alpha v, s^ = s.*iM, --------
+// t = A s^, <s,s>, <t,s>, <t,t>; s and t are not stored
+template <class T>
+struct PassBBicgHalf {
+    using V = V4<T>;
+    static constexpr int ND = 12, D = SEDI_PREFETCH_BATCH;
+    T a, b, c;
+    const V *r, *v;
+    T im;
+    V alpha;
+    struct Raw { V r, v; };
+    struct Aux {};
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i)}; }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
+    __device__ V sval(const Raw& q) const { return q.r - alpha * q.v; }
+    __device__ V f(const Raw& q) const { return im * sval(q); }
+    __device__ void finish(cidx_t, const Raw& q, const Aux&, const V&, const V& t, double (&acc)[ND]) const {
+        const V sv = sval(q);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc[k] += (double)sv[k] * (double)sv[k];
+            acc[4 + k] += (double)t[k] * (double)sv[k];
+            acc[8 + k] += (double)t[k] * (double)t[k];
+        }
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, BCtl c, T a, T b, T cc,
+                                                                   long long n, const V4<T>* r0,
+                                                                   const V4<T>* r1, const V4<T>* v0,
+                                                                   const V4<T>* v1, T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ double red[12 * 4];
+    __shared__ int lastflag;
+    const int it = c.iter[rb];
+    const int cur = it & 1;
+    const long long off = (long long)rb * n;
+    V4<T> alpha;
+#pragma unroll
+    for (int k = 0; k < 4; k++) alpha[k] = (T)c.alpha[4 * rb + k];
+    PassBBicgHalf<T> P{a, b, cc, (cur ? r1 : r0) + off, (cur ? v1 : v0) + off, im, alpha};
+    double acc[12];
+    run_tiles<PassBBicgHalf<T>, ZC>(P, g, lds, acc, cur == 0);
+    if (!grid_sum<12>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        if (c.done[k]) continue;
+        c.ss[k] = acc[q];                                // :193-194
+        if (acc[q] < c.tol * c.tol) c.sexit[k] = 1;      // :195 (absolute test)
+        else c.omega[k] = acc[4 + q] / acc[8 + q];       // :213
+    }
+}
+
+// ---- update (BiCgStab.hpp:217-224 and :160 of the next iteration): ---------
+// s, s^, t recomputed; x = x + alpha p^ + omega s^; r = s - omega t; <r,r>,
+// the next rho = <r, r^>.  A RHS that left at the s-test takes x + alpha p^
+// only (:195-199).
+template <class T>
+struct PassBBicgUpd {
+    using V = V4<T>;
+    static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
+    T a, b, c;
+    const V *r, *v, *p, *rhat;
+    V *x, *rnew;
+    T im;
+    V alpha, omega;
+    int sexit[4];
+    struct Raw { V r, v; };
+    struct Aux { V p, x, rhat; };
+    __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i)}; }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(p, i), ldc((const V*)x, i), ldc(rhat, i)}; }
+    __device__ V sval(const Raw& q) const { return q.r - alpha * q.v; }
+    __device__ V f(const Raw& q) const { return im * sval(q); }
+    __device__ void finish(cidx_t i, const Raw& q, const Aux& o, const V& sh, const V& t,
+                           double (&acc)[ND]) const {
+        const V sv = sval(q);
+        const V ph = im * o.p;
+        const V xa = o.x + alpha * ph;
+        const V xv = xa + omega * sh;  // combine3 (:217)
+        const V rv = sv - omega * t;
+        stv(x, i, sel(sexit, xa, xv));
+        stv(rnew, i, rv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc[k] += (double)rv[k] * (double)rv[k];
+            acc[4 + k] += (double)rv[k] * (double)o.rhat[k];
+        }
+    }
+};
+
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BCtl c, T a, T b, T cc,
+                                                                  long long n, V4<T>* r0, V4<T>* r1,
+                                                                  const V4<T>* v0, const V4<T>* v1,
+                                                                  const V4<T>* p0, const V4<T>* p1,
+                                                                  const V4<T>* rhat, V4<T>* x, T im) {
+    const int rb = blockIdx.y;
+    if (c.rb_done[rb]) return;
+    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ double red[8 * 4];
+    __shared__ int lastflag;
+    const int it = c.iter[rb];
+    const int cur = it & 1;
+    const long long off = (long long)rb * n;
+    PassBBicgUpd<T> P;
+    P.a = a; P.b = b; P.c = cc;
+    P.r = (cur ? r1 : r0) + off;
+    P.v = (cur ? v1 : v0) + off;
+    P.p = (cur ? p1 : p0) + off;
+    P.rhat = rhat + off;
+    P.x = x + off;
+    P.rnew = (cur ? r0 : r1) + off;
+    P.im = im;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int kk = 4 * rb + k;
+        const bool d = c.done[kk];
+        P.alpha[k] = d ? (T)0 : (T)c.alpha[kk];
+        P.sexit[k] = !d && c.sexit[kk];
+        P.omega[k] = (d || P.sexit[k]) ? (T)0 : (T)c.omega[kk];
+    }
+    double acc[8];
+    run_tiles<PassBBicgUpd<T>, ZC>(P, g, lds, acc, cur == 1);
+    if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+        threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 4; q++) {
+        const int k = 4 * rb + q;
+        if (c.done[k]) continue;
+        if (c.sexit[k]) { bbicg_outcome(c, k, it); continue; }  // :195-199
+        const double err = sqrt(acc[q] / c.bb[k]);  // :223-227
+        c.err[k] = err;
+        c.hist[(long long)k * c.hstride + it - 1] = err;
+        if (c.omega[k] == 0.0) { bbicg_outcome(c, k, it); continue; }  // :229-230
+        if (!c.mixed && err <= c.tol) { bbicg_outcome(c, k, it); continue; }
+        if (!c.mixed && it >= c.maxiter) { bbicg_outcome(c, k, c.maxiter + 1); continue; }
+        c.rho_prev[k] = c.rho[k];  // :232
+        c.rho[k] = acc[4 + q];     // next iteration's :160
+        const double beta = (c.rho[k] / c.rho_prev[k]) * (c.alpha[k] / c.omega[k]);  // :167
+        c.beta[k] = beta;
+        c.cbw[k] = -beta * c.omega[k];  // the combine3 coefficient of :169
+        if (c.mixed) {
+            // the FP64 pass decides convergence, maxiter and rho breakdown
+            c.rep[k] = (err <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter) ||
+                       (err < c.drop * c.elast[k]) || (c.rho[k] == 0.0);
+        } else if (c.rho[k] == 0.0) {
+            bbicg_outcome(c, k, it + 1);  // :163
+        }
+    }
+    c.iter[rb] = it + 1;
+    brb_update(c, rb);
+}
+
+// ---- host ------------------------------------------------------------------
+template <class T>
+int bicg_iteration(const BRun& r) {
+    const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
+    auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
+    const KGeom g1 = bgeom(batch_bicg_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g2 = bgeom(batch_bicg_half_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g3 = bgeom(batch_bicg_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_dir_kernel", batch_bicg_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb),
+                             r.stream, g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
+                             V(BI_P0), V(BI_P1), V(BI_V0), V(BI_V1), (const V4<T>*)V(BI_RH), im));
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_half_kernel", batch_bicg_half_kernel, T, g2, dim3(g2.nblocks, r.nrb),
+                             r.stream, g2.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
+                             (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), im));
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_kernel, T, g3, dim3(g3.nblocks, r.nrb),
+                             r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1), (const V4<T>*)V(BI_V0),
+                             (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0), (const V4<T>*)V(BI_P1),
+                             (const V4<T>*)V(BI_RH), V(SLOT_XLO), im));
+    if (r.mixed) return replace_and_fold<T>(r, V(BI_R0), (const V4<T>*)V(BI_RH));
+    return SEDI_OK;
+}
+
+template int bicg_iteration<float>(const BRun&);
+template int bicg_iteration<double>(const BRun&);
+
+}  // namespace sedi

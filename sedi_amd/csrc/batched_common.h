@@ -1,0 +1,222 @@
+// batched_common.h -- what the batched CG, pipelined CG and BiCGStab passes
+// share: the 4-RHS value types, the per-RHS control block, launch geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "batched.h"
+#include "march.h"
+#include "profiler.h"
+
+#ifndef SEDI_PREFETCH_BATCH
+#define SEDI_PREFETCH_BATCH 1
+#endif
+
+namespace sedi {
+
+// ---- 4-wide values: one cell of one RHS block ------------------------------
+template <class T>
+struct alignas(4 * sizeof(T)) V4 {
+    T v[4];
+    __device__ __forceinline__ T& operator[](int i) { return v[i]; }
+    __device__ __forceinline__ const T& operator[](int i) const { return v[i]; }
+};
+template <class T>
+__device__ __forceinline__ V4<T> operator+(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] + b[0], a[1] + b[1], a[2] + b[2], a[3] + b[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator-(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] - b[0], a[1] - b[1], a[2] - b[2], a[3] - b[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator*(T s, const V4<T>& a) {
+    return {{s * a[0], s * a[1], s * a[2], s * a[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> operator*(const V4<T>& a, const V4<T>& b) {
+    return {{a[0] * b[0], a[1] * b[1], a[2] * b[2], a[3] * b[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<double> to_f64(const V4<T>& a) {
+    return {{(double)a[0], (double)a[1], (double)a[2], (double)a[3]}};
+}
+template <class T>
+__device__ __forceinline__ V4<T> from_f64(const V4<double>& a) {
+    return {{(T)a[0], (T)a[1], (T)a[2], (T)a[3]}};
+}
+
+// Two FP64 fields marched together (the replacement passes of pipelined CG
+// apply the stencil to two vectors in one sweep).
+struct D2 {
+    V4<double> a, b;
+};
+__device__ __forceinline__ D2 operator+(const D2& x, const D2& y) { return {x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ D2 operator-(const D2& x, const D2& y) { return {x.a - y.a, x.b - y.b}; }
+__device__ __forceinline__ D2 operator*(double s, const D2& x) { return {s * x.a, s * x.b}; }
+
+// Store of a 4-RHS value.  SEDI_BATCH_NT selects non-temporal stores (the
+// vectors written by a pass are not read again until the next pass).
+template <class T>
+__device__ __forceinline__ void stv(V4<T>* base, cidx_t i, const V4<T>& v) {
+#ifdef SEDI_BATCH_NT
+    typedef T vec4 __attribute__((ext_vector_type(4)));
+    const vec4 w = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(
+        w, reinterpret_cast<vec4*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(V4<T>))));
+#else
+    stc(base, i, v);
+#endif
+}
+
+// Per-lane select: a where m[k], else b.
+template <class T>
+__device__ __forceinline__ V4<T> sel(const int (&m)[4], const V4<T>& a, const V4<T>& b) {
+    return {{m[0] ? a[0] : b[0], m[1] ? a[1] : b[1], m[2] ? a[2] : b[2], m[3] ? a[3] : b[3]}};
+}
+
+// ---- device control --------------------------------------------------------
+// Per-RHS scalars live in device memory and are updated by the last
+// workgroup of each pass of the RHS's block (grid_sum), like the plain
+// solvers' DevState.
+struct BCtl {
+    // per RHS (K entries each)
+    double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *elast, *hist;
+    double *gamma, *delta, *errp, *gamma_prev, *alpha_prev;  // pipelined CG
+    double *rho, *rho_prev, *omega, *ss, *cbw;               // BiCGStab
+    int *fiter, *code, *done, *rep, *fold, *sexit;
+    // per RHS block
+    int *iter, *rb_done;
+    double* partials;
+    unsigned* counters;
+    double tol;
+    double drop;     // replacement when the residual dropped by this factor
+    double restart;  // restart when true > restart * recursive residual
+    int maxiter, rep_every, mixed, K, kind;
+    long long hstride;  // history entries per RHS
+    long long pstride;  // partial sums per RHS block
+};
+
+__device__ __forceinline__ void bset_done(const BCtl& c, int k, int code, int it, double e) {
+    c.done[k] = 1;
+    c.code[k] = code;
+    c.fiter[k] = it;
+    c.ferr[k] = e;
+}
+
+__device__ __forceinline__ void brb_update(const BCtl& c, int rb) {
+    c.rb_done[rb] = c.done[4 * rb] & c.done[4 * rb + 1] & c.done[4 * rb + 2] & c.done[4 * rb + 3];
+}
+
+// Cg.hpp:196-206 / PipelinedCg.hpp:232-242 when the loop ran out.
+__device__ __forceinline__ void bcg_exhausted(const BCtl& c, int k, double e) {
+    if (e <= c.tol) bset_done(c, k, 0, c.maxiter + 1, e);
+    else bset_done(c, k, 1, c.maxiter, e);
+}
+
+// BiCgStab.hpp:235-263 for RHS k, `iter` = the loop variable when it ended.
+__device__ __forceinline__ void bbicg_outcome(const BCtl& c, int k, int iter) {
+    const double e = c.err[k];
+    if (e <= c.tol) bset_done(c, k, 0, iter, e);
+    else if (c.ss[k] <= c.tol * c.tol) bset_done(c, k, 0, iter, sqrt(c.ss[k] / c.bb[k]));
+    else if (c.omega[k] == 0.0) bset_done(c, k, 3, iter, e);
+    else if (c.rho[k] == 0.0) bset_done(c, k, 2, iter, e);
+    else bset_done(c, k, 1, c.maxiter, e);
+}
+
+// ---- host side -------------------------------------------------------------
+inline int batch_cus() {
+    static int v = 0;
+    if (!v) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+    }
+    return v;
+}
+
+// Geometry of a batched pass, per RHS block: the workgroups resident at once
+// (a multiple of 8).  It depends on the grid and the kernel only, not on the
+// batch size, so the order of every reduction -- and with it each RHS's
+// result -- is the same whichever batch an RHS is solved in.
+template <typename Kern>
+KGeom bgeom(Kern kern16, const Shape& s, int D) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
+        (void)hipGetLastError();
+        occ = 2;
+    }
+#ifdef SEDI_BATCH_MAXOCC
+    occ = std::min(occ, SEDI_BATCH_MAXOCC);
+#endif
+    occ = std::min(occ, 8);
+    const int per = std::max(8, occ * batch_cus() / 8 * 8);
+#ifdef SEDI_BATCH_ZC
+    KGeom k;
+    k.tg = make_tile_grid(s, SEDI_BATCH_ZC);
+    k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
+    return cache.emplace(key, k).first->second;
+#else
+    return cache.emplace(key, plan_geometry(s, per, D)).first->second;
+#endif
+}
+
+// Workgroups per RHS block of the elementwise kernels (independent of the
+// batch size, like bgeom).
+inline unsigned ew_blocks(long long n) {
+    const long long g = std::min<long long>((n + NTHREADS - 1) / NTHREADS, 2LL * batch_cus());
+    return (unsigned)std::max<long long>(1, g);
+}
+
+// kernel<TT, ZC> for the ZC the geometry chose
+#define SEDI_BZC_LAUNCH(name, K, TT, geo, grid, ...)                                             \
+    ((geo).tg.zc == 8    ? launch(name, K<TT, 8>, grid, dim3(NTHREADS), __VA_ARGS__)             \
+     : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(NTHREADS), __VA_ARGS__)            \
+                         : launch(name, K<TT, 16>, grid, dim3(NTHREADS), __VA_ARGS__))
+
+// Vector slots of a batch (each [K/4][N] of V4<T>).
+enum { SLOT_B = 0, SLOT_XLO = 1, SLOT_K0 = 2 };
+// CG: r, p0, p1
+enum { CG_R = SLOT_K0, CG_P0, CG_P1, CG_NVEC };
+// pipelined CG: r, u, w0, w1, z, q, s, p
+enum { PCG_R = SLOT_K0, PCG_U, PCG_W0, PCG_W1, PCG_Z, PCG_Q, PCG_S, PCG_P, PCG_NVEC };
+// BiCGStab: r0, r1, r^, p0, p1, v0, v1
+enum { BI_R0 = SLOT_K0, BI_R1, BI_RH, BI_P0, BI_P1, BI_V0, BI_V1, BI_NVEC };
+
+// What a solver's launch functions need.
+struct BRun {
+    Shape shape;
+    long long n;
+    int nrb;
+    BCtl c;
+    double a, b, cc, im;
+    hipStream_t stream;
+    void* v[12];
+    double* xhi;  // FP64 part of x ([K][N]; mixed form only)
+    bool mixed;
+};
+
+// Per-solver kernel sets (batched_pcg.hip, batched_bicg.hip); each launches
+// the set-up passes after the common initialisation, or one iteration
+// including the mixed form's replacement and fold.
+template <class T> int pcg_setup(const BRun& r);
+template <class T> int pcg_iteration(const BRun& r);
+template <class T> int bicg_iteration(const BRun& r);
+// the FP64 replacement r = b - A x with <r,u> (CG) or <r,r^> (BiCGStab) and
+// the fold x_hi += x_lo that follows it (batched.hip)
+template <class T> int replace_and_fold(const BRun& r, V4<T>* res, const V4<T>* rhat);
+template <class T> int fold_only(const BRun& r);
+
+}  // namespace sedi

@@ -75,13 +75,13 @@ int Estimator::create(const sedi_estimate_args& a) {
     const size_t n = (size_t)shape_.n();
     batched_ = a.batch_k > 1 || a.precision == SEDI_FP32_MIXED;
     if (batched_) {
-        SEDI_REQUIRE(a.kind == SEDI_CG, SEDI_ERR_UNSUPPORTED,
-                     "the batched and mixed-precision estimator runs CG only");
+        SEDI_REQUIRE(a.kind == SEDI_CG || a.kind == SEDI_PCG || a.kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
+                     "unknown solver kind %d", a.kind);
         // blocks of 4 probes, no more than the run needs
         int K = std::max(4, (a.batch_k + 3) / 4 * 4);
         K = std::min(K, std::max(4, (a.samples + 3) / 4 * 4));
         batch_.reset(new BatchSolver());
-        int rc = batch_->create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, K,
+        int rc = batch_->create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, a.kind, K,
                                 a.precision == SEDI_FP32_MIXED, a.rep_every);
         if (rc) return rc;
         SEDI_HIP(buf_.reserve(sizeof(double) * n * 2));

@@ -113,6 +113,108 @@ def test_batch_eigenvector_one_iteration():
     assert np.allclose(x[0], phi / lam, atol=1e-14)
 
 
+# ------------------------------------------- pipelined CG and BiCGStab ------
+KINDS = [O.CG, O.PCG, O.BICGSTAB]
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+def test_batch_fp64_other_solvers_fixed_iterations_vs_oracle(kind):
+    b = probes(1234, S32, 0, 6)
+    bs = sedi.Batch(S32, 8, sedi.FP64, kind=kind)
+    x, iters, errs, codes = bs.solve(b, maxiter=20, tol=1e-30)
+    for k in range(6):
+        code, ox, ofe, ofi, oh = O.solve(kind, S32, b[k], maxiter=20, tol=1e-30)
+        assert (codes[k], iters[k]) == (code, ofi) == (1, 20)
+        # BiCGStab's rounding differences grow along its iterations
+        bar = 1e-11 if kind == O.PCG else 1e-9
+        assert normwise(x[k], ox) < bar
+        h = bs.err_history(k)
+        assert len(h) == len(oh) and np.max(np.abs(h - oh) / oh) < 10 * bar
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+def test_batch_fp64_other_solvers_converged_vs_oracle(kind):
+    b = probes(1234, S32, 0, 8)
+    x, iters, errs, codes = sedi.Batch(S32, 8, sedi.FP64, kind=kind).solve(b, maxiter=500, tol=1e-9)
+    assert np.all(codes == 0)
+    for k in range(8):
+        code, ox, ofe, ofi, oh = O.solve(kind, S32, b[k], maxiter=500, tol=1e-9)
+        if kind == O.PCG:
+            assert abs(int(iters[k]) - ofi) <= 1
+        else:  # rounding differences grow along BiCGStab's iterations
+            assert abs(int(iters[k]) - ofi) <= max(3, ofi // 5)
+        assert true_rel_residual(S32, b[k], x[k]) < 3e-9
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+@pytest.mark.parametrize("prec", [sedi.FP64, sedi.FP32_MIXED])
+def test_batch_other_solvers_zero_rhs_and_maxiter(kind, prec):
+    shape = (8, 8, 8)
+    b = np.stack([O.rademacher(3, 0, 512), np.zeros(512), O.rademacher(3, 1024, 512)])
+    bs = sedi.Batch(shape, 4, prec, kind=kind)
+    x, iters, errs, codes = bs.solve(b, maxiter=10, tol=1e-12)
+    assert (codes[1], iters[1], errs[1]) == (0, 0, 0.0) and not x[1].any()
+    for m in (0, 1, 3):
+        x, iters, errs, codes = bs.solve(b, maxiter=m, tol=1e-12)
+        ocode, ox, ofe, ofi, oh = O.solve(kind, shape, b[0], maxiter=m, tol=1e-12)
+        assert (codes[0], iters[0]) == (ocode, ofi), (m, codes[0], iters[0], ocode, ofi)
+        assert (codes[1], iters[1]) == (0, 0)
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+@pytest.mark.parametrize("shape,tol", [(S32, 1e-8), (S32, 1e-10), ((12, 10, 8), 1e-12)])
+def test_batch_mixed_other_solvers_converge_on_true_residual(kind, shape, tol):
+    b = probes(77, shape, 0, 8)
+    x, iters, errs, codes = sedi.Batch(shape, 8, sedi.FP32_MIXED, kind=kind).solve(b, maxiter=2000, tol=tol)
+    xd, itd, _, cd = sedi.Batch(shape, 8, sedi.FP64, kind=kind).solve(b, maxiter=2000, tol=tol)
+    assert np.all(codes == 0), (iters, errs, codes)
+    for k in range(8):
+        t = true_rel_residual(shape, b[k], x[k])
+        # the error reported is the replaced (true) residual's; pipelined CG
+        # stops on the residual of the previous iteration, so its final x is
+        # one update further and its residual smaller still
+        assert errs[k] <= tol and t <= 1.05 * tol, (k, errs[k], t)
+        assert iters[k] <= itd[k] + max(8, itd[k] // 5), (iters[k], itd[k])
+        assert normwise(x[k], xd[k]) < max(1e-6, 1e3 * tol)
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+def test_result_same_alone_and_in_the_mix_other_solvers(kind):
+    shape = (20, 18, 16)
+    b = probes(5, shape, 0, 6)
+    x8, it8, e8, c8 = sedi.Batch(shape, 8, sedi.FP32_MIXED, kind=kind).solve(b, maxiter=600, tol=1e-9)
+    bs1 = sedi.Batch(shape, 4, sedi.FP32_MIXED, kind=kind)
+    for k in (1, 4):
+        x1, it1, e1, c1 = bs1.solve(b[k], maxiter=600, tol=1e-9)
+        assert np.array_equal(x1[0], x8[k]) and (it1[0], e1[0], c1[0]) == (it8[k], e8[k], c8[k])
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+def test_estimator_batched_other_solvers_vs_reference(ref32, kind):
+    r = sedi.estimate(S32, kind=kind, samples=8, maxiter=500, tol=1e-9, seed=1234, batch_k=8)
+    ref_it = ref32[f"s{kind}_iters"]
+    assert r["status"] == 0 and np.all(r["codes"] == 0)
+    if kind == O.PCG:
+        assert np.all(np.abs(r["iters"] - ref_it) <= 1), (r["iters"], ref_it)
+        assert normwise(r["D"], ref32[f"s{kind}_D"]) < (1e-10 if np.array_equal(r["iters"], ref_it) else 1e-8)
+    else:
+        assert np.all(np.abs(r["iters"] - ref_it) <= np.maximum(3, ref_it // 5)), (r["iters"], ref_it)
+        assert normwise(r["D"], ref32[f"s{kind}_D"]) < 1e-6
+    f = sedi.estimate(S32, kind=kind, samples=8, maxiter=20, tol=1e-9, seed=1234, batch_k=8,
+                      ignore_errors=True)
+    assert np.array_equal(f["iters"], ref32[f"s{kind}_fixed20_iters"])
+    assert normwise(f["D"], ref32[f"s{kind}_fixed20_D"]) < 1e-10
+
+
+@pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
+def test_estimator_mixed_other_solvers_vs_fp64(kind):
+    p = sedi.estimate(S32, kind=kind, samples=8, maxiter=500, tol=1e-10, seed=1234)
+    m = sedi.estimate(S32, kind=kind, samples=8, maxiter=500, tol=1e-10, seed=1234,
+                      precision=sedi.FP32_MIXED, batch_k=8)
+    assert m["status"] == 0 and np.all(m["codes"] == 0)
+    assert normwise(m["D"], p["D"]) < 1e-6
+
+
 # ------------------------------------------------------------------ mixed ----
 @pytest.mark.parametrize("shape,tol,rep", [(S32, 1e-8, 0), (S32, 1e-9, 0), (S32, 1e-9, 10), (S32, 1e-12, 0),
                                            ((12, 10, 8), 1e-8, 0), ((12, 10, 8), 1e-11, 0),
@@ -238,6 +340,6 @@ def test_estimator_batched_log_lines():
     assert len(lines) == 2 + int(r["iters"].sum())
 
 
-def test_estimator_batched_rejects_other_solvers():
+def test_estimator_batched_rejects_unknown_solver():
     with pytest.raises(sedi.SediError):
-        sedi.Estimator((8, 8, 8), kind=sedi.PCG, samples=4, batch_k=4)
+        sedi.Estimator((8, 8, 8), kind=7, samples=4, batch_k=4)
