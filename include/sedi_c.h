@@ -136,9 +136,9 @@ typedef struct sedi_estimate_args {
                                above 1 (or with SEDI_FP32_MIXED): the batched
                                solvers, probes in blocks of 4 (padded with
                                zero right-hand sides)                         */
-    int rep_every;          /* SEDI_FP32_MIXED: FP64 residual replacement
-                               every rep_every iterations (0 = 50), besides
-                               the tenfold-decrease and convergence triggers  */
+    int rep_every;          /* SEDI_FP32_MIXED: CG's FP64 residual
+                               replacement period (0 = 50); see the batched
+                               solvers below for the other triggers           */
 } sedi_estimate_args;
 
 /* Allocates everything the run needs (excluded from the timed region, as the
@@ -194,11 +194,15 @@ int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
  * kept per right-hand side: one stencil sweep per pass serves every
  * right-hand side, with per-RHS scalars, convergence and masks.
  *   precision SEDI_FP64: FP64 vectors, as the plain solver;
- *   SEDI_FP32_MIXED: FP32 vectors and stencil, FP64 dot products, scalars and
- *   solution accumulator, with an FP64 residual replacement r = b - A x every
- *   rep_every iterations (0 = 50), whenever the residual has dropped tenfold
- *   since the last replacement (reliable updates), and before a right-hand
- *   side is declared converged (its returned err is then the true residual's).
+ *   SEDI_FP32_MIXED: FP32 vectors, FP64 dot products, scalars and solution
+ *   accumulator (CG: FP32 stencil and updates; pipelined CG and BiCGStab:
+ *   FP64 arithmetic on the FP32-stored vectors), with an FP64 residual
+ *   replacement r = b - A x before a right-hand side is declared converged
+ *   (its returned err is then the true residual's) and: CG every rep_every
+ *   iterations (0 = 50) and at each tenfold decrease; pipelined CG the same
+ *   but at least every 10 iterations (all of r, u, w, s, q, z are replaced);
+ *   BiCGStab when the residual has dropped by sqrt(2^-24) since the last
+ *   replacement, restarting r^ and the directions there.
  * Results of a right-hand side do not depend on the others in its batch or on
  * K.  K is rounded up to a multiple of 4. */
 typedef struct sedi_batch sedi_batch;
@@ -218,6 +222,53 @@ void sedi_batch_destroy(sedi_batch *batch);
  * [count][ncells] doubles (host or device), each generated from its own
  * jump-ahead start: equal to sedi_rademacher_fill(seed, 2*ncells*k, ...). */
 int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double *v);
+
+/* ---- serving (no reference counterpart) ----------------------------------- */
+/* A server accepts estimation jobs at any time and runs them concurrently on
+ * the GPU with continuous batching: each (grid, solver, stencil) has a pool of
+ * right-hand-side slots in blocks of 4 (the batched solvers above), a block
+ * of a new job starts as soon as a slot is free while the pool's other blocks
+ * are mid-solve, and pools of different grids run on their own streams.  Jobs
+ * are admitted in arrival order against a device memory budget.  A job's D
+ * is accumulated in probe order and equals sedi_estimate's batched result for
+ * the same job bitwise, whatever else is in flight.  Thread-safe submit and
+ * wait; one engine thread owns the device. */
+typedef struct sedi_server sedi_server;
+typedef struct sedi_server_config {
+    double hbm_budget_bytes; /* device memory for pools and jobs             */
+    int precision;           /* SEDI_FP32_MIXED or SEDI_FP64                 */
+    int rep_every;           /* mixed: replacement period (0 = 50)           */
+    int chunk;               /* iterations per pool step (0 = 8)             */
+    int max_slots;           /* right-hand sides per pool at most (0 = 64)   */
+    int max_iter;            /* largest maxiter a job may ask for (0 = 10000) */
+} sedi_server_config;
+typedef struct sedi_job {
+    sedi_grid grid;
+    sedi_stencil stencil;
+    int kind;       /* SEDI_CG / SEDI_PCG / SEDI_BICGSTAB                    */
+    int samples;    /* probes                                                */
+    double tol;
+    int maxiter;
+    uint32_t seed;  /* the job's probe stream (the estimator's convention)   */
+} sedi_job;
+int sedi_server_create(const sedi_server_config *cfg, sedi_server **out);
+/* Returns the job id (>= 0) or a negative error code. */
+int64_t sedi_server_submit(sedi_server *srv, const sedi_job *job);
+/* Waits for a job (timeout_s < 0: no limit).  Returns 0 when it finished
+ * (then *status is 0, or a negative error code if the job failed, and its
+ * result is released), 1 on timeout, < 0 on a bad call.  D (n cells),
+ * iters/errs/codes (per probe) and times4 = {submitted, admitted, first
+ * block started, done} in seconds since the server started may be NULL. */
+int sedi_server_wait(sedi_server *srv, int64_t job, double timeout_s, double *D, int *iters,
+                     double *errs, int *codes, double *times4, int *status);
+typedef struct sedi_server_info {
+    int64_t submitted, done, probes;
+    double bytes_in_use, peak_bytes;
+    int pools;
+} sedi_server_info;
+int sedi_server_stats(sedi_server *srv, sedi_server_info *out);
+/* Finishes every submitted job, then stops. */
+void sedi_server_destroy(sedi_server *srv);
 
 /* ---- validation archives (HDF5) ------------------------------------------- */
 /* The reference's validation archives (ValidateSolver.hpp:30-85: datasets
@@ -243,6 +294,14 @@ int sedi_profile_enable(const char *kernel_name);
  * sedi_profile_reset().  Returns 0, or SEDI_ERR_ARG if it was never timed. */
 int sedi_profile_get(const char *kernel_name, int64_t *launches, double *total_ms);
 int sedi_profile_reset(void);
+
+/* The solvers launch their iterations in chunks; from the second chunk of a
+ * solve on, a chunk is captured once into a HIP graph and replayed (the
+ * kernels read every per-iteration value from device memory, so one graph
+ * serves every chunk).  This turns it off (0) or on (1, the default); kernels
+ * timed with sedi_profile_enable are never captured.  No reference
+ * counterpart (the reference's iterations are host loops). */
+int sedi_graphs_enable(int on);
 
 #ifdef __cplusplus
 }

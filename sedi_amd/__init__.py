@@ -26,8 +26,9 @@ import numpy as np
 __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEGA",
            "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
            "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
-           "Estimator", "estimate", "Batch", "probe_batch", "h5_available", "h5_read", "h5_write",
-           "HOOK_X", "HOOK_V", "HOOK_D", "profile_enable", "profile_get", "profile_reset"]
+           "Estimator", "estimate", "Batch", "probe_batch", "Server", "h5_available", "h5_read", "h5_write",
+           "HOOK_X", "HOOK_V", "HOOK_D", "graphs_enable", "profile_enable", "profile_get",
+           "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 # SEDI_AMD_LIB selects another build of the library (e.g. a tuning variant)
@@ -59,6 +60,21 @@ class _EstArgs(ctypes.Structure):
                 ("tol", ctypes.c_double), ("ignore_errors", ctypes.c_int),
                 ("seed", ctypes.c_uint32), ("precision", ctypes.c_int), ("batch_k", ctypes.c_int),
                 ("rep_every", ctypes.c_int)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("precision", ctypes.c_int), ("rep_every", ctypes.c_int),
+                ("chunk", ctypes.c_int), ("max_slots", ctypes.c_int), ("max_iter", ctypes.c_int)]
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("grid", _Grid), ("stencil", _Stencil), ("kind", ctypes.c_int), ("samples", ctypes.c_int),
+                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("seed", ctypes.c_uint32)]
+
+
+class _ServerInfo(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("done", ctypes.c_int64), ("probes", ctypes.c_int64),
+                ("bytes_in_use", ctypes.c_double), ("peak_bytes", ctypes.c_double), ("pools", ctypes.c_int)]
 
 
 LOG_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
@@ -106,6 +122,14 @@ def lib():
         L.sedi_h5_read.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_Grid), vp]
         L.sedi_h5_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                     ctypes.POINTER(vp), ctypes.POINTER(_Grid)]
+        L.sedi_graphs_enable.argtypes = [ctypes.c_int]
+        L.sedi_server_create.argtypes = [ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
+        L.sedi_server_submit.argtypes = [vp, ctypes.POINTER(_Job)]
+        L.sedi_server_submit.restype = ctypes.c_int64
+        L.sedi_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, vp, ip, dp, ip, dp, ip]
+        L.sedi_server_stats.argtypes = [vp, ctypes.POINTER(_ServerInfo)]
+        L.sedi_server_destroy.argtypes = [vp]
+        L.sedi_server_destroy.restype = None
         L.sedi_profile_enable.argtypes = [ctypes.c_char_p]
         L.sedi_profile_get.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64), dp]
         _lib = L
@@ -372,6 +396,66 @@ def estimate(grid, kind=CG, samples=8, maxiter=1000, tol=1e-4, seed=1234, ignore
         est.close()
 
 
+class Server:
+    """Continuous-batching estimation server (sedi_server_*): submit jobs at
+    any time, wait for their results; see include/sedi_c.h."""
+
+    def __init__(self, hbm_budget=200e9, precision=FP32_MIXED, rep_every=0, chunk=0, max_slots=0,
+                 max_iter=0):
+        cfg = _ServerConfig(float(hbm_budget), int(precision), int(rep_every), int(chunk), int(max_slots),
+                            int(max_iter))
+        h = ctypes.c_void_p()
+        _check(lib().sedi_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._jobs = {}
+
+    def submit(self, grid, kind=CG, samples=8, tol=1e-8, maxiter=2000, seed=0, stencil=None):
+        """Returns the job id."""
+        g, n = _grid(grid)
+        st = stencil or Stencil()
+        job = _Job(g, st._c(), int(kind), int(samples), float(tol), int(maxiter), int(seed) & 0xffffffff)
+        jid = lib().sedi_server_submit(self._h, ctypes.byref(job))
+        _check(jid if jid < 0 else 0)
+        self._jobs[jid] = (n, int(samples))
+        return jid
+
+    def wait(self, jid, timeout=-1.0):
+        """dict(status, D, iters, errs, codes, times) or None on timeout."""
+        n, s = self._jobs[jid]
+        D = np.zeros(n)
+        iters = np.zeros(s, dtype=np.int32)
+        codes = np.full(s, -1, dtype=np.int32)
+        errs = np.zeros(s)
+        times = np.zeros(4)
+        status = ctypes.c_int()
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        rc = _check(lib().sedi_server_wait(self._h, int(jid), float(timeout), _ptr(D), iters.ctypes.data_as(ip),
+                                           errs.ctypes.data_as(dp), codes.ctypes.data_as(ip),
+                                           times.ctypes.data_as(dp), ctypes.byref(status)))
+        if rc == 1:
+            return None
+        del self._jobs[jid]
+        return dict(status=status.value, D=D, iters=iters, errs=errs, codes=codes,
+                    times=dict(submit=times[0], admit=times[1], start=times[2], done=times[3]))
+
+    def stats(self):
+        info = _ServerInfo()
+        _check(lib().sedi_server_stats(self._h, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in _ServerInfo._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sedi_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def h5_available():
     """True if libhdf5 can be loaded (validation archives)."""
     return lib().sedi_h5_available() == 1
@@ -393,6 +477,11 @@ def h5_write(path, grid, **datasets):
     cn = (ctypes.c_char_p * len(names))(*[k.encode() for k in names])
     ca = (ctypes.c_void_p * len(names))(*[a.ctypes.data for a in arrs])
     _check(lib().sedi_h5_write(str(path).encode(), len(names), cn, ca, ctypes.byref(g)))
+
+
+def graphs_enable(on=True):
+    """Capture chunks of solver iterations into HIP graphs (default) or not."""
+    _check(lib().sedi_graphs_enable(1 if on else 0))
 
 
 def profile_enable(kernel_name):

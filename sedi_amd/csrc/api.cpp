@@ -9,6 +9,7 @@
 #include "batched.h"
 #include "estimator.h"
 #include "h5io.h"
+#include "serve.h"
 #include "mt_jump.h"
 #include "profiler.h"
 #include "plain.h"
@@ -22,6 +23,10 @@ struct sedi_solver {
 
 struct sedi_estimator {
     sedi::Estimator impl;
+};
+
+struct sedi_server {
+    sedi::Server impl;
 };
 
 struct sedi_batch {
@@ -281,6 +286,94 @@ int sedi_estimator_set_sample_hook(sedi_estimator* est, sedi_sample_fn fn, int w
         return SEDI_OK;
     });
 }
+
+int sedi_server_create(const sedi_server_config* cfg, sedi_server** out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(cfg && out, SEDI_ERR_ARG, "cfg and out must not be NULL");
+        *out = nullptr;
+        int rc = sedi::ensure_device();
+        if (rc) return rc;
+        sedi::ServeConfig c;
+        c.hbm_budget = cfg->hbm_budget_bytes;
+        c.precision = cfg->precision;
+        c.rep_every = cfg->rep_every > 0 ? cfg->rep_every : 50;
+        c.chunk = cfg->chunk > 0 ? cfg->chunk : 8;
+        c.max_slots = cfg->max_slots > 0 ? cfg->max_slots : 64;
+        c.hist_cap = cfg->max_iter > 0 ? cfg->max_iter : 10000;
+        sedi_server* h = new sedi_server();
+        if ((rc = h->impl.start(c))) {
+            delete h;
+            return rc;
+        }
+        *out = h;
+        return SEDI_OK;
+    });
+}
+
+int64_t sedi_server_submit(sedi_server* srv, const sedi_job* job) {
+    int64_t id = -1;
+    const int rc = guarded([&]() -> int {
+        SEDI_REQUIRE(srv && job, SEDI_ERR_ARG, "server and job must not be NULL");
+        sedi::Shape s;
+        int r = check_grid(&job->grid, &s);
+        if (r) return r;
+        sedi::ServeJob j;
+        j.s = s;
+        j.a = job->stencil.a;
+        j.b = job->stencil.b;
+        j.c = job->stencil.c;
+        j.kind = job->kind;
+        j.samples = job->samples;
+        j.tol = job->tol;
+        j.maxiter = job->maxiter;
+        j.seed = job->seed;
+        const long long v = srv->impl.submit(j);
+        if (v < 0) return (int)v;
+        id = v;
+        return SEDI_OK;
+    });
+    return rc < 0 ? rc : id;
+}
+
+int sedi_server_wait(sedi_server* srv, int64_t job, double timeout_s, double* D, int* iters, double* errs,
+                     int* codes, double* times4, int* status) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(srv, SEDI_ERR_ARG, "server is NULL");
+        sedi::ServeResult r;
+        const int rc = srv->impl.wait(job, &r, timeout_s);
+        if (rc) return rc;
+        if (status) *status = r.status;
+        if (r.status == 0) {
+            if (D) std::copy(r.D.begin(), r.D.end(), D);
+            if (iters) std::copy(r.iters.begin(), r.iters.end(), iters);
+            if (errs) std::copy(r.errs.begin(), r.errs.end(), errs);
+            if (codes) std::copy(r.codes.begin(), r.codes.end(), codes);
+        }
+        if (times4) {
+            times4[0] = r.t_submit;
+            times4[1] = r.t_admit;
+            times4[2] = r.t_start;
+            times4[3] = r.t_done;
+        }
+        return SEDI_OK;
+    });
+}
+
+int sedi_server_stats(sedi_server* srv, sedi_server_info* out) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(srv && out, SEDI_ERR_ARG, "server and out must not be NULL");
+        const sedi::ServeStats s = srv->impl.stats();
+        out->submitted = s.submitted;
+        out->done = s.done;
+        out->probes = s.probes;
+        out->bytes_in_use = s.bytes_in_use;
+        out->peak_bytes = s.peak_bytes;
+        out->pools = s.pools;
+        return SEDI_OK;
+    });
+}
+
+void sedi_server_destroy(sedi_server* srv) { delete srv; }
 
 int sedi_h5_available(void) {
     return guarded([&]() -> int { return sedi::h5_available() ? 1 : 0; });

@@ -17,6 +17,8 @@ struct BatchResult {
     std::vector<double> errs;
 };
 
+struct BRun;
+
 class BatchSolver {
   public:
     ~BatchSolver();
@@ -50,7 +52,34 @@ class BatchSolver {
     // Device bytes of the vectors of a batch of K right-hand sides on grid s.
     static double bytes_needed(const Shape& s, int kind, int K, bool fp32);
 
+    // ---- block-level control (continuous batching) -------------------------
+    // The K/4 RHS blocks are independent slots: each keeps its own iteration
+    // counter, so blocks can be started while others are mid-solve.
+    // Opens slot mode: every block free; histories of up to hist_cap
+    // iterations.  Synchronous.
+    int slots_open(int hist_cap);
+    // b of block rb (4 right-hand sides in the storage type), for the probe
+    // generator.
+    void* block_rhs(int rb) const;
+    // Starts block rb from x = 0 with its b in place (asynchronous).
+    int start_block(int rb, double tol, int maxiter);
+    // Launches `iters` iterations of every running block (asynchronous).
+    int step(int iters);
+    // Queues a copy of the status (block done flags and per-RHS results) into
+    // host memory `snap` of status_bytes() bytes (pinned) and records `ev`
+    // after it; the snap_* accessors read it once ev has completed.
+    size_t status_bytes() const;
+    int post_status(void* snap, hipEvent_t ev);
+    const int* snap_block_done(const void* snap) const;
+    const int* snap_iters(const void* snap) const;
+    const int* snap_codes(const void* snap) const;
+    const double* snap_errs(const void* snap) const;
+    int nblocks() const { return nrb_; }
+
   private:
+    template <class T> int bind_t(int hist_need);
+    template <class T> int iterate_t(int count);
+    template <class T> int start_t(int rb0, int count, double tol, int maxiter);
     template <class T> int solve_t(int maxiter, double tol, BatchResult* res);
     template <class T> int accumulate_t(double* t, double* q, int k0, int k1);
     template <class T> int pack_t(const double* b, int kact, long long row_stride);
@@ -67,6 +96,11 @@ class BatchSolver {
     int* pinned_ = nullptr;
     int hist_cap_ = 0;
     std::vector<double> hist_host_;
+    BRun* run_ = nullptr;
+    // the captured chunk of iterations (count, executable graph)
+    int graph_count_ = 0;
+    hipGraphExec_t graph_ = nullptr;
+    bool geometry_ready_ = false;
 };
 
 }  // namespace sedi

@@ -18,7 +18,8 @@
 // replacement r = b - A (x_hi + x_lo) (computed in FP64, stored as FP32;
 // then x_hi += x_lo, x_lo = 0) every rep_every iterations, when its recursive
 // residual has dropped tenfold since the last replacement, when it reaches the
-// tolerance, and at maxiter.  The tenfold rule is the reliable-update
+// tolerance, and at maxiter (CG; the pipelined CG and BiCGStab policies are
+// in bind_t).  The tenfold rule is the reliable-update
 // criterion (Sleijpen & van der Vorst, "Reliable updated residuals in hybrid
 // Bi-CG methods", Computing 56, 1996; delta = 0.1 as in Clark et al., "Solving
 // lattice QCD systems of equations using mixed precision solvers on GPUs",
@@ -46,12 +47,12 @@ namespace sedi {
 // Cg.hpp:111-147, PipelinedCg.hpp:113-136 and BiCgStab.hpp:117-163 for x = 0:
 // A x = 0, so r = b exactly; BiCGStab also takes r^ = r (:154).
 template <class T>
-__global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl c, const V4<T>* b,
-                                                              V4<T>* r, V4<T>* rhat, V4<T>* xlo,
-                                                              double* xhi, T im) {
+__global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl c, int rb0, double tol,
+                                                              int maxit, const V4<T>* b, V4<T>* r,
+                                                              V4<T>* rhat, V4<T>* xlo, double* xhi, T im) {
     __shared__ double red[4 * 8];
     __shared__ int lastflag;
-    const int rb = blockIdx.y;
+    const int rb = rb0 + blockIdx.y;
     const V4<T>* bq = b + (long long)rb * n;
     V4<T>* rq = r + (long long)rb * n;
     V4<T>* xq = xlo + (long long)rb * n;
@@ -83,6 +84,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         double bb = acc[q];
         if (bb == 0.0) bb = 1.0;  // Cg.hpp:115
         const double e = sqrt(acc[q] / bb);
+        c.tolv[k] = tol;
+        c.maxv[k] = maxit;
         c.bb[k] = bb;
         c.err[k] = e;
         c.elast[k] = e;
@@ -101,12 +104,12 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
             c.ss[k] = 0.0;
             c.cbw[k] = 0.0;
         }
-        if (e < c.tol) {  // Cg.hpp:130-134 (strict <)
+        if (e < c.tolv[k]) {  // Cg.hpp:130-134 (strict <)
             bset_done(c, k, 0, 0, e);
         } else if (c.kind == SEDI_BICGSTAB) {
-            if (c.maxiter < 1) bbicg_outcome(c, k, c.maxiter + 1);
+            if (c.maxv[k] < 1) bbicg_outcome(c, k, c.maxv[k] + 1);
             else if (c.rho[k] == 0.0) bbicg_outcome(c, k, 1);  // :163
-        } else if (c.maxiter < 1) {
+        } else if (c.maxv[k] < 1) {
             bcg_exhausted(c, k, e);
         }
     }
@@ -233,13 +236,16 @@ __global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c,
         c.hist[(long long)k * c.hstride + it - 1] = e;
         if (c.mixed) {
             // converged on the recursive residual, a replacement due, or the
-            // last iteration: the FP64 pass decides
-            c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter) ||
+            // last iteration: the FP64 pass decides.  elast is the largest
+            // residual since the last replacement (reliable updates compare
+            // against it, Clark et al. 2010).
+            c.elast[k] = fmax(c.elast[k], e);
+            c.rep[k] = (e <= c.tolv[k]) || (it % c.rep_every == 0) || (it >= c.maxv[k]) ||
                        (e < c.drop * c.elast[k]);
-        } else if (e <= c.tol) {
+        } else if (e <= c.tolv[k]) {
             bset_done(c, k, 0, it, e);
-        } else if (it >= c.maxiter) {
-            bset_done(c, k, 1, c.maxiter, e);
+        } else if (it >= c.maxv[k]) {
+            bset_done(c, k, 1, c.maxv[k], e);
         }
     }
     c.iter[rb] = it + 1;
@@ -261,9 +267,10 @@ struct PassBRep {
     const V4<T>* xlo;
     V4<T>* r;
     const V4<T>* rhs;
-    const V4<T>* rhat;
+    V4<T>* rhat;
     T im;
     int rep[4];
+    bool reset;  // BiCGStab: r^ restarts from the replaced residual
     struct Raw { V xh; V4<T> xl; };
     struct Aux { V4<T> b, r, rh; };
     __device__ Raw load(cidx_t i) const {
@@ -277,7 +284,7 @@ struct PassBRep {
         Aux o;
         o.b = ldc(rhs, i);
         o.r = ldc((const V4<T>*)r, i);
-        if (RHAT) o.rh = ldc(rhat, i);
+        if (RHAT) o.rh = ldc((const V4<T>*)rhat, i);
         return o;
     }
     __device__ V f(const Raw& q) const {
@@ -293,7 +300,13 @@ struct PassBRep {
         for (int k = 0; k < 4; k++)
             if (rep[k]) rn[k] = (T)((double)o.b[k] - Ax[k]);  // Cg.hpp:123, in FP64
         stv(r, i, rn);
-        const V4<T> u = RHAT ? o.rh : im * rn;
+        V4<T> u = RHAT ? o.rh : im * rn;
+        if (RHAT && reset) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (rep[k]) u[k] = rn[k];
+            stv(rhat, i, u);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             acc[k] += (double)rn[k] * (double)u[k];
@@ -306,7 +319,7 @@ template <class T, bool RHAT, int ZC>
 __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c, double a, double b,
                                                              double cc, long long n, const double* xhi,
                                                              const V4<T>* xlo, V4<T>* r0, V4<T>* r1,
-                                                             const V4<T>* rhs, const V4<T>* rhat,
+                                                             const V4<T>* rhs, V4<T>* rhat,
                                                              T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
@@ -331,6 +344,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     P.rhs = rhs + off;
     P.rhat = RHAT ? rhat + off : nullptr;
     P.im = im;
+    P.reset = c.bicg_reset != 0;
     double acc[8];
     run_tiles<PassBRep<T, RHAT>, ZC>(P, g, lds, acc, false);
     if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -351,16 +365,17 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
             c.ru[k] = acc[q];
             // beta from the replaced residual, or a restart if the recursion drifted
             c.beta[k] = drifted ? 0.0 : acc[q] / c.ru_old[k];
-            if (e <= c.tol) bset_done(c, k, 0, it, e);
-            else if (it >= c.maxiter) bset_done(c, k, 1, c.maxiter, e);
+            if (e <= c.tolv[k]) bset_done(c, k, 0, it, e);
+            else if (it >= c.maxv[k]) bset_done(c, k, 1, c.maxv[k], e);
         } else {
             // BiCgStab.hpp:229-232 and :160-169 of the next iteration on the
             // replaced residual
-            c.rho[k] = acc[q];
-            if (e <= c.tol) { bbicg_outcome(c, k, it); continue; }
-            if (it >= c.maxiter) { bbicg_outcome(c, k, c.maxiter + 1); continue; }
+            c.rho[k] = acc[q];  // <r, r^> (with a restarted r^ = r: <r, r>)
+            if (e <= c.tolv[k]) { bbicg_outcome(c, k, it); continue; }
+            if (it >= c.maxv[k]) { bbicg_outcome(c, k, c.maxv[k] + 1); continue; }
             if (c.rho[k] == 0.0) { bbicg_outcome(c, k, it + 1); continue; }
-            const double beta = drifted ? 0.0 : (c.rho[k] / c.rho_prev[k]) * (c.alpha[k] / c.omega[k]);
+            const double beta = (drifted || c.bicg_reset) ? 0.0
+                                                          : (c.rho[k] / c.rho_prev[k]) * (c.alpha[k] / c.omega[k]);
             c.beta[k] = beta;
             c.cbw[k] = -beta * c.omega[k];
         }
@@ -462,7 +477,7 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // host side
 // ---------------------------------------------------------------------------
 template <class T, bool RHAT>
-static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, const V4<T>* rhat) {
+static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
     const KGeom g = bgeom(batch_rep_kernel<T, RHAT, 16>, r.shape, SEDI_PREFETCH_BATCH);
     const dim3 grid(g.nblocks, r.nrb);
     const T im = (T)r.im;
@@ -489,7 +504,7 @@ int fold_only(const BRun& r) {
 // CG: res is r and rhat is null.  BiCGStab: res is r0 and rhat r^; the kernel
 // picks r0 / r1 by the parity of the next iteration.
 template <class T>
-int replace_and_fold(const BRun& r, V4<T>* res, const V4<T>* rhat) {
+int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat) {
     hipError_t e;
     if (rhat) e = launch_rep<T, true>(r, res, static_cast<V4<T>*>(r.v[BI_R1]), rhat);
     else e = launch_rep<T, false>(r, res, nullptr, nullptr);
@@ -520,10 +535,10 @@ static int cg_iteration(const BRun& r) {
 // control block layout (doubles, then ints)
 enum {
     CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
-    CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW,
+    CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW, CD_TOL,
     CD_NDOUBLE
 };
-enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_NINT };
+enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_MAXIT, CI_NINT };
 
 static int nvec_of(int kind) {
     return kind == SEDI_CG ? CG_NVEC : (kind == SEDI_PCG ? PCG_NVEC : BI_NVEC);
@@ -536,8 +551,10 @@ double BatchSolver::bytes_needed(const Shape& s, int kind, int K, bool fp32) {
 
 BatchSolver::~BatchSolver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
     if (pinned_) (void)hipHostFree(pinned_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    delete run_;
 }
 
 int BatchSolver::create(const Shape& s, double a, double b, double c, int kind, int K, bool fp32,
@@ -556,6 +573,7 @@ int BatchSolver::create(const Shape& s, double a, double b, double c, int kind, 
     fp32_ = fp32;
     rep_every_ = rep_every > 0 ? rep_every : 50;
     nvec_ = nvec_of(kind);
+    run_ = new BRun();
     SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SEDI_HIP(hipHostMalloc((void**)&pinned_, sizeof(int) * 2 * nrb_, hipHostMallocDefault));
     const size_t n = (size_t)s.n();
@@ -576,16 +594,22 @@ int BatchSolver::create(const Shape& s, double a, double b, double c, int kind, 
     return SEDI_OK;
 }
 
+// Fills run_ (the control block's pointers and the pass parameters) for a
+// history capacity of at least hist_need iterations per RHS.
 template <class T>
-int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
-    SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
-    const long long n = shape_.n();
+int BatchSolver::bind_t(int hist_need) {
     const int K = K_;
-    if (hist_cap_ < std::max(maxiter, 1)) {
-        SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)K * std::max(maxiter, 1)));
-        hist_cap_ = std::max(maxiter, 1);
+    if (hist_cap_ < std::max(hist_need, 1)) {
+        SEDI_HIP(hipStreamSynchronize(stream_));
+        SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)K * std::max(hist_need, 1)));
+        hist_cap_ = std::max(hist_need, 1);
+        if (graph_) {  // the graph holds the old history pointer
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+            graph_count_ = 0;
+        }
     }
-    BRun run;
+    BRun& run = *run_;
     BCtl& c = run.c;
     double* d = ctl_.as<double>();
     c.bb = d + CD_BB * K; c.ru = d + CD_RU * K; c.ru_old = d + CD_RUOLD * K;
@@ -595,9 +619,10 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     c.gamma_prev = d + CD_GAMMAP * K; c.alpha_prev = d + CD_ALPHAP * K;
     c.rho = d + CD_RHO * K; c.rho_prev = d + CD_RHOP * K; c.omega = d + CD_OMEGA * K; c.ss = d + CD_SS * K;
     c.cbw = d + CD_CBW * K;
+    c.tolv = d + CD_TOL * K;
     int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
     c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
-    c.fold = ip + CI_FOLD * K; c.sexit = ip + CI_SEXIT * K;
+    c.fold = ip + CI_FOLD * K; c.sexit = ip + CI_SEXIT * K; c.maxv = ip + CI_MAXIT * K;
     c.iter = ip + CI_NINT * K;
     c.rb_done = c.iter + nrb_;
     c.counters = reinterpret_cast<unsigned*>(c.rb_done + nrb_);
@@ -607,50 +632,116 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     c.pstride = 12LL * 8 * batch_cus();
     c.hist = hist_.as<double>();
     c.hstride = hist_cap_;
-    c.tol = tol;
-    c.maxiter = maxiter;
-    c.rep_every = rep_every_;
-    c.drop = 0.1;
+    // Replacement policy per solver (mixed form), from measurements at 128^3
+    // to 512^3 against the FP64 solvers on the same probes:
+    //  * CG: every rep_every iterations and at each tenfold decrease;
+    //  * pipelined CG: its matrix-vector recurrences (w = A u, s = A p,
+    //    z = A q) drift much faster, so at least every 10 iterations (at 50
+    //    it stalls near 1e-6 at 512^3; at 10 it matches FP64's counts);
+    //  * BiCGStab: replacements disturb its bi-orthogonality, so only when
+    //    the residual has dropped by sqrt(eps32) ~ 2.4e-4 since the last one
+    //    (the criterion of van der Vorst & Ye, "Residual replacement
+    //    strategies for Krylov subspace iterative methods for the
+    //    convergence of true residuals", SIAM J. Sci. Comput. 22, 2000),
+    //    with r^ and the directions restarted from the replaced residual;
+    //    periodic replacement or a tenfold rule made it diverge at 512^3.
+    c.rep_every = kind_ == SEDI_PCG ? std::min(rep_every_, 10)
+                  : kind_ == SEDI_BICGSTAB ? 1 << 30 : rep_every_;
+    c.drop = kind_ == SEDI_BICGSTAB ? SEDI_BICG_DROP : 0.1;
     c.restart = 2.0;
+    c.bicg_reset = SEDI_BICG_RESET;
     c.mixed = fp32_ ? 1 : 0;
     c.K = K;
     c.kind = kind_;
     run.shape = shape_;
-    run.n = n;
+    run.n = shape_.n();
     run.nrb = nrb_;
     run.a = a_; run.b = b_c_; run.cc = c_; run.im = 1.0 / a_;
     run.stream = stream_;
     for (int k = 0; k < 12; k++) run.v[k] = v_[k];
     run.xhi = xhi_;
     run.mixed = fp32_;
+    return SEDI_OK;
+}
 
+// Blocks rb0 .. rb0+count-1: r = b, x = 0, their scalars, tolerance and cap
+// (and the pipelined set-up pass).
+template <class T>
+int BatchSolver::start_t(int rb0, int count, double tol, int maxiter) {
+    const BRun& run = *run_;
+    const long long n = shape_.n();
     const T im = (T)(1.0 / a_);
     V4<T>* B = static_cast<V4<T>*>(v_[SLOT_B]);
     V4<T>* XL = static_cast<V4<T>*>(v_[SLOT_XLO]);
     // the first residual: r (CG, pipelined CG) or r1, read by iteration 1 (BiCGStab)
     V4<T>* R = static_cast<V4<T>*>(kind_ == SEDI_BICGSTAB ? v_[BI_R1] : v_[CG_R]);
     V4<T>* RH = kind_ == SEDI_BICGSTAB ? static_cast<V4<T>*>(v_[BI_RH]) : nullptr;
-    SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ew_blocks(n), nrb_), dim3(NTHREADS),
-                    stream_, n, c, (const V4<T>*)B, R, RH, XL, xhi_, im));
-    if (kind_ == SEDI_PCG) {
-        const int rc = pcg_setup<T>(run);
-        if (rc) return rc;
+    SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ew_blocks(n), count), dim3(NTHREADS),
+                    stream_, n, run.c, rb0, tol, maxiter, (const V4<T>*)B, R, RH, XL, xhi_, im));
+    if (kind_ == SEDI_PCG) return pcg_setup<T>(run, rb0, count);
+    return SEDI_OK;
+}
+
+// `count` iterations of every running block.  The first call on a solver
+// launches them directly (which also settles every pass's geometry); later
+// calls replay one captured graph of the chunk, whose kernels read the
+// iteration state from device memory.
+template <class T>
+int BatchSolver::iterate_t(int count) {
+    const BRun& run = *run_;
+    auto direct = [&]() -> int {
+        int rc = SEDI_OK;
+        for (int k = 0; k < count && rc == SEDI_OK; k++) {
+            if (kind_ == SEDI_CG) rc = cg_iteration<T>(run);
+            else if (kind_ == SEDI_PCG) rc = pcg_iteration<T>(run);
+            else rc = bicg_iteration<T>(run);
+        }
+        return rc;
+    };
+    if (!graphs_enabled() || profile_any() || !geometry_ready_) {
+        const int rc = direct();
+        geometry_ready_ = true;
+        return rc;
     }
+    if (!graph_ || graph_count_ != count) {
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+        SEDI_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        const int rc = direct();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SEDI_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SEDI_HIP(ei);
+        graph_count_ = count;
+    }
+    SEDI_HIP(hipGraphLaunch(graph_, stream_));
+    return SEDI_OK;
+}
+
+template <class T>
+int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
+    SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
+    const int K = K_;
+    int rc = bind_t<T>(maxiter);
+    if (rc || (rc = start_t<T>(0, nrb_, tol, maxiter))) return rc;
+    const BCtl& c = run_->c;
     const int chunk = 16;
     int launched = 0, slot = 0;
     bool pending = false;
     hipEvent_t ev[2];
     SEDI_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     SEDI_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int rc = SEDI_OK;
     while (launched < maxiter && rc == SEDI_OK) {
         const int cnt = std::min(chunk, maxiter - launched);
-        for (int k = 0; k < cnt && rc == SEDI_OK; k++) {
-            if (kind_ == SEDI_CG) rc = cg_iteration<T>(run);
-            else if (kind_ == SEDI_PCG) rc = pcg_iteration<T>(run);
-            else rc = bicg_iteration<T>(run);
-        }
-        if (rc) break;
+        if ((rc = iterate_t<T>(cnt))) break;
         launched += cnt;
         if (hipMemcpyAsync(pinned_ + slot * nrb_, c.rb_done, sizeof(int) * nrb_, hipMemcpyDeviceToHost,
                            stream_) != hipSuccess ||
@@ -689,6 +780,63 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     SEDI_HIP(hipMemcpy(hist_host_.data(), hist_.p, sizeof(double) * hist_host_.size(), hipMemcpyDeviceToHost));
     return SEDI_OK;
 }
+
+// ---- block-level control ---------------------------------------------------
+__global__ void batch_free_all_kernel(BCtl c, int nrb) {
+    for (int i = threadIdx.x; i < nrb; i += blockDim.x) c.rb_done[i] = 1;
+    for (int i = threadIdx.x; i < 4 * nrb; i += blockDim.x) c.done[i] = 1;
+}
+
+int BatchSolver::slots_open(int hist_cap) {
+    int rc = fp32_ ? bind_t<float>(hist_cap) : bind_t<double>(hist_cap);
+    if (rc) return rc;
+    SEDI_HIP(launch("batch_free_all_kernel", batch_free_all_kernel, dim3(1), dim3(256), stream_, run_->c, nrb_));
+    SEDI_HIP(hipStreamSynchronize(stream_));
+    return SEDI_OK;
+}
+
+void* BatchSolver::block_rhs(int rb) const {
+    const size_t bytes = (size_t)shape_.n() * 4 * (fp32_ ? 4 : 8);
+    return static_cast<char*>(v_[SLOT_B]) + bytes * rb;
+}
+
+int BatchSolver::start_block(int rb, double tol, int maxiter) {
+    SEDI_REQUIRE(rb >= 0 && rb < nrb_, SEDI_ERR_ARG, "block %d of %d", rb, nrb_);
+    SEDI_REQUIRE(maxiter >= 0 && maxiter <= hist_cap_, SEDI_ERR_ARG, "maxiter %d beyond the history capacity %d",
+                 maxiter, hist_cap_);
+    return fp32_ ? start_t<float>(rb, 1, tol, maxiter) : start_t<double>(rb, 1, tol, maxiter);
+}
+
+int BatchSolver::step(int iters) {
+    if (iters <= 0) return SEDI_OK;
+    return fp32_ ? iterate_t<float>(iters) : iterate_t<double>(iters);
+}
+
+// Status snapshot: [ferr K doubles][the control block's ints: per-RHS
+// (CI_NINT x K), then iter and rb_done per block].
+size_t BatchSolver::status_bytes() const {
+    return sizeof(double) * K_ + sizeof(int) * ((size_t)CI_NINT * K_ + 2 * nrb_);
+}
+
+int BatchSolver::post_status(void* snap, hipEvent_t ev) {
+    const BCtl& c = run_->c;
+    char* h = static_cast<char*>(snap);
+    SEDI_HIP(hipMemcpyAsync(h, c.ferr, sizeof(double) * K_, hipMemcpyDeviceToHost, stream_));
+    SEDI_HIP(hipMemcpyAsync(h + sizeof(double) * K_, c.fiter, sizeof(int) * ((size_t)CI_NINT * K_ + 2 * nrb_),
+                            hipMemcpyDeviceToHost, stream_));
+    SEDI_HIP(hipEventRecord(ev, stream_));
+    return SEDI_OK;
+}
+
+static const int* snap_ints(const void* snap, int K) {
+    return reinterpret_cast<const int*>(static_cast<const char*>(snap) + sizeof(double) * K);
+}
+const int* BatchSolver::snap_block_done(const void* snap) const {
+    return snap_ints(snap, K_) + CI_NINT * K_ + nrb_;
+}
+const int* BatchSolver::snap_iters(const void* snap) const { return snap_ints(snap, K_) + CI_FITER * K_; }
+const int* BatchSolver::snap_codes(const void* snap) const { return snap_ints(snap, K_) + CI_CODE * K_; }
+const double* BatchSolver::snap_errs(const void* snap) const { return static_cast<const double*>(snap); }
 
 int BatchSolver::solve(int maxiter, double tol, BatchResult* res) {
     return fp32_ ? solve_t<float>(maxiter, tol, res) : solve_t<double>(maxiter, tol, res);
@@ -755,8 +903,8 @@ std::vector<double> BatchSolver::history(int k) const {
     return h;
 }
 
-template int replace_and_fold<float>(const BRun&, V4<float>*, const V4<float>*);
-template int replace_and_fold<double>(const BRun&, V4<double>*, const V4<double>*);
+template int replace_and_fold<float>(const BRun&, V4<float>*, V4<float>*);
+template int replace_and_fold<double>(const BRun&, V4<double>*, V4<double>*);
 template int fold_only<float>(const BRun&);
 template int fold_only<double>(const BRun&);
 

@@ -18,49 +18,55 @@ namespace sedi {
 // (p = r at the first iteration), p^ = p.*iM, v = A p^, <v, r^>
 template <class T>
 struct PassBBicgDir {
-    using V = V4<T>;
+    using C = compute_t<T>;
+    using V = V4<C>;
+    using S = V4<T>;
     static constexpr int ND = 4, D = SEDI_PREFETCH_BATCH;
-    T a, b, c;
-    const V *r, *pold, *vold, *rhat;
-    V *pnew, *vnew;
-    T im;
+    C a, b, c;
+    const S *r, *pold, *vold, *rhat;
+    S *pnew, *vnew;
+    C im;
     V beta, cbw;
     bool first;
-    struct Raw { V r, p, v; };
-    struct Aux { V rhat; };
+    struct Raw { S r, p, v; };
+    struct Aux { S rhat; };
     __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i), ldc(vold, i)}; }
     __device__ Aux load_aux(cidx_t i) const { return {ldc(rhat, i)}; }
-    __device__ V pdir(const Raw& q) const { return first ? q.r : q.r + beta * q.p + cbw * q.v; }  // :169
-    __device__ V f(const Raw& q) const { return im * pdir(q); }
+    __device__ S pdir(const Raw& q) const {  // :169
+        return first ? q.r : cvt<T>(cvt<C>(q.r) + beta * cvt<C>(q.p) + cbw * cvt<C>(q.v));
+    }
+    __device__ V f(const Raw& q) const { return im * cvt<C>(pdir(q)); }
     __device__ void finish(cidx_t i, const Raw& q, const Aux& o, const V&, const V& v,
                            double (&acc)[ND]) const {
+        const S vs = cvt<T>(v);
         stv(pnew, i, pdir(q));
-        stv(vnew, i, v);
+        stv(vnew, i, vs);
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += (double)v[k] * (double)o.rhat[k];
+        for (int k = 0; k < 4; k++) acc[k] += (double)vs[k] * (double)o.rhat[k];
     }
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_bicg_dir_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                                  long long n, const V4<T>* r0,
-                                                                  const V4<T>* r1, V4<T>* p0, V4<T>* p1,
-                                                                  V4<T>* v0, V4<T>* v1, const V4<T>* rhat,
-                                                                  T im) {
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_dir_kernel(TileGrid g, BCtl c, compute_t<T> a,
+                                                                  compute_t<T> b, compute_t<T> cc, long long n,
+                                                                  const V4<T>* r0, const V4<T>* r1, V4<T>* p0,
+                                                                  V4<T>* p1, V4<T>* v0, V4<T>* v1,
+                                                                  const V4<T>* rhat, compute_t<T> im) {
+    using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ V4<C> lds[2][TY + 2][TX + 2];
     __shared__ double red[4 * 4];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const int cur = it & 1;
     const long long off = (long long)rb * n;
-    V4<T> beta, cbw;
+    V4<C> beta, cbw;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const bool d = c.done[4 * rb + k];
-        beta[k] = d ? (T)0 : (T)c.beta[4 * rb + k];
-        cbw[k] = d ? (T)0 : (T)c.cbw[4 * rb + k];
+        beta[k] = d ? (C)0 : (C)c.beta[4 * rb + k];
+        cbw[k] = d ? (C)0 : (C)c.cbw[4 * rb + k];
     }
     PassBBicgDir<T> P{a, b, cc, (cur ? r1 : r0) + off, (cur ? p0 : p1) + off, (cur ? v0 : v1) + off,
                       rhat + off, (cur ? p1 : p0) + off, (cur ? v1 : v0) + off, im, beta, cbw, it == 1};
@@ -79,45 +85,51 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_dir_kernel(TileGrid g, BC
 // t = A s^, <s,s>, <t,s>, <t,t>; s and t are not stored
 template <class T>
 struct PassBBicgHalf {
-    using V = V4<T>;
+    using C = compute_t<T>;
+    using V = V4<C>;
+    using S = V4<T>;
     static constexpr int ND = 12, D = SEDI_PREFETCH_BATCH;
-    T a, b, c;
-    const V *r, *v;
-    T im;
+    C a, b, c;
+    const S *r, *v;
+    C im;
     V alpha;
-    struct Raw { V r, v; };
+    struct Raw { S r, v; };
     struct Aux {};
     __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i)}; }
     __device__ Aux load_aux(cidx_t) const { return {}; }
-    __device__ V sval(const Raw& q) const { return q.r - alpha * q.v; }
-    __device__ V f(const Raw& q) const { return im * sval(q); }
+    // s rounded to the storage type, as the update pass will see it
+    __device__ S sval(const Raw& q) const { return cvt<T>(cvt<C>(q.r) - alpha * cvt<C>(q.v)); }
+    __device__ V f(const Raw& q) const { return im * cvt<C>(sval(q)); }
     __device__ void finish(cidx_t, const Raw& q, const Aux&, const V&, const V& t, double (&acc)[ND]) const {
-        const V sv = sval(q);
+        const S sv = sval(q);
+        const S ts = cvt<T>(t);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             acc[k] += (double)sv[k] * (double)sv[k];
-            acc[4 + k] += (double)t[k] * (double)sv[k];
-            acc[8 + k] += (double)t[k] * (double)t[k];
+            acc[4 + k] += (double)ts[k] * (double)sv[k];
+            acc[8 + k] += (double)ts[k] * (double)ts[k];
         }
     }
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                                   long long n, const V4<T>* r0,
-                                                                   const V4<T>* r1, const V4<T>* v0,
-                                                                   const V4<T>* v1, T im) {
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, BCtl c, compute_t<T> a,
+                                                                   compute_t<T> b, compute_t<T> cc, long long n,
+                                                                   const V4<T>* r0, const V4<T>* r1,
+                                                                   const V4<T>* v0, const V4<T>* v1,
+                                                                   compute_t<T> im) {
+    using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ V4<C> lds[2][TY + 2][TX + 2];
     __shared__ double red[12 * 4];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const int cur = it & 1;
     const long long off = (long long)rb * n;
-    V4<T> alpha;
+    V4<C> alpha;
 #pragma unroll
-    for (int k = 0; k < 4; k++) alpha[k] = (T)c.alpha[4 * rb + k];
+    for (int k = 0; k < 4; k++) alpha[k] = (C)c.alpha[4 * rb + k];
     PassBBicgHalf<T> P{a, b, cc, (cur ? r1 : r0) + off, (cur ? v1 : v0) + off, im, alpha};
     double acc[12];
     run_tiles<PassBBicgHalf<T>, ZC>(P, g, lds, acc, cur == 0);
@@ -127,9 +139,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, B
     for (int q = 0; q < 4; q++) {
         const int k = 4 * rb + q;
         if (c.done[k]) continue;
-        c.ss[k] = acc[q];                                // :193-194
-        if (acc[q] < c.tol * c.tol) c.sexit[k] = 1;      // :195 (absolute test)
-        else c.omega[k] = acc[4 + q] / acc[8 + q];       // :213
+        c.ss[k] = acc[q];                                        // :193-194
+        if (acc[q] < c.tolv[k] * c.tolv[k]) c.sexit[k] = 1;      // :195 (absolute test)
+        else c.omega[k] = acc[4 + q] / acc[8 + q];               // :213
     }
 }
 
@@ -139,46 +151,50 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, B
 // only (:195-199).
 template <class T>
 struct PassBBicgUpd {
-    using V = V4<T>;
+    using C = compute_t<T>;
+    using V = V4<C>;
+    using S = V4<T>;
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
-    T a, b, c;
-    const V *r, *v, *p, *rhat;
-    V *x, *rnew;
-    T im;
+    C a, b, c;
+    const S *r, *v, *p, *rhat;
+    S *x, *rnew;
+    C im;
     V alpha, omega;
     int sexit[4];
-    struct Raw { V r, v; };
-    struct Aux { V p, x, rhat; };
+    struct Raw { S r, v; };
+    struct Aux { S p, x, rhat; };
     __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i)}; }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(p, i), ldc((const V*)x, i), ldc(rhat, i)}; }
-    __device__ V sval(const Raw& q) const { return q.r - alpha * q.v; }
-    __device__ V f(const Raw& q) const { return im * sval(q); }
+    __device__ Aux load_aux(cidx_t i) const { return {ldc(p, i), ldc((const S*)x, i), ldc(rhat, i)}; }
+    __device__ S sval(const Raw& q) const { return cvt<T>(cvt<C>(q.r) - alpha * cvt<C>(q.v)); }
+    __device__ V f(const Raw& q) const { return im * cvt<C>(sval(q)); }
     __device__ void finish(cidx_t i, const Raw& q, const Aux& o, const V& sh, const V& t,
                            double (&acc)[ND]) const {
-        const V sv = sval(q);
-        const V ph = im * o.p;
-        const V xa = o.x + alpha * ph;
+        const V sv = cvt<C>(sval(q));
+        const V ph = im * cvt<C>(o.p);
+        const V xa = cvt<C>(o.x) + alpha * ph;
         const V xv = xa + omega * sh;  // combine3 (:217)
-        const V rv = sv - omega * t;
-        stv(x, i, sel(sexit, xa, xv));
-        stv(rnew, i, rv);
+        const S rs = cvt<T>(sv - omega * t);
+        stv(x, i, cvt<T>(sel(sexit, xa, xv)));
+        stv(rnew, i, rs);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            acc[k] += (double)rv[k] * (double)rv[k];
-            acc[4 + k] += (double)rv[k] * (double)o.rhat[k];
+            acc[k] += (double)rs[k] * (double)rs[k];
+            acc[4 + k] += (double)rs[k] * (double)o.rhat[k];
         }
     }
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                                  long long n, V4<T>* r0, V4<T>* r1,
-                                                                  const V4<T>* v0, const V4<T>* v1,
-                                                                  const V4<T>* p0, const V4<T>* p1,
-                                                                  const V4<T>* rhat, V4<T>* x, T im) {
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BCtl c, compute_t<T> a,
+                                                                  compute_t<T> b, compute_t<T> cc, long long n,
+                                                                  V4<T>* r0, V4<T>* r1, const V4<T>* v0,
+                                                                  const V4<T>* v1, const V4<T>* p0,
+                                                                  const V4<T>* p1, const V4<T>* rhat, V4<T>* x,
+                                                                  compute_t<T> im) {
+    using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ V4<C> lds[2][TY + 2][TX + 2];
     __shared__ double red[8 * 4];
     __shared__ int lastflag;
     const int it = c.iter[rb];
@@ -197,9 +213,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
     for (int k = 0; k < 4; k++) {
         const int kk = 4 * rb + k;
         const bool d = c.done[kk];
-        P.alpha[k] = d ? (T)0 : (T)c.alpha[kk];
+        P.alpha[k] = d ? (C)0 : (C)c.alpha[kk];
         P.sexit[k] = !d && c.sexit[kk];
-        P.omega[k] = (d || P.sexit[k]) ? (T)0 : (T)c.omega[kk];
+        P.omega[k] = (d || P.sexit[k]) ? (C)0 : (C)c.omega[kk];
     }
     double acc[8];
     run_tiles<PassBBicgUpd<T>, ZC>(P, g, lds, acc, cur == 1);
@@ -214,16 +230,20 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
         c.err[k] = err;
         c.hist[(long long)k * c.hstride + it - 1] = err;
         if (c.omega[k] == 0.0) { bbicg_outcome(c, k, it); continue; }  // :229-230
-        if (!c.mixed && err <= c.tol) { bbicg_outcome(c, k, it); continue; }
-        if (!c.mixed && it >= c.maxiter) { bbicg_outcome(c, k, c.maxiter + 1); continue; }
+        if (!c.mixed && err <= c.tolv[k]) { bbicg_outcome(c, k, it); continue; }
+        if (!c.mixed && it >= c.maxv[k]) { bbicg_outcome(c, k, c.maxv[k] + 1); continue; }
         c.rho_prev[k] = c.rho[k];  // :232
         c.rho[k] = acc[4 + q];     // next iteration's :160
         const double beta = (c.rho[k] / c.rho_prev[k]) * (c.alpha[k] / c.omega[k]);  // :167
         c.beta[k] = beta;
         c.cbw[k] = -beta * c.omega[k];  // the combine3 coefficient of :169
         if (c.mixed) {
-            // the FP64 pass decides convergence, maxiter and rho breakdown
-            c.rep[k] = (err <= c.tol) || (it % c.rep_every == 0) || (it >= c.maxiter) ||
+            // the FP64 pass decides convergence, maxiter and rho breakdown;
+            // the drop is measured from the largest residual since the last
+            // replacement (van der Vorst & Ye 2000: BiCGStab's residual
+            // spikes are what the recursion loses accuracy to)
+            c.elast[k] = fmax(c.elast[k], err);
+            c.rep[k] = (err <= c.tolv[k]) || (it % c.rep_every == 0) || (it >= c.maxv[k]) ||
                        (err < c.drop * c.elast[k]) || (c.rho[k] == 0.0);
         } else if (c.rho[k] == 0.0) {
             bbicg_outcome(c, k, it + 1);  // :163
@@ -236,7 +256,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
 // ---- host ------------------------------------------------------------------
 template <class T>
 int bicg_iteration(const BRun& r) {
-    const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
+    using C = compute_t<T>;
+    const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
     const KGeom g1 = bgeom(batch_bicg_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     const KGeom g2 = bgeom(batch_bicg_half_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
@@ -251,7 +272,7 @@ int bicg_iteration(const BRun& r) {
                              r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1), (const V4<T>*)V(BI_V0),
                              (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0), (const V4<T>*)V(BI_P1),
                              (const V4<T>*)V(BI_RH), V(SLOT_XLO), im));
-    if (r.mixed) return replace_and_fold<T>(r, V(BI_R0), (const V4<T>*)V(BI_RH));
+    if (r.mixed) return replace_and_fold<T>(r, V(BI_R0), V(BI_RH));
     return SEDI_OK;
 }
 

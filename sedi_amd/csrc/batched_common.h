@@ -16,6 +16,12 @@
 #ifndef SEDI_PREFETCH_BATCH
 #define SEDI_PREFETCH_BATCH 1
 #endif
+#ifndef SEDI_BICG_RESET
+#define SEDI_BICG_RESET 1
+#endif
+#ifndef SEDI_BICG_DROP
+#define SEDI_BICG_DROP 2.44140625e-4  // sqrt(2^-24)
+#endif
 
 namespace sedi {
 
@@ -49,6 +55,29 @@ __device__ __forceinline__ V4<double> to_f64(const V4<T>& a) {
 template <class T>
 __device__ __forceinline__ V4<T> from_f64(const V4<double>& a) {
     return {{(T)a[0], (T)a[1], (T)a[2], (T)a[3]}};
+}
+
+// Arithmetic type of the pipelined CG and BiCGStab passes for storage type T.
+// With FP32 storage they compute in FP64 registers: the FP32 stencil loses
+// about eps32 * cond(A) of A p on smooth fields (cancellation in 6 p0 - sum of
+// neighbours), which CG's recurrences absorb but pipelined CG's matrix-vector
+// recurrences and BiCGStab's bi-orthogonality do not.  Memory traffic is
+// unchanged (loads and stores stay FP32).
+#ifndef SEDI_MIXED_COMPUTE_F64
+#define SEDI_MIXED_COMPUTE_F64 1
+#endif
+template <class T>
+struct ComputeOf { using type = T; };
+#if SEDI_MIXED_COMPUTE_F64
+template <>
+struct ComputeOf<float> { using type = double; };
+#endif
+template <class T>
+using compute_t = typename ComputeOf<T>::type;
+
+template <class C, class T>
+__device__ __forceinline__ V4<C> cvt(const V4<T>& a) {
+    return {{(C)a[0], (C)a[1], (C)a[2], (C)a[3]}};
 }
 
 // Two FP64 fields marched together (the replacement passes of pipelined CG
@@ -92,12 +121,14 @@ struct BCtl {
     int *fiter, *code, *done, *rep, *fold, *sexit;
     // per RHS block
     int *iter, *rb_done;
+    double* tolv;  // per RHS: tolerance
+    int* maxv;     // per RHS: iteration cap
     double* partials;
     unsigned* counters;
-    double tol;
     double drop;     // replacement when the residual dropped by this factor
     double restart;  // restart when true > restart * recursive residual
-    int maxiter, rep_every, mixed, K, kind;
+    int rep_every, mixed, K, kind;
+    int bicg_reset;  // BiCGStab replacement restarts r^ = r and the directions
     long long hstride;  // history entries per RHS
     long long pstride;  // partial sums per RHS block
 };
@@ -115,18 +146,18 @@ __device__ __forceinline__ void brb_update(const BCtl& c, int rb) {
 
 // Cg.hpp:196-206 / PipelinedCg.hpp:232-242 when the loop ran out.
 __device__ __forceinline__ void bcg_exhausted(const BCtl& c, int k, double e) {
-    if (e <= c.tol) bset_done(c, k, 0, c.maxiter + 1, e);
-    else bset_done(c, k, 1, c.maxiter, e);
+    if (e <= c.tolv[k]) bset_done(c, k, 0, c.maxv[k] + 1, e);
+    else bset_done(c, k, 1, c.maxv[k], e);
 }
 
 // BiCgStab.hpp:235-263 for RHS k, `iter` = the loop variable when it ended.
 __device__ __forceinline__ void bbicg_outcome(const BCtl& c, int k, int iter) {
     const double e = c.err[k];
-    if (e <= c.tol) bset_done(c, k, 0, iter, e);
-    else if (c.ss[k] <= c.tol * c.tol) bset_done(c, k, 0, iter, sqrt(c.ss[k] / c.bb[k]));
+    if (e <= c.tolv[k]) bset_done(c, k, 0, iter, e);
+    else if (c.ss[k] <= c.tolv[k] * c.tolv[k]) bset_done(c, k, 0, iter, sqrt(c.ss[k] / c.bb[k]));
     else if (c.omega[k] == 0.0) bset_done(c, k, 3, iter, e);
     else if (c.rho[k] == 0.0) bset_done(c, k, 2, iter, e);
-    else bset_done(c, k, 1, c.maxiter, e);
+    else bset_done(c, k, 1, c.maxv[k], e);
 }
 
 // ---- host side -------------------------------------------------------------
@@ -211,12 +242,12 @@ struct BRun {
 // Per-solver kernel sets (batched_pcg.hip, batched_bicg.hip); each launches
 // the set-up passes after the common initialisation, or one iteration
 // including the mixed form's replacement and fold.
-template <class T> int pcg_setup(const BRun& r);
+template <class T> int pcg_setup(const BRun& r, int rb0, int count);
 template <class T> int pcg_iteration(const BRun& r);
 template <class T> int bicg_iteration(const BRun& r);
 // the FP64 replacement r = b - A x with <r,u> (CG) or <r,r^> (BiCGStab) and
 // the fold x_hi += x_lo that follows it (batched.hip)
-template <class T> int replace_and_fold(const BRun& r, V4<T>* res, const V4<T>* rhat);
+template <class T> int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat);
 template <class T> int fold_only(const BRun& r);
 
 }  // namespace sedi

@@ -23,40 +23,44 @@ namespace sedi {
 // (PipelinedCg.hpp:138-149, 155, 170-181 at iteration 1)
 template <class T>
 struct PassBPcgInit {
-    using V = V4<T>;
+    using C = compute_t<T>;
+    using V = V4<C>;
+    using S = V4<T>;
     static constexpr int ND = 12, D = SEDI_PREFETCH_BATCH;
-    T a, b, c;
-    const V* r;
-    V *u, *w, *z, *q, *s, *p;
-    T im;
-    struct Raw { V r; };
+    C a, b, c;
+    const S* r;
+    S *u, *w, *z, *q, *s, *p;
+    C im;
+    struct Raw { S r; };
     struct Aux {};
     __device__ Raw load(cidx_t i) const { return {ldc(r, i)}; }
     __device__ Aux load_aux(cidx_t) const { return {}; }
-    __device__ V f(const Raw& c) const { return im * c.r; }
-    __device__ void finish(cidx_t i, const Raw& c, const Aux&, const V& uv, const V& wv,
+    __device__ V f(const Raw& o) const { return im * cvt<C>(o.r); }
+    __device__ void finish(cidx_t i, const Raw& o, const Aux&, const V& uv, const V& wv,
                            double (&acc)[ND]) const {
-        const V zero{};
-        stv(u, i, uv);
-        stv(w, i, wv);
+        const S zero{};
+        const S us = cvt<T>(uv), ws = cvt<T>(wv);
+        stv(u, i, us);
+        stv(w, i, ws);
         stv(z, i, zero); stv(q, i, zero); stv(s, i, zero); stv(p, i, zero);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            acc[k] += (double)c.r[k] * (double)uv[k];
-            acc[4 + k] += (double)wv[k] * (double)uv[k];
-            acc[8 + k] += (double)c.r[k] * (double)c.r[k];
+            acc[k] += (double)o.r[k] * (double)us[k];
+            acc[4 + k] += (double)ws[k] * (double)us[k];
+            acc[8 + k] += (double)o.r[k] * (double)o.r[k];
         }
     }
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_pcg_init_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                                  long long n, const V4<T>* r, V4<T>* u,
-                                                                  V4<T>* w, V4<T>* z, V4<T>* q, V4<T>* s,
-                                                                  V4<T>* p, T im) {
-    const int rb = blockIdx.y;
+__global__ __launch_bounds__(NTHREADS) void batch_pcg_init_kernel(TileGrid g, BCtl c, int rb0, compute_t<T> a,
+                                                                  compute_t<T> b, compute_t<T> cc, long long n,
+                                                                  const V4<T>* r, V4<T>* u, V4<T>* w, V4<T>* z,
+                                                                  V4<T>* q, V4<T>* s, V4<T>* p,
+                                                                  compute_t<T> im) {
+    const int rb = rb0 + blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ V4<compute_t<T>> lds[2][TY + 2][TX + 2];
     __shared__ double red[12 * 4];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
@@ -85,60 +89,65 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_init_kernel(TileGrid g, BC
 // recurrences in the reference's order, the next <r,u>, <w,u>, <r,r>
 template <class T>
 struct PassBPcgIter {
-    using V = V4<T>;
+    using C = compute_t<T>;
+    using V = V4<C>;
+    using S = V4<T>;
     static constexpr int ND = 12, D = SEDI_PREFETCH_BATCH;
-    T a, b, c;
-    const V* wold;
-    V* wnew;
-    V *x, *r, *u, *z, *q, *s, *p;
-    T im;
+    C a, b, c;
+    const S* wold;
+    S* wnew;
+    S *x, *r, *u, *z, *q, *s, *p;
+    C im;
     V alpha, beta;
-    struct Raw { V w; };
-    struct Aux { V q, s, u, p, z, r, x; };
+    struct Raw { S w; };
+    struct Aux { S q, s, u, p, z, r, x; };
     __device__ Raw load(cidx_t i) const { return {ldc(wold, i)}; }
     __device__ Aux load_aux(cidx_t i) const {
         return {ldc(q, i), ldc(s, i), ldc(u, i), ldc(p, i), ldc(z, i), ldc(r, i), ldc(x, i)};
     }
-    __device__ V f(const Raw& c) const { return im * c.w; }
+    __device__ V f(const Raw& o) const { return im * cvt<C>(o.w); }
     __device__ void finish(cidx_t i, const Raw& c, const Aux& o, const V& m, const V& nv,
                            double (&acc)[ND]) const {
-        const V qv = m + beta * o.q;
-        const V sv = c.w + beta * o.s;
-        const V pv = o.u + beta * o.p;
-        const V zv = nv + beta * o.z;
-        const V rv = o.r - alpha * sv;
-        const V xv = o.x + alpha * pv;
-        const V uv = o.u - alpha * qv;
-        const V wv = c.w - alpha * zv;
-        stv(q, i, qv); stv(s, i, sv); stv(p, i, pv); stv(z, i, zv);
-        stv(r, i, rv); stv(x, i, xv); stv(u, i, uv); stv(wnew, i, wv);
+        const V w = cvt<C>(c.w), uo = cvt<C>(o.u);
+        const V qv = m + beta * cvt<C>(o.q);
+        const V sv = w + beta * cvt<C>(o.s);
+        const V pv = uo + beta * cvt<C>(o.p);
+        const V zv = nv + beta * cvt<C>(o.z);
+        const S rs = cvt<T>(cvt<C>(o.r) - alpha * sv);
+        const S xs = cvt<T>(cvt<C>(o.x) + alpha * pv);
+        const S us = cvt<T>(uo - alpha * qv);
+        const S ws = cvt<T>(w - alpha * zv);
+        stv(q, i, cvt<T>(qv)); stv(s, i, cvt<T>(sv)); stv(p, i, cvt<T>(pv)); stv(z, i, cvt<T>(zv));
+        stv(r, i, rs); stv(x, i, xs); stv(u, i, us); stv(wnew, i, ws);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            acc[k] += (double)rv[k] * (double)uv[k];
-            acc[4 + k] += (double)wv[k] * (double)uv[k];
-            acc[8 + k] += (double)rv[k] * (double)rv[k];
+            acc[k] += (double)rs[k] * (double)us[k];
+            acc[4 + k] += (double)ws[k] * (double)us[k];
+            acc[8 + k] += (double)rs[k] * (double)rs[k];
         }
     }
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                                  long long n, V4<T>* w0, V4<T>* w1,
-                                                                  V4<T>* x, V4<T>* r, V4<T>* u, V4<T>* z,
-                                                                  V4<T>* q, V4<T>* s, V4<T>* p, T im) {
+__global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BCtl c, compute_t<T> a,
+                                                                  compute_t<T> b, compute_t<T> cc, long long n,
+                                                                  V4<T>* w0, V4<T>* w1, V4<T>* x, V4<T>* r,
+                                                                  V4<T>* u, V4<T>* z, V4<T>* q, V4<T>* s,
+                                                                  V4<T>* p, compute_t<T> im) {
+    using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
+    __shared__ V4<C> lds[2][TY + 2][TX + 2];
     __shared__ double red[12 * 4];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
-    V4<T> alpha, beta;
+    V4<C> alpha, beta;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const bool d = c.done[4 * rb + k];
-        alpha[k] = d ? (T)0 : (T)c.alpha[4 * rb + k];
-        beta[k] = d ? (T)0 : (T)c.beta[4 * rb + k];
+        alpha[k] = d ? (C)0 : (C)c.alpha[4 * rb + k];
+        beta[k] = d ? (C)0 : (C)c.beta[4 * rb + k];
     }
     // iteration 1 reads w from w0 (written by the set-up pass)
     PassBPcgIter<T> P{a, b, cc, ((it & 1) ? w0 : w1) + off, ((it & 1) ? w1 : w0) + off, x + off, r + off,
@@ -154,8 +163,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BC
         const double err = c.errp[k];  // rr before this iteration's update (:227)
         c.err[k] = err;
         c.hist[(long long)k * c.hstride + it - 1] = err;
-        if (err <= c.tol) { bset_done(c, k, 0, it, err); continue; }
-        if (it >= c.maxiter) { bcg_exhausted(c, k, err); continue; }
+        if (err <= c.tolv[k]) { bset_done(c, k, 0, it, err); continue; }
+        if (it >= c.maxv[k]) { bcg_exhausted(c, k, err); continue; }
         // scalars of iteration it+1 (PipelinedCg.hpp:167-178)
         const double gamma_old = c.gamma[k];
         const double gamma = acc[q4], delta = acc[4 + q4];
@@ -169,8 +178,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BC
         c.alpha[k] = alpha_n;
         const double e = sqrt(acc[8 + q4] / c.bb[k]);
         c.errp[k] = e;
+        if (c.mixed) c.elast[k] = fmax(c.elast[k], e);
         if (c.mixed)
-            c.rep[k] = (e <= c.tol) || (it % c.rep_every == 0) || (it + 1 >= c.maxiter) ||
+            c.rep[k] = (e <= c.tolv[k]) || (it % c.rep_every == 0) || (it + 1 >= c.maxv[k]) ||
                        (e < c.drop * c.elast[k]);
     }
     c.iter[rb] = it + 1;
@@ -336,19 +346,21 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_repb_kernel(TileGrid g, BC
 
 // ---- host ------------------------------------------------------------------
 template <class T>
-int pcg_setup(const BRun& r) {
-    const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
+int pcg_setup(const BRun& r, int rb0, int count) {
+    using C = compute_t<T>;
+    const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
     const KGeom g = bgeom(batch_pcg_init_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
-    SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_init_kernel", batch_pcg_init_kernel, T, g, dim3(g.nblocks, r.nrb),
-                             r.stream, g.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(PCG_R), V(PCG_U), V(PCG_W0),
-                             V(PCG_Z), V(PCG_Q), V(PCG_S), V(PCG_P), im));
+    SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_init_kernel", batch_pcg_init_kernel, T, g, dim3(g.nblocks, count),
+                             r.stream, g.tg, r.c, rb0, a, b, cc, r.n, (const V4<T>*)V(PCG_R), V(PCG_U),
+                             V(PCG_W0), V(PCG_Z), V(PCG_Q), V(PCG_S), V(PCG_P), im));
     return SEDI_OK;
 }
 
 template <class T>
 int pcg_iteration(const BRun& r) {
-    const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
+    using C = compute_t<T>;
+    const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
     const KGeom g = bgeom(batch_pcg_iter_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_iter_kernel", batch_pcg_iter_kernel, T, g, dim3(g.nblocks, r.nrb),
@@ -367,8 +379,8 @@ int pcg_iteration(const BRun& r) {
     return fold_only<T>(r);
 }
 
-template int pcg_setup<float>(const BRun&);
-template int pcg_setup<double>(const BRun&);
+template int pcg_setup<float>(const BRun&, int, int);
+template int pcg_setup<double>(const BRun&, int, int);
 template int pcg_iteration<float>(const BRun&);
 template int pcg_iteration<double>(const BRun&);
 

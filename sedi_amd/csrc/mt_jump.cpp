@@ -225,6 +225,29 @@ const std::vector<uint64_t>& mt_probe_poly(long long ncells, int k) {
 
 int mt_poly_words() { return W; }
 
+void mt_poly_powers(unsigned long long J, int count, std::vector<uint64_t>& out) {
+    static std::mutex mu;
+    static std::map<std::pair<unsigned long long, int>, std::vector<uint64_t>> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find({J, count});
+        if (it != cache.end()) {
+            out = it->second;
+            return;
+        }
+    }
+    std::vector<uint64_t> all((size_t)W * count, 0);
+    const Poly& step = mt_jump_poly(J);
+    Poly cur(W, 0);
+    cur[0] = 1;
+    for (int j = 0; j < count; j++) {
+        std::memcpy(all.data() + (size_t)W * j, cur.data(), sizeof(uint64_t) * W);
+        if (j + 1 < count) cur = mul_mod(cur, step, charpoly());
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    out = cache.emplace(std::make_pair(J, count), std::move(all)).first->second;
+}
+
 void mt_word_sequence(const uint32_t* s, uint32_t* w, int count) {
     std::vector<uint32_t> v;
     word_sequence(s, v, (size_t)count);

@@ -16,6 +16,9 @@ bool mt_charpoly_ok();
 // x^(2*ncells*k) mod phi: the jump from a stream's start to probe k of a grid
 // of ncells cells (two outputs per cell).  Cached; independent of the seed.
 const std::vector<uint64_t>& mt_probe_poly(long long ncells, int k);
+// x^(J*j) mod phi for j = 0 .. count-1, concatenated (count * mt_poly_words()
+// words; cached per (J, count)).
+void mt_poly_powers(unsigned long long J, int count, std::vector<uint64_t>& out);
 // Words per cached polynomial (19937 bits rounded up, plus one).
 int mt_poly_words();
 // Untempered word sequence w_0 .. w_{count-1} from state s (w_0..w_623 = s).

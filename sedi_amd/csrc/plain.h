@@ -49,6 +49,8 @@ class PlainSolver {
   private:
     template <class PC> int launch_setup(PC pc, const double* b, double* x);
     template <class PC> int launch_iterations(int count, PC pc, double* x);
+    // count iterations: direct launches, or the replay of a captured graph
+    int iterate(int count, const double* iM, double* x);
     Shape shape_{};
     double a_ = 6.0, b_ = 1.0, c_ = 1.0;
     int kind_ = 2, mode_ = 0;
@@ -61,6 +63,11 @@ class PlainSolver {
     double* v_[10] = {};        // the solver's device vectors, named in create()
     std::vector<double> hist_host_;
     int nhist_ = 0;
+    // captured chunk of iterations and what its launches were bound to
+    hipGraphExec_t graph_ = nullptr;
+    int graph_count_ = 0;
+    const void *graph_x_ = nullptr, *graph_m_ = nullptr, *graph_hist_ = nullptr;
+    bool geometry_ready_[2] = {false, false};  // scalar / vector preconditioner kernels
 };
 
 }  // namespace sedi

@@ -609,6 +609,7 @@ int stencil_apply_dev(const Shape& s, double a, double b, double c, const double
 
 PlainSolver::~PlainSolver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (pinned_) (void)hipHostFree(pinned_);
@@ -697,6 +698,46 @@ int PlainSolver::launch_iterations(int count, PC pc, double* x) {
     return SEDI_OK;
 }
 
+// The first chunk of a solver (per preconditioner form) is launched directly,
+// which settles the passes' geometry; later chunks replay one captured graph.
+// Every per-iteration value lives in device memory, so the graph only depends
+// on the pointers its launches were given, which are checked here.
+int PlainSolver::iterate(int count, const double* iM, double* x) {
+    const PrecondScalar pcs{1.0 / a_};  // stochastic-estimator.cc:103
+    const PrecondVec pcv{iM};
+    auto direct = [&]() { return iM ? launch_iterations(count, pcv, x) : launch_iterations(count, pcs, x); };
+    bool& ready = geometry_ready_[iM ? 1 : 0];
+    if (!graphs_enabled() || profile_any() || !ready) {
+        const int rc = direct();
+        ready = true;
+        return rc;
+    }
+    if (!graph_ || graph_count_ != count || graph_x_ != x || graph_m_ != iM || graph_hist_ != hist_.p) {
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+        SEDI_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        const int rc = direct();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream_, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SEDI_HIP(e);
+        const hipError_t ei = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SEDI_HIP(ei);
+        graph_count_ = count;
+        graph_x_ = x;
+        graph_m_ = iM;
+        graph_hist_ = hist_.p;
+    }
+    SEDI_HIP(hipGraphLaunch(graph_, stream_));
+    return SEDI_OK;
+}
+
 int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int maxiter, double tol,
                            double* final_err, int* final_iter) {
     SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
@@ -718,8 +759,7 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
     bool pending = false;
     while (launched < maxiter) {
         const int cnt = std::min(chunk, maxiter - launched);
-        rc = iM ? launch_iterations(cnt, pcv, x) : launch_iterations(cnt, pcs, x);
-        if (rc) return rc;
+        if ((rc = iterate(cnt, iM, x))) return rc;
         launched += cnt;
         SEDI_HIP(hipMemcpyAsync(&pinned_[slot], st, sizeof(DevState), hipMemcpyDeviceToHost, stream_));
         SEDI_HIP(hipEventRecord(ev_[slot], stream_));

@@ -8,6 +8,12 @@ namespace sedi {
 
 // True when launches of `name` are to be bracketed with events.
 bool profile_wants(const char* name);
+// True when any kernel is being timed (launches are then not graph-captured).
+bool profile_any();
+// Whether chunks of solver iterations are captured into HIP graphs and
+// replayed (default on; sedi_graphs_enable).
+bool graphs_enabled();
+void set_graphs_enabled(bool on);
 // Records the start event (returns a token, -1 if not profiling this kernel).
 int profile_begin(const char* name, hipStream_t stream);
 void profile_end(int token, hipStream_t stream);

@@ -29,6 +29,24 @@ class MtStream {
     DevBuf state_;
 };
 
+// Probes k0 .. k0+count-1 of the stream mt19937(seed) on a grid of ncells
+// cells, generated in parallel: init() uploads the stream's start and the
+// probes' jump polynomials and computes each probe's start (asynchronous on
+// `stream` after a synchronous upload); emit() writes probes
+// k0+first .. k0+first+nprobe-1 into slots 0 .. nprobe-1 of a batch layout
+// [K/4][ncells][4] (float or double) and zeros into the other slots.
+class ProbeSource {
+  public:
+    int init(uint32_t seed, long long ncells, int k0, int count, hipStream_t stream);
+    int emit(int first, int nprobe, int K, bool fp32, void* out, hipStream_t stream) const;
+    int count() const { return count_; }
+
+  private:
+    DevBuf up_, wk_;
+    long long ncells_ = 0;
+    int count_ = 0;
+};
+
 // Probes k0 .. k0+kact-1 of the stream mt19937(seed) on a grid of ncells
 // cells, written as +-1 into the batch layout [K/4][ncells][4] (float or
 // double); slots kact .. K-1 are zeroed.  K is a multiple of 4.  Uses

@@ -20,7 +20,11 @@
 #include "mt_jump.h"
 #include "profiler.h"
 
+#include <algorithm>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 namespace sedi {
@@ -151,110 +155,212 @@ __global__ __launch_bounds__(256) void mt_fill_kernel(MtState* s, double* out, l
 }
 
 // ---------------------------------------------------------------------------
-// Batched probes: workgroup k produces probe k0+k of the stream.  Its start
-// state is the jump of the stream's start state by 2N(k0+k) outputs: with
-// q = x^(2N(k0+k)) mod phi (cached on the host per grid size) the jumped
-// state's word j is the XOR of w_{i+j} over the set bits i of q, where w is
-// the stream's untempered word sequence from its start (uploaded once per
-// seed).  The workgroup then regenerates and tempers like mt_fill_kernel and
-// writes its probe into lane k%4 of RHS block k/4 of `out` ([K/4][N][4] of T).
+// Batched probes, in parallel.  Probe k of the stream starts 2Nk outputs
+// after the stream's start; with q = x^J mod phi the state J outputs after a
+// state s has word j = XOR of w_{i+j} over the set bits i of q, where w is the
+// untempered word sequence from s (Haramoto et al. 2008, see mt_jump.cpp).
+//   1. mt_probe_start_kernel, one workgroup per probe: jumps the stream's
+//      start (whose word sequence is uploaded once per seed) by 2Nk and
+//      writes the probe's own word sequence (MT_WINDOW words);
+//   2. mt_probe_seg_kernel, one workgroup per (segment, probe): jumps that
+//      sequence by 2*S*j to the start of segment j (S cells; the polynomials
+//      are seed-independent and cached per grid size) and generates the
+//      segment like mt_fill_kernel, into lane k%4 of RHS block k/4 of the
+//      batch layout [K/4][N][4].
+// Segments start at even outputs, so a cell's two outputs never straddle.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void mt_batch_kernel(const uint32_t* __restrict__ wseq,
-                                                       const uint64_t* __restrict__ polys, int pw,
-                                                       long long ncells, int kact,
-                                                       T* __restrict__ out) {
-    __shared__ uint32_t S[2][MT_N];
-    __shared__ uint32_t O[2][MT_N];
+
+// The state `q` jumps to, from the word sequence w: S[j] = XOR_{q_i = 1} w[i+j].
+__device__ __forceinline__ void jump_state(const uint32_t* __restrict__ w, const uint64_t* __restrict__ q,
+                                           int pw, uint32_t* S) {
     const int t = threadIdx.x;
-    const int k = blockIdx.x;
-    if (k >= kact) {  // padding: a zero right-hand side
-        T* z = out + (long long)(k >> 2) * ncells * 4 + (k & 3);
-        for (long long i = t; i < ncells; i += 256) z[i * 4] = (T)0;
-        return;
-    }
-    const uint64_t* q = polys + (long long)k * pw;
     uint32_t a0 = 0, a1 = 0, a2 = 0;
     for (int wi = 0; wi < pw; wi++) {
         uint64_t bits = q[wi];
         while (bits) {
             const int b = __builtin_ctzll(bits);
             bits &= bits - 1;
-            const uint32_t* src = wseq + wi * 64 + b;
+            const uint32_t* src = w + wi * 64 + b;
             a0 ^= src[t];
             a1 ^= src[t + 256];
             if (t < MT_N - 512) a2 ^= src[t + 512];
         }
     }
-    S[0][t] = a0;
-    S[0][t + 256] = a1;
-    if (t < MT_N - 512) S[0][t + 512] = a2;
-    __syncthreads();
+    S[t] = a0;
+    S[t + 256] = a1;
+    if (t < MT_N - 512) S[t + 512] = a2;
+}
 
-    T* dst = out + (long long)(k >> 2) * ncells * 4 + (k & 3);
+// One regeneration of the 624-word state S[cur] into S[nxt] (and, if O is
+// given, the tempered outputs into O[nxt]); three dependent phases.
+__device__ __forceinline__ void regenerate(uint32_t (*S)[MT_N], uint32_t (*O)[MT_N], int cur) {
+    const int t = threadIdx.x;
+    const int nxt = cur ^ 1;
+    if (t < 227) {
+        const uint32_t v = twist(S[cur][t], S[cur][t + 1], S[cur][t + MT_M]);
+        S[nxt][t] = v;
+        if (O) O[nxt][t] = temper(v);
+    }
+    __syncthreads();
+    if (t < 227) {
+        const int i = t + 227;
+        const uint32_t v = twist(S[cur][i], S[cur][i + 1], S[nxt][i - 227]);
+        S[nxt][i] = v;
+        if (O) O[nxt][i] = temper(v);
+    }
+    __syncthreads();
+    if (t < 170) {
+        const int i = t + 454;
+        const uint32_t nx = (i + 1 < MT_N) ? S[cur][i + 1] : S[nxt][0];
+        const uint32_t v = twist(S[cur][i], nx, S[nxt][i - 227]);
+        S[nxt][i] = v;
+        if (O) O[nxt][i] = temper(v);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void mt_probe_start_kernel(const uint32_t* __restrict__ wseq,
+                                                             const uint64_t* __restrict__ polys, int pw,
+                                                             uint32_t* __restrict__ wk) {
+    __shared__ uint32_t S[2][MT_N];
+    const int k = blockIdx.x;
+    uint32_t* out = wk + (long long)k * MT_WINDOW;
+    jump_state(wseq, polys + (long long)k * pw, pw, S[0]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < MT_N; i += 256) out[i] = S[0][i];
+    int cur = 0;
+    for (int produced = MT_N; produced < MT_WINDOW; produced += MT_N) {
+        regenerate(S, nullptr, cur);
+        cur ^= 1;
+        for (int i = threadIdx.x; i < MT_N && produced + i < MT_WINDOW; i += 256) out[produced + i] = S[cur][i];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void mt_probe_seg_kernel(const uint32_t* __restrict__ wk,
+                                                           const uint64_t* __restrict__ segpolys, int pw,
+                                                           long long ncells, long long seg_cells, int nprobe,
+                                                           T* __restrict__ out) {
+    __shared__ uint32_t S[2][MT_N];
+    __shared__ uint32_t O[2][MT_N];
+    const int t = threadIdx.x;
+    const int j = blockIdx.x, k = blockIdx.y;
+    const long long cell0 = (long long)j * seg_cells;
+    if (cell0 >= ncells) return;
+    const long long cells = min(seg_cells, ncells - cell0);
+    T* dst = out + (long long)(k >> 2) * ncells * 4 + (k & 3) + cell0 * 4;
+    if (k >= nprobe) {  // padding: a zero right-hand side
+        for (long long i = t; i < cells; i += 256) dst[i * 4] = (T)0;
+        return;
+    }
+    jump_state(wk + (long long)k * MT_WINDOW, segpolys + (long long)j * pw, pw, S[0]);
+    __syncthreads();
     int cur = 0;
     long long done = 0;
-    while (done < ncells) {
-        // regenerate (as mt_fill_kernel)
-        const int nxt = cur ^ 1;
-        if (t < 227) {
-            const uint32_t v = twist(S[cur][t], S[cur][t + 1], S[cur][t + MT_M]);
-            S[nxt][t] = v;
-            O[nxt][t] = temper(v);
-        }
-        __syncthreads();
-        if (t < 227) {
-            const int i = t + 227;
-            const uint32_t v = twist(S[cur][i], S[cur][i + 1], S[nxt][i - 227]);
-            S[nxt][i] = v;
-            O[nxt][i] = temper(v);
-        }
-        __syncthreads();
-        if (t < 170) {
-            const int i = t + 454;
-            const uint32_t nx = (i + 1 < MT_N) ? S[cur][i + 1] : S[nxt][0];
-            const uint32_t v = twist(S[cur][i], nx, S[nxt][i - 227]);
-            S[nxt][i] = v;
-            O[nxt][i] = temper(v);
-        }
-        __syncthreads();
-        cur = nxt;
-        // 312 cells per block: the probe starts at an even output, so pairs
-        // never straddle two blocks
+    while (done < cells) {
+        regenerate(S, O, cur);
+        cur ^= 1;
+        // 312 cells per block of outputs
         long long pairs = MT_N / 2;
-        if (pairs > ncells - done) pairs = ncells - done;
+        if (pairs > cells - done) pairs = cells - done;
         for (int p = t; p < pairs; p += 256)
             dst[(done + p) * 4] = (T)rademacher(O[cur][2 * p], O[cur][2 * p + 1]);
         done += pairs;
     }
 }
 
-int fill_probe_batch(uint32_t seed, long long ncells, int k0, int K, int kact, bool fp32, void* out,
-                     hipStream_t stream, DevBuf& scratch) {
-    SEDI_REQUIRE(K > 0 && K % 4 == 0 && kact >= 0 && kact <= K && k0 >= 0, SEDI_ERR_ARG,
-                 "bad probe batch (k0 %d, K %d, active %d)", k0, K, kact);
+namespace {
+struct SegPlan {
+    long long seg_cells = 0;
+    int nseg = 0;
+    DevBuf polys;  // nseg x pw words: x^(2*seg_cells*j) mod phi
+};
+
+// Segment plan of a grid size: at least 65536 cells and at most 256
+// segments per probe; built once per process.
+const SegPlan* seg_plan(long long ncells, int* rc) {
+    static std::mutex mu;
+    static std::map<long long, std::unique_ptr<SegPlan>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(ncells);
+    if (it != cache.end()) return it->second.get();
+    std::unique_ptr<SegPlan> p(new SegPlan());
+    p->seg_cells = std::max<long long>(65536, (ncells + 255) / 256);
+    p->nseg = (int)((ncells + p->seg_cells - 1) / p->seg_cells);
+    const int pw = mt_poly_words();
+    std::vector<uint64_t> host;
+    mt_poly_powers(2ull * (unsigned long long)p->seg_cells, p->nseg, host);
+    if (p->polys.reserve(sizeof(uint64_t) * host.size()) != hipSuccess ||
+        hipMemcpy(p->polys.p, host.data(), sizeof(uint64_t) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("cannot upload the probe segment polynomials");
+        *rc = SEDI_ERR_HIP;
+        return nullptr;
+    }
+    (void)pw;
+    return cache.emplace(ncells, std::move(p)).first->second.get();
+}
+}  // namespace
+
+int ProbeSource::init(uint32_t seed, long long ncells, int k0, int count, hipStream_t stream) {
+    SEDI_REQUIRE(ncells > 0 && k0 >= 0 && count >= 0, SEDI_ERR_ARG, "bad probe source (k0 %d, count %d)", k0,
+                 count);
+    SEDI_REQUIRE(mt_charpoly_ok(), SEDI_ERR_STATE, "mt19937 characteristic polynomial unavailable");
+    ncells_ = ncells;
+    count_ = count;
+    if (count == 0) return SEDI_OK;
+    int rc = SEDI_OK;
+    if (!seg_plan(ncells, &rc)) return rc;
     const int pw = mt_poly_words();
     const size_t wbytes = sizeof(uint32_t) * MT_WINDOW;
-    const size_t pbytes = sizeof(uint64_t) * (size_t)pw * K;
+    const size_t pbytes = sizeof(uint64_t) * (size_t)pw * count;
     std::vector<uint8_t> host(wbytes + pbytes);
     uint32_t s0[624];
     mt_seed_state(seed, s0);
     mt_word_sequence(s0, reinterpret_cast<uint32_t*>(host.data()), MT_WINDOW);
-    for (int k = 0; k < kact; k++) {
+    for (int k = 0; k < count; k++) {
         const std::vector<uint64_t>& q = mt_probe_poly(ncells, k0 + k);
         std::memcpy(host.data() + wbytes + sizeof(uint64_t) * (size_t)pw * k, q.data(), sizeof(uint64_t) * pw);
     }
-    SEDI_HIP(scratch.reserve(host.size()));
-    SEDI_HIP(hipMemcpyAsync(scratch.p, host.data(), host.size(), hipMemcpyHostToDevice, stream));
-    const uint32_t* wseq = scratch.as<uint32_t>();
-    const uint64_t* polys = reinterpret_cast<const uint64_t*>(static_cast<char*>(scratch.p) + wbytes);
+    SEDI_HIP(up_.reserve(host.size()));
+    SEDI_HIP(wk_.reserve(sizeof(uint32_t) * (size_t)MT_WINDOW * count));
+    // the host staging vector must outlive the copy: a synchronous one
+    SEDI_HIP(hipMemcpy(up_.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    const uint32_t* wseq = up_.as<uint32_t>();
+    const uint64_t* polys = reinterpret_cast<const uint64_t*>(static_cast<char*>(up_.p) + wbytes);
+    SEDI_HIP(launch("mt_probe_start_kernel", mt_probe_start_kernel, dim3(count), dim3(256), stream, wseq, polys,
+                    pw, wk_.as<uint32_t>()));
+    return SEDI_OK;
+}
+
+int ProbeSource::emit(int first, int nprobe, int K, bool fp32, void* out, hipStream_t stream) const {
+    SEDI_REQUIRE(K > 0 && K % 4 == 0 && first >= 0 && nprobe >= 0 && nprobe <= K && first + nprobe <= count_,
+                 SEDI_ERR_ARG, "bad probe emit (first %d, count %d of %d, K %d)", first, nprobe, count_, K);
+    int rc = SEDI_OK;
+    const SegPlan* plan = seg_plan(ncells_, &rc);
+    if (!plan) return rc;
+    const uint32_t* wk = wk_.as<uint32_t>() + (size_t)MT_WINDOW * first;
+    const dim3 grid(plan->nseg, K);
     if (fp32)
-        SEDI_HIP(launch("mt_batch_kernel", mt_batch_kernel<float>, dim3(K), dim3(256), stream, wseq, polys,
-                        pw, ncells, kact, static_cast<float*>(out)));
+        SEDI_HIP(launch("mt_probe_seg_kernel", mt_probe_seg_kernel<float>, grid, dim3(256), stream, wk,
+                        plan->polys.as<uint64_t>(), mt_poly_words(), ncells_, plan->seg_cells, nprobe,
+                        static_cast<float*>(out)));
     else
-        SEDI_HIP(launch("mt_batch_kernel", mt_batch_kernel<double>, dim3(K), dim3(256), stream, wseq, polys,
-                        pw, ncells, kact, static_cast<double*>(out)));
-    // the host staging buffer must outlive the asynchronous copy
+        SEDI_HIP(launch("mt_probe_seg_kernel", mt_probe_seg_kernel<double>, grid, dim3(256), stream, wk,
+                        plan->polys.as<uint64_t>(), mt_poly_words(), ncells_, plan->seg_cells, nprobe,
+                        static_cast<double*>(out)));
+    return SEDI_OK;
+}
+
+int fill_probe_batch(uint32_t seed, long long ncells, int k0, int K, int kact, bool fp32, void* out,
+                     hipStream_t stream, DevBuf& scratch) {
+    SEDI_REQUIRE(K > 0 && K % 4 == 0 && kact >= 0 && kact <= K && k0 >= 0, SEDI_ERR_ARG,
+                 "bad probe batch (k0 %d, K %d, active %d)", k0, K, kact);
+    (void)scratch;
+    ProbeSource src;
+    int rc = src.init(seed, ncells, k0, kact, stream);
+    if (!rc) rc = src.emit(0, kact, K, fp32, out, stream);
+    if (rc) return rc;
+    // the source's buffers are released on return
     SEDI_HIP(hipStreamSynchronize(stream));
     return SEDI_OK;
 }

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -77,6 +78,15 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
+bool profile_any() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return !g_target.empty();
+}
+
+static std::atomic<bool> g_graphs{true};
+bool graphs_enabled() { return g_graphs.load(); }
+void set_graphs_enabled(bool on) { g_graphs.store(on); }
+
 bool profile_wants(const char* name) {
     if (g_target.empty()) return false;
     return g_target == "*" || g_target == name;
@@ -129,6 +139,11 @@ int sedi_init(int device) {
     if (rc) return rc;
     SEDI_HIP(hipSetDevice(device));
     SEDI_HIP(hipFree(nullptr));
+    return SEDI_OK;
+}
+
+int sedi_graphs_enable(int on) {
+    sedi::set_graphs_enabled(on != 0);
     return SEDI_OK;
 }
 
