@@ -14,7 +14,8 @@ inputs are resident in HBM when the timed region starts.
 Beside it, on the same grid, stream and tolerance: the plain FP64
 one-probe-at-a-time CG (the drop-in) on the first 16 probes, the speed-up of
 the batched form over it, and the max / RMS relative error of the mixed
-form's diagonal against the FP64 one on those 16 probes.
+form's diagonal against the FP64 one on those 16 probes; and the p50 / p99
+job latency of a 30-job sample of the serving stream (--serve-jobs).
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (its algorithmic bytes per launch over its average duration, measured with HIP
@@ -31,7 +32,13 @@ iteration counts and the mixed diagonal's error against the FP64 one.
 batches of 16, tol 1e-8, maxiter 4000; the replacement policy of each
 solver's mixed form) against the plain FP64 GPU solvers on the same probes:
 throughput and max / RMS diagonal error.
-These two print one JSON line per run (progress) and a summary line.
+--mode serve: the serving stream -- Poisson arrivals at --rate jobs/s, grid
+side uniform over {64, 96, 128, 192, 256}, samples uniform in [8, 64], tol
+uniform over {1e-6, 1e-8, 1e-9}, solver uniform over CG / pipelined CG /
+BiCGStab, seed = job id, mixed batched kernels, 200 GB admission budget --
+truncated to --jobs jobs: p50 / p99 job latency (submit to result) and
+aggregate probe-solves/s.
+These print progress lines and a summary line.
 """
 import argparse
 import json
@@ -68,7 +75,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512"])
+    p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512", "serve"])
+    p.add_argument("--jobs", type=int, default=200, help="serve: jobs in the stream")
+    p.add_argument("--rate", type=float, default=5.0, help="serve: Poisson arrival rate (jobs/s)")
+    p.add_argument("--serve-jobs", type=int, default=30,
+                   help="headline: jobs of the serving sample reported beside it (0 = none)")
     p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
     p.add_argument("--solver", default="cg", choices=sorted(NAMES), help="--mode plain only")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -327,6 +338,14 @@ def run_headline(args):
         line["speedup_vs_plain_fp64"] = round(value / pv, 3)
         line["mixed_vs_fp64_D"] = dict(d_errors(rm["D"], rp["D"]), probes=CMP_SAMPLES,
                                        iterations_mean_mixed=round(float(rm["iters"].mean()), 2))
+    if args.serve_jobs > 0:
+        # the metric's third part: job latency of the serving stream, on a
+        # bounded sample of it
+        line["serving"] = dict(run_serve(args.serve_jobs, 5.0),
+                               workload="the serving stream's first %d jobs at 5 jobs/s (Poisson; sides "
+                                        "64-256, 8-64 probes, tol 1e-6/1e-8/1e-9, CG/PipelinedCG/BiCGStab, "
+                                        "seed = job id, mixed, 200 GB budget); the GPU is saturated, so "
+                                        "latency includes queueing" % args.serve_jobs)
     line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(2, BATCH["seed"], BATCH["tol"], float(iters.mean()),
@@ -405,6 +424,78 @@ def run_c512(args):
     return out
 
 
+SERVE_SIDES = (64, 96, 128, 192, 256)
+SERVE_TOLS = (1e-6, 1e-8, 1e-9)
+SERVE_KINDS = (2, 3, 1)
+
+
+def serve_stream(jobs, rate, seed=0):
+    """The job stream: arrival times and job parameters (seed = job id)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=jobs))
+    out = []
+    for i in range(jobs):
+        out.append(dict(t=float(t[i]), side=int(rng.choice(SERVE_SIDES)), samples=int(rng.integers(8, 65)),
+                        tol=float(rng.choice(SERVE_TOLS)), kind=int(rng.choice(SERVE_KINDS)), seed=i))
+    return out
+
+
+def run_serve(jobs, rate, progress=True):
+    """Submits the stream in real time from a thread and waits for every job."""
+    import threading
+    import numpy as np
+    import sedi_amd
+    stream = serve_stream(jobs, rate)
+    srv = sedi_amd.Server(hbm_budget=200e9, precision=sedi_amd.FP32_MIXED)
+    ids = [None] * len(stream)
+    ready = [threading.Event() for _ in stream]
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, j in enumerate(stream):
+            dt = j["t"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit((j["side"],) * 3, kind=j["kind"], samples=j["samples"], tol=j["tol"],
+                                maxiter=MAXITER, seed=j["seed"])
+            ready[i].set()
+
+    th = threading.Thread(target=submitter)
+    t0 = time.perf_counter()
+    th.start()
+    lat, probes, failed, first_submit, last_done = [], 0, 0, None, 0.0
+    per_side = {}
+    for i, j in enumerate(stream):
+        ready[i].wait()
+        r = srv.wait(ids[i])
+        tm = r["times"]
+        first_submit = tm["submit"] if first_submit is None else min(first_submit, tm["submit"])
+        last_done = max(last_done, tm["done"])
+        if r["status"] != 0:
+            failed += 1
+            continue
+        lat.append(tm["done"] - tm["submit"])
+        probes += j["samples"]
+        per_side.setdefault(j["side"], []).append(tm["done"] - tm["submit"])
+        if progress and (i + 1) % 10 == 0:
+            print("serve: %d/%d jobs done, %.1f s" % (i + 1, len(stream), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    span = last_done - first_submit
+    return {"jobs": len(stream), "rate_jobs_per_s": rate, "failed": failed,
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+            "mean_latency_s": round(float(lat.mean()), 3),
+            "probe_solves_per_s": round(probes / span, 3), "span_s": round(span, 2),
+            "offered_span_s": round(stream[-1]["t"], 2), "peak_device_GB": round(st["peak_bytes"] / 1e9, 2),
+            "p50_latency_by_side_s": {str(k): round(float(np.percentile(v, 50)), 3)
+                                      for k, v in sorted(per_side.items())}}
+
+
 def main():
     args = parse()
     if args.pmc_workload:
@@ -417,7 +508,12 @@ def main():
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.mode == "ksweep":
+    if args.mode == "serve":
+        line = dict(run_serve(args.jobs, args.rate), mode="serve",
+                    workload="Poisson stream truncated to %d jobs at %.1f jobs/s (sides 64-256, 8-64 probes, "
+                             "tol 1e-6/1e-8/1e-9, CG/PipelinedCG/BiCGStab, seed = job id, mixed, 200 GB "
+                             "budget)" % (args.jobs, args.rate))
+    elif args.mode == "ksweep":
         line = run_ksweep(args)
     elif args.mode == "c512":
         line = run_c512(args)

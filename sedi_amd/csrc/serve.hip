@@ -63,6 +63,7 @@ struct Server::Pool {
     void* snap[2] = {nullptr, nullptr};  // pinned status snapshots
     hipEvent_t ev[2] = {nullptr, nullptr};
     int inflight = 0, head = 0;  // chunks posted, oldest snapshot index
+    int chunk = 8;               // iterations per step
     double bytes = 0;
     DevBuf dtmp;
     ~Pool() {
@@ -219,18 +220,14 @@ void Server::engine() {
             break;
         }
         if (!progress) {
-            // wait for the oldest chunk in flight anywhere
-            bool waited = false;
-            for (auto& p : pools_)
-                if (p->inflight > 0) {
-                    (void)hipEventSynchronize(p->ev[p->head]);
-                    waited = true;
-                    break;
-                }
-            if (!waited) {  // nothing on the device: a job waits for memory; poll new submissions
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_submit_.wait_for(lk, std::chrono::milliseconds(2));
-            }
+            // nothing to do until some pool's chunk completes (whichever is
+            // first: pools of small grids step much faster than large ones),
+            // a job arrives, or memory frees up
+            bool busy = false;
+            for (auto& p : pools_) busy = busy || p->inflight > 0;
+            std::unique_lock<std::mutex> lk(mu_);
+            if (busy) cv_submit_.wait_for(lk, std::chrono::microseconds(50));
+            else cv_submit_.wait_for(lk, std::chrono::milliseconds(2));
         }
     }
 }
@@ -246,7 +243,7 @@ int Server::engine_step(bool* progress) {
     for (auto& p : pools_) {
         // keep two chunks in flight while blocks run
         while (p->nrunning > 0 && p->inflight < 2) {
-            if ((rc = p->bs->step(cfg_.chunk))) return rc;
+            if ((rc = p->bs->step(p->chunk))) return rc;
             const int slot = (p->head + p->inflight) & 1;
             if ((rc = p->bs->post_status(p->snap[slot], p->ev[slot]))) return rc;
             p->snap_post[slot] = p->posts++;
@@ -266,7 +263,7 @@ int Server::engine_step(bool* progress) {
 }
 
 // Pool for a job's (grid, solver); creating it takes K slots sized so a pool
-// holds at most a quarter of the budget.
+// holds at most an eighth of the budget.
 Server::Pool* Server::pool_for(const ServeJob& j, bool create, int* rc) {
     *rc = SEDI_OK;
     for (auto& p : pools_)
@@ -274,7 +271,9 @@ Server::Pool* Server::pool_for(const ServeJob& j, bool create, int* rc) {
     if (!create) return nullptr;
     const bool fp32 = cfg_.precision == SEDI_FP32_MIXED;
     const double per4 = BatchSolver::bytes_needed(j.s, j.kind, 4, fp32) + 4.0 * cfg_.hist_cap * 8;
-    int K = (int)std::min<double>(cfg_.max_slots, std::floor(cfg_.hbm_budget / 4 / per4) * 4);
+    // at most an eighth of the budget per pool, so pools of several grids and
+    // solvers fit together
+    int K = (int)std::min<double>(cfg_.max_slots, std::floor(cfg_.hbm_budget / 8 / per4) * 4);
     K = std::max(4, K / 4 * 4);
     const double need = per4 * K / 4;
     if (bytes_ + need > cfg_.hbm_budget && !evict_idle(need, nullptr)) return nullptr;
@@ -286,6 +285,10 @@ Server::Pool* Server::pool_for(const ServeJob& j, bool create, int* rc) {
     if ((*rc = p->bs->create(j.s, j.a, j.b, j.c, j.kind, K, fp32, cfg_.rep_every))) return nullptr;
     if ((*rc = p->bs->slots_open(cfg_.hist_cap))) return nullptr;
     p->nrb = K / 4;
+    // steps of similar duration across grids: more iterations per step on
+    // small grids, whose iterations take microseconds
+    const long long n = j.s.n();
+    p->chunk = cfg_.chunk * (n >= (8LL << 20) ? 1 : n >= (1LL << 20) ? 2 : n >= (1LL << 18) ? 4 : 8);
     p->slot_job.assign(p->nrb, nullptr);
     p->slot_k0.assign(p->nrb, 0);
     p->slot_nact.assign(p->nrb, 0);
@@ -333,9 +336,16 @@ bool Server::evict_idle(double need, const Pool* keep) {
 }
 
 // Admits waiting jobs in arrival order while the budget allows.
+// Admits waiting jobs while the budget allows: in arrival order, except that
+// a job that does not fit yet lets later jobs that do fit go ahead, until it
+// has waited `kAgeLimit` seconds (then strict order, so it cannot starve).
+static constexpr double kAgeLimit = 20.0;
+
 int Server::admit() {
-    while (!pending_.empty()) {
-        Job* j = pending_.front();
+    bool blocked = false;  // an older job is waiting for memory
+    for (size_t i = 0; i < pending_.size();) {
+        Job* j = pending_[i];
+        if (blocked && now() - pending_.front()->res.t_submit > kAgeLimit) return SEDI_OK;
         int rc = SEDI_OK;
         Pool* p = pool_for(j->spec, true, &rc);
         if (rc) return rc;
@@ -346,12 +356,17 @@ int Server::admit() {
             // it waits for memory, unless nothing else holds any: then it never fits
             bool idle = admitted_.empty();
             for (auto& q : pools_) idle = idle && q->nrunning == 0 && q->inflight == 0;
-            if (!idle) return SEDI_OK;
-            if (p) evict_idle(1e300, p);
-            if (!p || bytes_ + need > cfg_.hbm_budget) {
-                set_error("job %lld does not fit the device budget", j->id);
-                fail_job(*j, SEDI_ERR_NOMEM);
-                pending_.pop_front();
+            if (idle) {
+                if (p) evict_idle(1e300, p);
+                if (!p || bytes_ + need > cfg_.hbm_budget) {
+                    set_error("job %lld does not fit the device budget", j->id);
+                    fail_job(*j, SEDI_ERR_NOMEM);
+                    pending_.erase(pending_.begin() + i);
+                    continue;
+                }
+            } else {
+                blocked = true;
+                i++;
                 continue;
             }
         }
@@ -373,7 +388,7 @@ int Server::admit() {
             j->res.t_admit = now();
         }
         admitted_.push_back(j);
-        pending_.pop_front();
+        pending_.erase(pending_.begin() + i);
     }
     return SEDI_OK;
 }
