@@ -82,6 +82,7 @@ def parse():
                    help="headline: jobs of the serving sample reported beside it (0 = none)")
     p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
     p.add_argument("--solver", default="cg", choices=sorted(NAMES), help="--mode plain only")
+    p.add_argument("--c512-only", choices=["pcg", "bicgstab"], help="--mode c512: one solver only")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-compare", action="store_true",
                    help="headline: skip the plain FP64 comparison run")
@@ -410,7 +411,10 @@ def run_c512(args):
     grid = (512, 512, 512)
     out = {"mode": "c512", "grid": list(grid), "samples": 32, "seed": 23, "tol": 1e-8, "batch_k": 16,
            "maxiter": 4000, "runs": []}
+    fp64_D = {}
     for kind, name in ((3, "PipelinedCG"), (1, "BiCGStab")):
+        if args.c512_only and NAMES[args.c512_only] != kind:
+            continue
         m = estimator_run(grid, progress=True, maxiter=4000, kind=kind, samples=32, tol=1e-8, seed=23,
                           precision=sedi_amd.FP32_MIXED, batch_k=16, rep_every=50)
         print(json.dumps(dict(m["summary"], solver=name, precision="mixed")), flush=True)
@@ -420,7 +424,12 @@ def run_c512(args):
                             "speedup": round(m["summary"]["probe_solves_per_s"] /
                                              p["summary"]["probe_solves_per_s"], 3),
                             "D_vs_fp64": d_errors(m["D"], p["D"])})
+        fp64_D[name] = p["D"]
         del m, p
+    if len(fp64_D) == 2:
+        # how far two FP64 solvers' estimates lie apart at this tolerance: the
+        # scale against which a mixed result's distance from FP64 is read
+        out["fp64_pcg_vs_fp64_bicgstab_D"] = d_errors(fp64_D["BiCGStab"], fp64_D["PipelinedCG"])
     return out
 
 

@@ -201,8 +201,8 @@ int sedi_estimate(const sedi_estimate_args *args, double *D, int *iters,
  *   (its returned err is then the true residual's) and: CG every rep_every
  *   iterations (0 = 50) and at each tenfold decrease; pipelined CG the same
  *   but at least every 10 iterations (all of r, u, w, s, q, z are replaced);
- *   BiCGStab when the residual has dropped by sqrt(2^-24) since the last
- *   replacement, restarting r^ and the directions there.
+ *   BiCGStab at each tenfold decrease (its solution accumulated in FP64),
+ *   restarting r^ and the directions there.
  * Results of a right-hand side do not depend on the others in its batch or on
  * K.  K is rounded up to a multiple of 4. */
 typedef struct sedi_batch sedi_batch;

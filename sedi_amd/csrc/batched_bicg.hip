@@ -2,12 +2,15 @@
 // codes and absolute s-test of BiCgStab.hpp:117-264, in the three fused
 // passes of the plain solver (direction | half step | update).
 //
-// Mixed form: the FP64 replacement r = b - A x (batched.hip) runs after the
-// update when due; the next iteration's rho = <r, r^> and beta are then taken
-// from the replaced residual (Sleijpen & van der Vorst 1996, the reliable
-// updates BiCGStab was analysed with).  The vectors r, p, v alternate between
-// two buffers with the parity of the iteration, because each pass reads them
-// at the halo while writing their successors.
+// Mixed form: FP32 vectors, FP64 arithmetic and dot products, the solution
+// accumulated in FP64.  The FP64 replacement r = b - A x (batched.hip) runs
+// after the update at each tenfold decrease of the residual and restarts the
+// method there (r^ = r, p = r); the reliable-update analysis of Sleijpen &
+// van der Vorst (1996) motivates replacing at decreases, the restart keeps
+// the FP32 recurrences' loss of bi-orthogonality from building up.  The
+// vectors r, p, v alternate between two buffers with the parity of the
+// iteration, because each pass reads them at the halo while writing their
+// successors.
 #include "batched_common.h"
 
 #include <cmath>
@@ -149,22 +152,44 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, B
 // s, s^, t recomputed; x = x + alpha p^ + omega s^; r = s - omega t; <r,r>,
 // the next rho = <r, r^>.  A RHS that left at the s-test takes x + alpha p^
 // only (:195-199).
-template <class T>
+//
+// XD: x is accumulated in FP64 (four [N] arrays, one per RHS of the block)
+// instead of in the storage type.  The mixed form uses it: BiCGStab runs
+// hundreds of iterations between replacements, and an FP32 accumulator over
+// that many updates loses about eps32 * sqrt(m) * |x_lo| per cell, which A
+// amplifies to ~ eps32 * sqrt(m) * |r| / lambda_min in the replaced residual
+// -- at 512^3 more than the drop that triggered the replacement, so the
+// solver cannot converge (CG and pipelined CG replace every <= 50 and <= 10
+// iterations, where the FP32 accumulator suffices).
+template <class T, bool XD>
 struct PassBBicgUpd {
     using C = compute_t<T>;
     using V = V4<C>;
     using S = V4<T>;
+    using XV = typename std::conditional<XD, V4<double>, S>::type;
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
     C a, b, c;
     const S *r, *v, *p, *rhat;
     S *x, *rnew;
+    double* xh[4];  // XD: x of the block's four RHS
     C im;
     V alpha, omega;
     int sexit[4];
     struct Raw { S r, v; };
-    struct Aux { S p, x, rhat; };
+    struct Aux { S p; XV x; S rhat; };
     __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(v, i)}; }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(p, i), ldc((const S*)x, i), ldc(rhat, i)}; }
+    __device__ Aux load_aux(cidx_t i) const {
+        Aux o;
+        o.p = ldc(p, i);
+        if constexpr (XD) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) o.x[k] = ldc((const double*)xh[k], i);
+        } else {
+            o.x = ldc((const S*)x, i);
+        }
+        o.rhat = ldc(rhat, i);
+        return o;
+    }
     __device__ S sval(const Raw& q) const { return cvt<T>(cvt<C>(q.r) - alpha * cvt<C>(q.v)); }
     __device__ V f(const Raw& q) const { return im * cvt<C>(sval(q)); }
     __device__ void finish(cidx_t i, const Raw& q, const Aux& o, const V& sh, const V& t,
@@ -174,7 +199,13 @@ struct PassBBicgUpd {
         const V xa = cvt<C>(o.x) + alpha * ph;
         const V xv = xa + omega * sh;  // combine3 (:217)
         const S rs = cvt<T>(sv - omega * t);
-        stv(x, i, cvt<T>(sel(sexit, xa, xv)));
+        if constexpr (XD) {
+            const V xn = sel(sexit, xa, xv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) stc(xh[k], i, (double)xn[k]);
+        } else {
+            stv(x, i, cvt<T>(sel(sexit, xa, xv)));
+        }
         stv(rnew, i, rs);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -184,13 +215,10 @@ struct PassBBicgUpd {
     }
 };
 
-template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BCtl c, compute_t<T> a,
-                                                                  compute_t<T> b, compute_t<T> cc, long long n,
-                                                                  V4<T>* r0, V4<T>* r1, const V4<T>* v0,
-                                                                  const V4<T>* v1, const V4<T>* p0,
-                                                                  const V4<T>* p1, const V4<T>* rhat, V4<T>* x,
-                                                                  compute_t<T> im) {
+template <class T, bool XD, int ZC>
+__device__ __forceinline__ void bicg_upd_body(TileGrid g, BCtl c, compute_t<T> a, compute_t<T> b, compute_t<T> cc, long long n,
+    V4<T>* r0, V4<T>* r1, const V4<T>* v0, const V4<T>* v1, const V4<T>* p0, const V4<T>* p1,
+    const V4<T>* rhat, V4<T>* x, double* xhi, compute_t<T> im) {
     using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
@@ -200,13 +228,15 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
     const int it = c.iter[rb];
     const int cur = it & 1;
     const long long off = (long long)rb * n;
-    PassBBicgUpd<T> P;
+    PassBBicgUpd<T, XD> P;
     P.a = a; P.b = b; P.c = cc;
     P.r = (cur ? r1 : r0) + off;
     P.v = (cur ? v1 : v0) + off;
     P.p = (cur ? p1 : p0) + off;
     P.rhat = rhat + off;
     P.x = x + off;
+#pragma unroll
+    for (int k = 0; k < 4; k++) P.xh[k] = XD ? xhi + (long long)(4 * rb + k) * n : nullptr;
     P.rnew = (cur ? r0 : r1) + off;
     P.im = im;
 #pragma unroll
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
         P.omega[k] = (d || P.sexit[k]) ? (C)0 : (C)c.omega[kk];
     }
     double acc[8];
-    run_tiles<PassBBicgUpd<T>, ZC>(P, g, lds, acc, cur == 1);
+    run_tiles<PassBBicgUpd<T, XD>, ZC>(P, g, lds, acc, cur == 1);
     if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
@@ -240,8 +270,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
         if (c.mixed) {
             // the FP64 pass decides convergence, maxiter and rho breakdown;
             // the drop is measured from the largest residual since the last
-            // replacement (van der Vorst & Ye 2000: BiCGStab's residual
-            // spikes are what the recursion loses accuracy to)
+            // replacement (BiCGStab's residual spikes are what the recursion
+            // loses accuracy to: van der Vorst & Ye 2000)
             c.elast[k] = fmax(c.elast[k], err);
             c.rep[k] = (err <= c.tolv[k]) || (it % c.rep_every == 0) || (it >= c.maxv[k]) ||
                        (err < c.drop * c.elast[k]) || (c.rho[k] == 0.0);
@@ -253,6 +283,21 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BC
     brb_update(c, rb);
 }
 
+#define SEDI_BICG_UPD_ARGS g, c, a, b, cc, n, r0, r1, v0, v1, p0, p1, rhat, x, xhi, im
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_kernel(TileGrid g, BCtl c, compute_t<T> a, compute_t<T> b, compute_t<T> cc, long long n,
+    V4<T>* r0, V4<T>* r1, const V4<T>* v0, const V4<T>* v1, const V4<T>* p0, const V4<T>* p1,
+    const V4<T>* rhat, V4<T>* x, double* xhi, compute_t<T> im) {
+    bicg_upd_body<T, false, ZC>(SEDI_BICG_UPD_ARGS);
+}
+template <class T, int ZC>
+__global__ __launch_bounds__(NTHREADS) void batch_bicg_upd_xd_kernel(TileGrid g, BCtl c, compute_t<T> a, compute_t<T> b, compute_t<T> cc, long long n,
+    V4<T>* r0, V4<T>* r1, const V4<T>* v0, const V4<T>* v1, const V4<T>* p0, const V4<T>* p1,
+    const V4<T>* rhat, V4<T>* x, double* xhi, compute_t<T> im) {
+    bicg_upd_body<T, true, ZC>(SEDI_BICG_UPD_ARGS);
+}
+#undef SEDI_BICG_UPD_ARGS
+
 // ---- host ------------------------------------------------------------------
 template <class T>
 int bicg_iteration(const BRun& r) {
@@ -261,18 +306,29 @@ int bicg_iteration(const BRun& r) {
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
     const KGeom g1 = bgeom(batch_bicg_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     const KGeom g2 = bgeom(batch_bicg_half_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
-    const KGeom g3 = bgeom(batch_bicg_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    constexpr bool mixed_t = std::is_same<T, float>::value;  // FP32 storage: x in FP64
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_dir_kernel", batch_bicg_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb),
                              r.stream, g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
                              V(BI_P0), V(BI_P1), V(BI_V0), V(BI_V1), (const V4<T>*)V(BI_RH), im));
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_half_kernel", batch_bicg_half_kernel, T, g2, dim3(g2.nblocks, r.nrb),
                              r.stream, g2.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
                              (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), im));
-    SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_kernel, T, g3, dim3(g3.nblocks, r.nrb),
-                             r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1), (const V4<T>*)V(BI_V0),
-                             (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0), (const V4<T>*)V(BI_P1),
-                             (const V4<T>*)V(BI_RH), V(SLOT_XLO), im));
-    if (r.mixed) return replace_and_fold<T>(r, V(BI_R0), V(BI_RH));
+    if constexpr (mixed_t) {
+        const KGeom g3 = bgeom(batch_bicg_upd_xd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+        SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_xd_kernel, T, g3,
+                                 dim3(g3.nblocks, r.nrb), r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1),
+                                 (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0),
+                                 (const V4<T>*)V(BI_P1), (const V4<T>*)V(BI_RH), V(SLOT_XLO), r.xhi, im));
+    } else {
+        const KGeom g3 = bgeom(batch_bicg_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+        SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_kernel, T, g3,
+                                 dim3(g3.nblocks, r.nrb), r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1),
+                                 (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0),
+                                 (const V4<T>*)V(BI_P1), (const V4<T>*)V(BI_RH), V(SLOT_XLO), (double*)nullptr,
+                                 im));
+    }
+    // x is in FP64 (x_lo stays zero): the replacement needs no fold
+    if (r.mixed) return replace_and_fold<T>(r, V(BI_R0), V(BI_RH), false);
     return SEDI_OK;
 }
 

@@ -19,9 +19,7 @@
 #ifndef SEDI_BICG_RESET
 #define SEDI_BICG_RESET 1
 #endif
-#ifndef SEDI_BICG_DROP
-#define SEDI_BICG_DROP 2.44140625e-4  // sqrt(2^-24)
-#endif
+
 
 namespace sedi {
 
@@ -245,9 +243,9 @@ struct BRun {
 template <class T> int pcg_setup(const BRun& r, int rb0, int count);
 template <class T> int pcg_iteration(const BRun& r);
 template <class T> int bicg_iteration(const BRun& r);
-// the FP64 replacement r = b - A x with <r,u> (CG) or <r,r^> (BiCGStab) and
-// the fold x_hi += x_lo that follows it (batched.hip)
-template <class T> int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat);
+// the FP64 replacement r = b - A x with <r,u> (CG) or <r,r^> (BiCGStab) and,
+// when `fold`, the fold x_hi += x_lo that follows it (batched.hip)
+template <class T> int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold);
 template <class T> int fold_only(const BRun& r);
 
 }  // namespace sedi
