@@ -75,7 +75,6 @@ const char* progname = "sedi-estimator";
     fprintf(stderr, "\t-R M : mixed precision: FP64 residual replacement every M iterations (default 50)\n");
     fprintf(stderr, "\t-o FILE : write the diagonal estimate as raw float64\n");
     fprintf(stderr, "\t-X HDF5_FILE : write a validation archive (x_k, v_k of the last sample, diag_inv)\n");
-    fprintf(stderr, "\t-o FILE : write the estimated diagonal as raw float64\n");
     exit(1);
 }
 
