@@ -50,7 +50,7 @@ template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl c, int rb0, double tol,
                                                               int maxit, const V4<T>* b, V4<T>* r,
                                                               V4<T>* rhat, V4<T>* xlo, double* xhi, T im) {
-    __shared__ double red[4 * 8];
+    __shared__ double red[8 * NWAVES];
     __shared__ int lastflag;
     const int rb = rb0 + blockIdx.y;
     const V4<T>* bq = b + (long long)rb * n;
@@ -146,13 +146,13 @@ struct PassBDir {
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_dir_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                             long long n, const V4<T>* r,
-                                                             V4<T>* p0, V4<T>* p1, T im) {
+__global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batch_dir_kernel(
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r, V4<T>* p0, V4<T>* p1, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
-    __shared__ double red[4 * 4];
+    constexpr int TYK = CgTile<T>::ty;
+    __shared__ V4<T> lds[2][TYK + 2][TX + 2];
+    __shared__ double red[4 * TYK];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
@@ -162,8 +162,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_dir_kernel(TileGrid g, BCtl c,
     PassBDir<T> P{a, b, cc, r + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off, im,
                   beta, it == 1};
     double acc[4];
-    run_tiles<PassBDir<T>, ZC>(P, g, lds, acc, false);
-    if (!grid_sum<4>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+    run_tiles<PassBDir<T>, ZC, TYK>(P, g, lds, acc, false);
+    if (!grid_sum<4, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
     for (int q = 0; q < 4; q++) {
@@ -204,14 +204,14 @@ struct PassBUpd {
 };
 
 template <class T, int ZC>
-__global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c, T a, T b, T cc,
-                                                             long long n, const V4<T>* p0,
-                                                             const V4<T>* p1, V4<T>* xlo,
-                                                             V4<T>* r, T im) {
+__global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batch_upd_kernel(
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* p0, const V4<T>* p1, V4<T>* xlo,
+    V4<T>* r, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    __shared__ V4<T> lds[2][TY + 2][TX + 2];
-    __shared__ double red[8 * 4];
+    constexpr int TYK = CgTile<T>::ty;
+    __shared__ V4<T> lds[2][TYK + 2][TX + 2];
+    __shared__ double red[8 * TYK];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
@@ -220,8 +220,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_upd_kernel(TileGrid g, BCtl c,
     for (int q = 0; q < 4; q++) alpha[q] = (T)c.alpha[4 * rb + q];
     PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, xlo + off, r + off, im, alpha};
     double acc[8];
-    run_tiles<PassBUpd<T>, ZC>(P, g, lds, acc, true);
-    if (!grid_sum<8>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
+    run_tiles<PassBUpd<T>, ZC, TYK>(P, g, lds, acc, true);
+    if (!grid_sum<8, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
     for (int q = 0; q < 4; q++) {
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     const int* rp = c.rep + 4 * rb;
     if (!(rp[0] | rp[1] | rp[2] | rp[3])) return;
     __shared__ V4<double> lds[2][TY + 2][TX + 2];
-    __shared__ double red[8 * 4];
+    __shared__ double red[8 * NWAVES];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
     // the residual the next iteration reads: CG has one, BiCGStab alternates
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     int* fp = c.fold + 4 * rb;
     const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
     if (!(f0 | f1 | f2 | f3)) return;
-    __shared__ double red[4];
+    __shared__ double red[NWAVES];
     __shared__ int lastflag;
     const int fl[4] = {f0, f1, f2, f3};
     V4<T>* xq = xlo + (long long)rb * n;
@@ -522,8 +522,8 @@ static int cg_iteration(const BRun& r) {
     V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
     V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
     V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
-    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
-    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
+    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
                              g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, im));
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, r.nrb), r.stream,

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_dir_kernel(TileGrid g, BC
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<C> lds[2][TY + 2][TX + 2];
-    __shared__ double red[4 * 4];
+    __shared__ double red[4 * NWAVES];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const int cur = it & 1;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_bicg_half_kernel(TileGrid g, B
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<C> lds[2][TY + 2][TX + 2];
-    __shared__ double red[12 * 4];
+    __shared__ double red[12 * NWAVES];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const int cur = it & 1;
@@ -223,7 +223,7 @@ __device__ __forceinline__ void bicg_upd_body(TileGrid g, BCtl c, compute_t<T> a
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<C> lds[2][TY + 2][TX + 2];
-    __shared__ double red[8 * 4];
+    __shared__ double red[8 * NWAVES];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const int cur = it & 1;

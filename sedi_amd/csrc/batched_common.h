@@ -175,7 +175,7 @@ inline int batch_cus() {
 // batch size, so the order of every reduction -- and with it each RHS's
 // result -- is the same whichever batch an RHS is solved in.
 template <typename Kern>
-KGeom bgeom(Kern kern16, const Shape& s, int D) {
+KGeom bgeom(Kern kern16, const Shape& s, int D, int ty = TY) {
     static std::mutex mu;
     static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
     const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
@@ -183,7 +183,7 @@ KGeom bgeom(Kern kern16, const Shape& s, int D) {
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, TX * ty, 0) != hipSuccess || occ <= 0) {
         (void)hipGetLastError();
         occ = 2;
     }
@@ -194,11 +194,11 @@ KGeom bgeom(Kern kern16, const Shape& s, int D) {
     const int per = std::max(8, occ * batch_cus() / 8 * 8);
 #ifdef SEDI_BATCH_ZC
     KGeom k;
-    k.tg = make_tile_grid(s, SEDI_BATCH_ZC);
+    k.tg = make_tile_grid(s, SEDI_BATCH_ZC, ty);
     k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
     return cache.emplace(key, k).first->second;
 #else
-    return cache.emplace(key, plan_geometry(s, per, D)).first->second;
+    return cache.emplace(key, plan_geometry(s, per, D, ty)).first->second;
 #endif
 }
 
@@ -209,11 +209,19 @@ inline unsigned ew_blocks(long long n) {
     return (unsigned)std::max<long long>(1, g);
 }
 
-// kernel<TT, ZC> for the ZC the geometry chose
+// kernel<TT, ZC> for the ZC the geometry chose, one wave per tile row
 #define SEDI_BZC_LAUNCH(name, K, TT, geo, grid, ...)                                             \
-    ((geo).tg.zc == 8    ? launch(name, K<TT, 8>, grid, dim3(NTHREADS), __VA_ARGS__)             \
-     : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(NTHREADS), __VA_ARGS__)            \
-                         : launch(name, K<TT, 16>, grid, dim3(NTHREADS), __VA_ARGS__))
+    ((geo).tg.zc == 8    ? launch(name, K<TT, 8>, grid, dim3(TX * (geo).tg.ty), __VA_ARGS__)     \
+     : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(TX * (geo).tg.ty), __VA_ARGS__)    \
+                         : launch(name, K<TT, 16>, grid, dim3(TX * (geo).tg.ty), __VA_ARGS__))
+
+// Tile height of the batched CG passes: tall tiles in FP32 storage, where
+// the registers allow it; FP64 blocks keep 4 rows.
+template <class T>
+struct CgTile {
+    static constexpr int ty = sizeof(T) == 4 ? TY_TALL : TY;
+    static constexpr int min_waves = sizeof(T) == 4 ? 4 : 1;
+};
 
 // Vector slots of a batch (each [K/4][N] of V4<T>).
 enum { SLOT_B = 0, SLOT_XLO = 1, SLOT_K0 = 2 };

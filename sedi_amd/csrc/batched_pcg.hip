@@ -61,7 +61,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_init_kernel(TileGrid g, BC
     const int rb = rb0 + blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<compute_t<T>> lds[2][TY + 2][TX + 2];
-    __shared__ double red[12 * 4];
+    __shared__ double red[12 * NWAVES];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
     PassBPcgInit<T> P{a, b, cc, r + off, u + off, w + off, z + off, q + off, s + off, p + off, im};
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BC
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<C> lds[2][TY + 2][TX + 2];
-    __shared__ double red[12 * 4];
+    __shared__ double red[12 * NWAVES];
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_repb_kernel(TileGrid g, BC
     const int* rp = c.rep + 4 * rb;
     if (!(rp[0] | rp[1] | rp[2] | rp[3])) return;
     __shared__ D2 lds[2][TY + 2][TX + 2];
-    __shared__ double red[12 * 4];
+    __shared__ double red[12 * NWAVES];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
     // w as the next iteration (iter[rb]) reads it

@@ -6,14 +6,15 @@
 // and accumulate dot products.  f is never stored unless the pass writes it:
 // neighbours come from registers (z) and an LDS plane (x, y).
 //
-// Tiling (gfx950, wave64): a workgroup is 4 waves = 256 lanes covering a tile
-// of TX=64 cells along x (one wave row, 512 contiguous bytes per FP64 vector)
-// by TY=4 rows along y, and marches over a chunk of ZC planes along z.  For
-// each plane it evaluates f at its 256 cells plus the 2*64 + 2*4 in-plane
-// halo cells (lanes 0..31 of every wave take half a y-halo row, lanes 32/33
-// the two x-halo cells of their row), stores them in a double-buffered LDS
-// plane and needs one workgroup barrier per plane.  The z neighbours stay in
-// registers.
+// Tiling (gfx950, wave64): a workgroup has one wave per tile row; a tile is
+// TX=64 cells along x (one wave row, 512 contiguous bytes per FP64 vector) by
+// TYK rows along y (4, or 8 for the passes whose registers allow two such
+// workgroups per CU), and marches over a chunk of ZC planes along z.  For
+// each plane it evaluates f at its cells plus the 2*64 + 2*TYK in-plane halo
+// cells (lanes 0..31 of the first four waves take half a y-halo row, lanes
+// 32/33 of every wave the two x-halo cells of its row), stores them in a
+// double-buffered LDS plane and needs one workgroup barrier per plane.  The z
+// neighbours stay in registers.  The y halo costs 2/TYK extra rows per row.
 //
 // Latency: all loads are issued D planes before they are consumed, into a
 // ring of registers whose slots are compile-time constants (the z loop is
@@ -44,23 +45,29 @@
 namespace sedi {
 
 constexpr int TX = 64;
-constexpr int TY = 4;
+constexpr int TY = 4;  // default tile height: rows per tile = waves per workgroup
 constexpr int NTHREADS = TX * TY;
+constexpr int NWAVES = NTHREADS / 64;
+// Tall tiles (8 rows, 8 waves) halve the y-halo re-reads of a pass; kernels
+// whose registers allow it (four waves per SIMD, 128 VGPRs, so that two
+// workgroups fit on a CU) use them.
+constexpr int TY_TALL = 8;
 constexpr int NXCD = 8;
 
 struct TileGrid {
     int n1, n2, n3;
-    int nbx, nby, nbz, zc;
+    int nbx, nby, nbz, zc, ty;
     long long s2, s3;
     long long ntiles;
 };
 
-inline TileGrid make_tile_grid(const Shape& s, int zc) {
+inline TileGrid make_tile_grid(const Shape& s, int zc, int ty = TY) {
     TileGrid t;
     t.n1 = s.n1; t.n2 = s.n2; t.n3 = s.n3;
     t.nbx = (s.n1 + TX - 1) / TX;
-    t.nby = (s.n2 + TY - 1) / TY;
+    t.nby = (s.n2 + ty - 1) / ty;
     t.zc = zc;
+    t.ty = ty;
     t.nbz = (s.n3 + zc - 1) / zc;
     t.s2 = s.n1;
     t.s3 = (long long)s.n1 * s.n2;
@@ -81,8 +88,8 @@ struct KGeom {
 // The march is compiled for chunks of 8, 16 and 32 planes.
 constexpr int ZC_CHOICES[3] = {8, 16, 32};
 
-inline KGeom plan_geometry(const Shape& s, int resident, int D) {
-    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + TY - 1) / TY);
+inline KGeom plan_geometry(const Shape& s, int resident, int D, int ty = TY) {
+    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + ty - 1) / ty);
     long long best_cost = -1;
     int best_zc = 16;
     for (int zc : ZC_CHOICES) {
@@ -95,7 +102,7 @@ inline KGeom plan_geometry(const Shape& s, int resident, int D) {
         }
     }
     KGeom k;
-    k.tg = make_tile_grid(s, best_zc);
+    k.tg = make_tile_grid(s, best_zc, ty);
     k.nblocks = (int)std::min<long long>(k.tg.ntiles, resident);
     return k;
 }
@@ -109,19 +116,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Sums ND per-thread values over the workgroup; result valid in thread 0.
-template <int ND>
-__device__ __forceinline__ void block_sum(double (&acc)[ND], double* red /* LDS [ND][4] */) {
+template <int ND, int NW = NWAVES>
+__device__ __forceinline__ void block_sum(double (&acc)[ND], double* red /* LDS [ND][NW] */) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int d = 0; d < ND; d++) {
         const double s = wave_sum(acc[d]);
-        if (lane == 0) red[d * 4 + w] = s;
+        if (lane == 0) red[d * NW + w] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int d = 0; d < ND; d++)
-            acc[d] = ((red[d * 4 + 0] + red[d * 4 + 1]) + red[d * 4 + 2]) + red[d * 4 + 3];
+        for (int d = 0; d < ND; d++) {
+            double t = red[d * NW];
+#pragma unroll
+            for (int v = 1; v < NW; v++) t += red[d * NW + v];
+            acc[d] = t;
+        }
     }
 }
 
@@ -137,10 +148,10 @@ __device__ __forceinline__ void block_sum(double (&acc)[ND], double* red /* LDS 
 // stores (written through to the coherence point), the lane waits until they
 // have completed (vmcnt 0) before the agent-scope arrival increment, and the
 // last workgroup reads them with agent-scope atomic loads.
-template <int ND>
+template <int ND, int NW = NWAVES>
 __device__ bool grid_sum(double (&acc)[ND], double* partials, unsigned* counter,
-                         double* red /* LDS >= 4*ND doubles */, int* flag /* LDS */) {
-    block_sum<ND>(acc, red);
+                         double* red /* LDS >= ND*NW doubles */, int* flag /* LDS */) {
+    block_sum<ND, NW>(acc, red);
     const int nb = gridDim.x;
     if (threadIdx.x == 0) {
 #ifdef SEDI_GRID_SUM_FENCE
@@ -170,13 +181,13 @@ __device__ bool grid_sum(double (&acc)[ND], double* partials, unsigned* counter,
 #pragma unroll
     for (int d = 0; d < ND; d++) {
         double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += NTHREADS)
+        for (int b = threadIdx.x; b < nb; b += NW * 64)
             s += __hip_atomic_load(partials + (long long)d * nb + b, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         loc[d] = s;
     }
     __syncthreads();  // `red` is reused
-    block_sum<ND>(loc, red);
+    block_sum<ND, NW>(loc, red);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int d = 0; d < ND; d++) acc[d] = loc[d];
@@ -250,10 +261,11 @@ struct value_of<P, std::void_t<typename P::V>> { using type = typename P::V; };
 template <class P>
 using value_t = typename value_of<P>::type;
 
-template <class P, int ZC>
+template <class P, int ZC, int TYK = TY>
 __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long long tile,
-                                           value_t<P> (&lds)[2][TY + 2][TX + 2],
+                                           value_t<P> (&lds)[2][TYK + 2][TX + 2],
                                            double (&acc)[P::ND]) {
+    static_assert(TYK >= 4, "the halo rows are loaded by the first four waves");
     using V = value_t<P>;
     const V zero{};
     constexpr int D = P::D;
@@ -264,7 +276,7 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     const long long rem = tile - (long long)bz * t_xy;
     const int by = (int)(rem / g.nbx);
     const int bx = (int)(rem - (long long)by * g.nbx);
-    const int i0 = bx * TX, j0 = by * TY;
+    const int i0 = bx * TX, j0 = by * TYK;
     const int zs = bz * ZC;
     const int n3 = g.n3;
     const cidx_t s3 = (cidx_t)g.s3;
@@ -279,10 +291,10 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     // the one extra (halo) cell this lane evaluates per plane
     int ex_i = i, ex_j = j, ex_r = 0, ex_c = 0;
     bool ex_act = false;
-    if (lane < 32) {
-        ex_j = (w < 2) ? j0 - 1 : j0 + TY;
+    if (lane < 32 && w < 4) {
+        ex_j = (w < 2) ? j0 - 1 : j0 + TYK;
         ex_i = i0 + (w & 1) * 32 + lane;
-        ex_r = (w < 2) ? 0 : TY + 1;
+        ex_r = (w < 2) ? 0 : TYK + 1;
         ex_c = 1 + (w & 1) * 32 + lane;
         ex_act = true;
     } else if (lane == 32) {
@@ -353,14 +365,14 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
 }
 
 // Grid-stride loop over the tiles of a pass (tile order per tile_of).
-template <class P, int ZC>
+template <class P, int ZC, int TYK = TY>
 __device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
-                                          value_t<P> (&lds)[2][TY + 2][TX + 2],
+                                          value_t<P> (&lds)[2][TYK + 2][TX + 2],
                                           double (&acc)[P::ND], bool reverse = false) {
 #pragma unroll
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
     for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
-        march_tile<P, ZC>(p, g, tile_of(g, t, reverse), lds, acc);
+        march_tile<P, ZC, TYK>(p, g, tile_of(g, t, reverse), lds, acc);
 }
 
 }  // namespace sedi

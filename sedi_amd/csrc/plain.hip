@@ -316,32 +316,37 @@ struct PassBicgUpdate {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// The plain passes use tall tiles (8 rows, 8 waves): their registers fit two
+// such workgroups per CU, and the taller tile halves the y-halo re-reads.
+constexpr int PTY = TY_TALL;
+constexpr int PTHREADS = TX * PTY;
+
 #define PASS_SHARED                                  \
-    __shared__ double lds[2][TY + 2][TX + 2];        \
-    __shared__ double red[16];                       \
+    __shared__ double lds[2][PTY + 2][TX + 2];       \
+    __shared__ double red[4 * PTY];                  \
     __shared__ int lastflag
 
 
 template <int ZC>
-__global__ __launch_bounds__(NTHREADS) void stencil_apply_kernel(TileGrid g, double a, double b,
+__global__ __launch_bounds__(PTHREADS, 4) void stencil_apply_kernel(TileGrid g, double a, double b,
                                                                  double c, const double* x,
                                                                  double* y) {
-    __shared__ double lds[2][TY + 2][TX + 2];
+    __shared__ double lds[2][PTY + 2][TX + 2];
     PassApply P{a, b, c, x, y};
     double acc[1];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc);
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc);
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
+__global__ __launch_bounds__(PTHREADS, 4) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* x,
                                                             const double* rhs, double* r,
                                                             double* rhat, PC iM) {
     PASS_SHARED;
     PassResidual<PC> P{a, b, c, x, rhs, r, rhat, iM};
     double acc[3];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, false);
-    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, false);
+    if (!grid_sum<3, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     double bb = acc[0];
     if (bb == 0.0) bb = 1.0;  // Cg.hpp:115
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(NTHREADS) void residual_kernel(TileGrid g, Ctl ctl,
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
+__global__ __launch_bounds__(PTHREADS, 4) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
                                                                 double b, double c, const double* r,
                                                                 double* p0, double* p1, PC iM) {
     const DevState* cst = ctl.st;
@@ -380,14 +385,14 @@ __global__ __launch_bounds__(NTHREADS) void cg_direction_kernel(TileGrid g, Ctl 
     const int it = cst->iter;
     PassCgDirection<PC> P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
     double acc[1];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, false);
-    if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, false);
+    if (!grid_sum<1, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->alpha = st->ru / acc[0];  // Cg.hpp:166
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
+__global__ __launch_bounds__(PTHREADS, 4) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                              double c, const double* p0,
                                                              const double* p1, double* x, double* r,
                                                              PC iM) {
@@ -397,8 +402,8 @@ __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl
     const int it = cst->iter;
     PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
     double acc[2];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, true);
-    if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, true);
+    if (!grid_sum<2, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double ru_new = acc[0];
     st->beta = ru_new / st->ru;  // Cg.hpp:185
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(NTHREADS) void cg_update_kernel(TileGrid g, Ctl ctl
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl, double a, double b,
+__global__ __launch_bounds__(PTHREADS, 4) void pcg_init_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* r, double* u,
                                                             double* w, double* z, double* q,
                                                             double* s, double* p, PC iM) {
@@ -422,8 +427,8 @@ __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl,
     PASS_SHARED;
     PassPcgInit<PC> P{a, b, c, r, u, w, z, q, s, p, iM};
     double acc[3];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, true);
-    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, true);
+    if (!grid_sum<3, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->gamma = acc[0];  // PipelinedCg.hpp:171-173 at iter 1
     st->delta = acc[1];
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_init_kernel(TileGrid g, Ctl ctl,
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
+__global__ __launch_bounds__(PTHREADS, 4) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, double* w0, double* w1,
                                                             double* x, double* r, double* u,
                                                             double* z, double* q, double* s,
@@ -447,8 +452,8 @@ __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl,
     PassPcgIter<PC> P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
                   cst->alpha, cst->beta};
     double acc[3];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 0);
-    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (it & 1) == 0);
+    if (!grid_sum<3, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double err = st->err_pending;  // rr before this iteration's update (:227)
     st->err = err;
@@ -470,7 +475,7 @@ __global__ __launch_bounds__(NTHREADS) void pcg_iter_kernel(TileGrid g, Ctl ctl,
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
+__global__ __launch_bounds__(PTHREADS, 4) void bicgstab_direction_kernel(
     TileGrid g, Ctl ctl, double a, double b, double c, double* const* vv, PC iM) {
     // vv: r0, r1, rhat, p0, p1, v0, v1
     const DevState* cst = ctl.st;
@@ -481,14 +486,14 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_direction_kernel(
     PassBicgDirection<PC> P{a, b, c, vv[0 + cur], vv[3 + prv], vv[5 + prv], vv[2],
                         vv[3 + cur], vv[5 + cur], iM, cst->beta, cst->cbw, it == 1};
     double acc[1];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 1);
-    if (!grid_sum<1>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (it & 1) == 1);
+    if (!grid_sum<1, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->alpha = st->rho / acc[0];  // BiCgStab.hpp:187
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl ctl, double a,
+__global__ __launch_bounds__(PTHREADS, 4) void bicgstab_half_kernel(TileGrid g, Ctl ctl, double a,
                                                                  double b, double c,
                                                                  double* const* vv, PC iM) {
     const DevState* cst = ctl.st;
@@ -497,8 +502,8 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl
     const int cur = cst->iter & 1;
     PassBicgHalf<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], iM, cst->alpha};
     double acc[3];
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, (cst->iter & 1) == 0);
-    if (!grid_sum<3>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (cst->iter & 1) == 0);
+    if (!grid_sum<3, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     st->ss = acc[0];                               // :193-194
     if (acc[0] < st->tol * st->tol) st->sexit = 1;  // :195 (absolute test)
@@ -506,7 +511,7 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_half_kernel(TileGrid g, Ctl
 }
 
 template <class PC, int ZC>
-__global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, Ctl ctl, double a,
+__global__ __launch_bounds__(PTHREADS, 4) void bicgstab_update_kernel(TileGrid g, Ctl ctl, double a,
                                                                    double b, double c,
                                                                    double* const* vv, double* x,
                                                                    PC iM) {
@@ -521,21 +526,21 @@ __global__ __launch_bounds__(NTHREADS) void bicgstab_update_kernel(TileGrid g, C
         const double alpha = cst->alpha;
         const double* p = vv[3 + cur];
         const long long n = (long long)g.n1 * g.n2 * g.n3;
-        for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
-             i += (long long)gridDim.x * NTHREADS) {
+        for (long long i = (long long)blockIdx.x * PTHREADS + threadIdx.x; i < n;
+             i += (long long)gridDim.x * PTHREADS) {
             const cidx_t c = (cidx_t)i;
             const double ph = ldc(p, c) * iM.at(c);
             stc(x, c, 1.0 * ldc(x, c) + alpha * ph);
         }
         acc[0] = acc[1] = 0.0;
-        if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+        if (!grid_sum<2, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
         bicgstab_outcome(ctl.st, it);
         return;
     }
     PassBicgUpdate<PC> P{a, b, c, vv[0 + cur], vv[5 + cur], vv[3 + cur], vv[2],
                      x, vv[0 + (cur ^ 1)], iM, cst->alpha, cst->omega};
-    run_tiles<decltype(P), ZC>(P, g, lds, acc, (it & 1) == 1);
-    if (!grid_sum<2>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (it & 1) == 1);
+    if (!grid_sum<2, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
     const double err = sqrt(acc[0] / st->bb);  // :223-227
     st->err = err;
@@ -582,23 +587,23 @@ static const KGeom& geom_for(K kern16, const Shape& s, int D) {
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, NTHREADS, 0) != hipSuccess || occ <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern16, PTHREADS, 0) != hipSuccess || occ <= 0) {
         (void)hipGetLastError();
         occ = 4;
     }
     occ = std::min(occ, MAX_BLOCKS_PER_CU);
-    return cache.emplace(key, plan_geometry(s, occ * num_cus(), D)).first->second;
+    return cache.emplace(key, plan_geometry(s, occ * num_cus(), D, PTY)).first->second;
 }
 
 // kernel<..., ZC> for the ZC the geometry chose
 #define SEDI_ZC_LAUNCH(name, K, geo, ...)                                                          \
-    ((geo).tg.zc == 8    ? launch(name, K<8>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)    \
-     : (geo).tg.zc == 32 ? launch(name, K<32>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)   \
-                         : launch(name, K<16>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__))
+    ((geo).tg.zc == 8    ? launch(name, K<8>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__)    \
+     : (geo).tg.zc == 32 ? launch(name, K<32>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__)   \
+                         : launch(name, K<16>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__))
 #define SEDI_ZC_LAUNCH_PC(name, K, PCT, geo, ...)                                                   \
-    ((geo).tg.zc == 8    ? launch(name, K<PCT, 8>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__) \
-     : (geo).tg.zc == 32 ? launch(name, K<PCT, 32>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__)\
-                         : launch(name, K<PCT, 16>, dim3((geo).nblocks), dim3(NTHREADS), __VA_ARGS__))
+    ((geo).tg.zc == 8    ? launch(name, K<PCT, 8>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__) \
+     : (geo).tg.zc == 32 ? launch(name, K<PCT, 32>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__)\
+                         : launch(name, K<PCT, 16>, dim3((geo).nblocks), dim3(PTHREADS), __VA_ARGS__))
 
 int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
                       hipStream_t stream) {
