@@ -88,6 +88,7 @@ def parse():
                    help="headline: skip the plain FP64 comparison run")
     p.add_argument("--cpu-iters", type=int, default=300,
                    help="iterations of the bounded CPU-baseline sample")
+    p.add_argument("--stencil-side", type=int, default=0, help="--stencil: grid side (default 256)")
     p.add_argument("--stencil", action="store_true",
                    help="time the standalone 7-point stencil kernel (y = A x, FP64) at 256^3 and exit")
     p.add_argument("--pmc-workload", action="store_true",
@@ -98,10 +99,12 @@ def parse():
 
 def pmc_workload():
     """One probe per plain solver and one mixed batch of 64 CG probes, capped
-    at 10 iterations (-m 10 -i), on the 256^3 grid: a few dozen dispatches of
-    every kernel, including the estimator's streaming accumulate_kernel (reads
-    x, v, t, q; writes t, q), whose known traffic calibrates the counters."""
+    at 10 iterations (-m 10 -i), and 5 standalone stencil applies, on the 256^3
+    grid: a few dozen dispatches of every kernel, including the estimator's
+    streaming accumulate_kernel (reads x, v, t, q; writes t, q), whose known
+    traffic calibrates the counters."""
     import sedi_amd
+    stencil_only(reps=5)
     for kind in (2, 3, 1):
         est = sedi_amd.Estimator(GRID, kind=kind, samples=1, maxiter=10, tol=1e-9, seed=7,
                                  ignore_errors=True)
@@ -114,25 +117,41 @@ def pmc_workload():
     print("pmc workload done", flush=True)
 
 
-def stencil_only(reps=20):
-    """y = A x at 256^3 through sedi_stencil_apply with device-resident x, y
-    (allocated by the library's own staging on the first call), kernel time
-    from HIP events; algorithmic bytes 2*N*8 per launch (read x, write y)."""
+def stencil_only(reps=20, side=None):
+    """y = A x at 256^3 (or side^3) through sedi_stencil_apply with
+    device-resident x, y (torch allocations), kernel time from HIP events
+    around each launch; algorithmic bytes 2*N*8 per launch (read x, write y).
+    At 256^3 x fits the 256 MB Infinity Cache between launches; 512^3
+    (1 GiB vectors) streams from HBM."""
+    import ctypes
     import numpy as np
+    import torch
     import sedi_amd
-    n = GRID[0] * GRID[1] * GRID[2]
-    x = np.random.default_rng(0).standard_normal(n)
-    st = sedi_amd.Stencil()
-    st.apply(GRID, x)  # warm-up
+    grid = (side,) * 3 if side else GRID
+    n = grid[0] * grid[1] * grid[2]
+    x = torch.from_numpy(np.random.default_rng(0).standard_normal(n)).to("cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    g, _ = sedi_amd._grid(grid)
+    st = sedi_amd.Stencil()._c()
+    L = sedi_amd.lib()
+
+    def apply():
+        rc = L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), ctypes.c_void_p(x.data_ptr()),
+                                  ctypes.c_void_p(y.data_ptr()))
+        if rc < 0:
+            raise RuntimeError(L.sedi_last_error().decode())
+
+    apply()  # warm-up
     sedi_amd.profile_reset()
     sedi_amd.profile_enable("stencil_apply_kernel")
     for _ in range(reps):
-        st.apply(GRID, x)
+        apply()
     sedi_amd.profile_enable(None)
     launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
     avg = ms / 1e3 / launches
     gbs = 2 * n * 8 / avg / 1e9
-    return {"kernel": "stencil_apply_kernel", "grid": list(GRID), "launches": launches,
+    return {"kernel": "stencil_apply_kernel", "grid": list(grid), "launches": launches,
             "avg_launch_us": round(avg * 1e6, 2), "achieved_GBs": round(gbs, 1),
             "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
 
@@ -511,7 +530,7 @@ def main():
         pmc_workload()
         return
     if args.stencil:
-        print(json.dumps(stencil_only()), flush=True)
+        print(json.dumps(stencil_only(side=args.stencil_side or None)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "

@@ -235,6 +235,11 @@ template <class T>
 __device__ __forceinline__ void stc(T* base, cidx_t i, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(T))) = v;
 }
+// Non-temporal store: for outputs that no later kernel of the same sweep reads.
+__device__ __forceinline__ void stc_nt(double* base, cidx_t i, double v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<double*>(reinterpret_cast<char*>(base) +
+                                                             (unsigned)(i * (unsigned)sizeof(double))));
+}
 
 // ---- the march -------------------------------------------------------------
 // Pass interface (offsets are cell indices; the field f has type P::V, double
