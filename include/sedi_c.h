@@ -135,7 +135,10 @@ typedef struct sedi_estimate_args {
     int batch_k;            /* probes solved together (1 = plain, one at a time);
                                above 1 (or with SEDI_FP32_MIXED): the batched
                                solvers, probes in blocks of 4 (padded with
-                               zero right-hand sides)                         */
+                               zero right-hand sides); 0 = the batch size the
+                               autotuner measured best for this solver,
+                               precision and grid (tuning.txt beside the
+                               library), within 80% of free device memory  */
     int rep_every;          /* SEDI_FP32_MIXED: CG's FP64 residual
                                replacement period (0 = 50); see the batched
                                solvers below for the other triggers           */
@@ -302,6 +305,15 @@ int sedi_profile_reset(void);
  * timed with sedi_profile_enable are never captured.  No reference
  * counterpart (the reference's iterations are host loops). */
 int sedi_graphs_enable(int on);
+
+/* The launch parameters an offline autotuner measured (python -m
+ * sedi_amd.autotune writes tuning.txt beside the library; SEDI_TUNING_FILE
+ * names another table, empty for none): the value stored for `key` and the
+ * grid, or `fallback`.  Keys "zc:<kernel>" (planes per tile of a solver
+ * kernel) and "k:<solver>:<precision>" (probes per batch; nearest grid by
+ * cell count).  No reference counterpart (its tiling is Platform.hpp's
+ * compile-time constants). */
+int sedi_tuning_lookup(const char *key, const sedi_grid *grid, int fallback);
 
 #ifdef __cplusplus
 }

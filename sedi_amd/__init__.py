@@ -499,3 +499,11 @@ def profile_get(kernel_name):
 
 def profile_reset():
     _check(lib().sedi_profile_reset())
+
+
+def tuning_lookup(key, grid, fallback=0):
+    """The autotuner's value for `key` on `grid` (see sedi_tuning_lookup)."""
+    g, _ = _grid(grid)
+    L = lib()
+    L.sedi_tuning_lookup.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Grid), ctypes.c_int]
+    return L.sedi_tuning_lookup(key.encode(), ctypes.byref(g), int(fallback))

@@ -1,0 +1,202 @@
+"""Offline autotuner: measures, on the GPU it runs on, the z chunk (planes per
+tile, i.e. the unroll of the march) of every solver kernel and the batch size
+k of the estimator for a set of grids, and writes the table the library reads
+at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
+
+    python -m sedi_amd.autotune [--sides 64,128,256,512] [--iters 40] [--k-iters 200] [--out sedi_amd/tuning.txt]
+
+1. z chunk: for each of 8, 16 and 32 planes, a child process runs every solver
+   (plain FP64 and batched mixed, CG / pipelined CG / BiCGStab) for a fixed
+   number of iterations on each grid with that chunk forced on every kernel
+   (``zc:* 0 0 0 Z``), timing each kernel with HIP events; each kernel then
+   gets the chunk of its least mean launch time on each grid.
+2. batch size: with those chunks, the batched estimator runs k = 8..64 probes
+   (as memory allows) for a fixed number of iterations; each (solver, precision,
+   grid) gets the k of the highest probe-iterations per second.  In FP64 the
+   plain solver (k = 1) competes too.
+
+The table is data: a run's results depend only on it and the grid, so the
+same table gives the same results on every run.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(PKG) not in sys.path:
+    sys.path.insert(0, os.path.dirname(PKG))
+
+KINDS = {2: "CG", 3: "PipelinedCG", 1: "BiCGStab"}
+# kernels whose launches the chunk decides (the replacement and set-up passes
+# run rarely and follow the occupancy plan)
+PLAIN_KERNELS = {2: ["cg_direction_kernel", "cg_update_kernel"], 3: ["pcg_iter_kernel"],
+                 1: ["bicgstab_direction_kernel", "bicgstab_half_kernel", "bicgstab_update_kernel"]}
+BATCH_KERNELS = {2: ["batch_dir_kernel", "batch_upd_kernel"], 3: ["batch_pcg_iter_kernel"],
+                 1: ["batch_bicg_dir_kernel", "batch_bicg_half_kernel", "batch_bicg_upd_kernel"]}
+# device bytes per right-hand side and cell of a batch (FP32 storage + FP64
+# solution for the mixed form), for sizing k to the memory
+BYTES_MIXED = {2: 28, 3: 48, 1: 44}
+BYTES_FP64 = {2: 40, 3: 80, 1: 72}
+K_CHOICES = (8, 16, 32, 64)
+
+
+def device_bytes():
+    try:
+        import torch
+        free, total = torch.cuda.mem_get_info()
+        return float(free)
+    except Exception:
+        return 250e9
+
+
+def k_max(side, kind, mixed, free):
+    n = side ** 3
+    per = (BYTES_MIXED if mixed else BYTES_FP64)[kind] * n
+    if not mixed and n * 32 >= 2 ** 32:  # an FP64 4-RHS block must stay below 4 GiB
+        return 0
+    return int(0.6 * free / per) // 4 * 4
+
+
+def run_est(side, kind, samples, iters, precision, batch_k):
+    import sedi_amd
+    est = sedi_amd.Estimator((side,) * 3, kind=kind, samples=samples, maxiter=iters, tol=1e-30, seed=1,
+                             ignore_errors=True, precision=precision, batch_k=batch_k)
+    try:
+        r = est.run(want_D=False)
+    finally:
+        est.close()
+    return r["seconds"], float(r["iters"].sum())
+
+
+def measure_chunks(sides, iters):
+    """Child: per grid, the mean launch time of every tuned kernel."""
+    import sedi_amd
+    free = device_bytes()
+    out = {}
+    for side in sides:
+        res = {}
+        for kind in KINDS:
+            for mixed in (False, True):
+                names = (BATCH_KERNELS if mixed else PLAIN_KERNELS)[kind]
+                k = min(64, k_max(side, kind, True, free)) if mixed else 1
+                if mixed and k < 4:
+                    continue
+                prec = sedi_amd.FP32_MIXED if mixed else sedi_amd.FP64
+                run_est(side, kind, k, 2, prec, k)  # first launches load the code objects
+                sedi_amd.profile_reset()
+                sedi_amd.profile_enable("*")
+                run_est(side, kind, k, iters, prec, k)
+                sedi_amd.profile_enable(None)
+                for name in names:
+                    try:
+                        launches, ms = sedi_amd.profile_get(name)
+                        res[name] = 1e3 * ms / launches
+                    except sedi_amd.SediError:
+                        pass
+        out[side] = res
+        print("  side %d: %s" % (side, json.dumps({k: round(v, 1) for k, v in res.items()})), file=sys.stderr,
+              flush=True)
+    return out
+
+
+def measure_batches(sides, iters):
+    """Child: per (solver, precision, grid), probe-iterations/s for each k."""
+    import sedi_amd
+    free = device_bytes()
+    out = []
+    for side in sides:
+        for kind in KINDS:
+            for mixed in (True, False):
+                prec = sedi_amd.FP32_MIXED if mixed else sedi_amd.FP64
+                rates = {}
+                cands = ([1] if not mixed else []) + [k for k in K_CHOICES if k <= k_max(side, kind, mixed, free)]
+                for k in cands:
+                    # untimed first run (code objects, graphs), then the best of two
+                    run_est(side, kind, max(k, 4), 2, prec, k)
+                    best = 0.0
+                    for _ in range(2):
+                        secs, its = run_est(side, kind, max(k, 4), iters, prec, k)
+                        best = max(best, its / secs)
+                    rates[k] = best
+                if rates:
+                    best = max(rates, key=rates.get)
+                    out.append(dict(side=side, kind=kind, precision=prec, k=best,
+                                    rates={str(k): round(v, 1) for k, v in rates.items()}))
+                    print("  side %d %s %s: k=%d %s" % (side, KINDS[kind], "mixed" if mixed else "fp64", best,
+                                                         out[-1]["rates"]), file=sys.stderr, flush=True)
+    return out
+
+
+def child(args, table_lines, mode):
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+        f.write("\n".join(table_lines) + "\n")
+        path = f.name
+    env = dict(os.environ, SEDI_TUNING_FILE=path)
+    iters = args.iters if mode == "chunks" else args.k_iters
+    cmd = [sys.executable, "-m", "sedi_amd.autotune", "--child", mode, "--sides",
+           ",".join(str(s) for s in args.sides), "--iters", str(iters)]
+    try:
+        res = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, cwd=os.path.dirname(PKG), check=True)
+    finally:
+        os.unlink(path)
+    return json.loads(res.stdout.decode().strip().splitlines()[-1])
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    p.add_argument("--sides", default="64,128,256,512", type=lambda s: [int(v) for v in s.split(",")])
+    p.add_argument("--iters", type=int, default=40, help="iterations per kernel timing")
+    p.add_argument("--k-iters", type=int, default=200,
+                   help="iterations per batch-size timing (long enough that probe generation, which the "
+                        "plain FP64 path does one probe at a time, is a small part)")
+    p.add_argument("--out", default=os.path.join(PKG, "tuning.txt"))
+    p.add_argument("--child", choices=["chunks", "batches"], help=argparse.SUPPRESS)
+    args = p.parse_args()
+    if args.child == "chunks":
+        print(json.dumps(measure_chunks(args.sides, args.iters)))
+        return
+    if args.child == "batches":
+        print(json.dumps(measure_batches(args.sides, args.iters)))
+        return
+
+    t0 = time.time()
+    timings = {}
+    for zc in (8, 16, 32):
+        print("z chunk %d" % zc, file=sys.stderr, flush=True)
+        timings[zc] = child(args, ["zc:* 0 0 0 %d" % zc], "chunks")
+    lines = []
+    for side in args.sides:
+        names = sorted(set().union(*(timings[z].get(str(side), {}).keys() for z in timings)))
+        for name in names:
+            t = {z: timings[z][str(side)][name] for z in timings if name in timings[z].get(str(side), {})}
+            best = min(t, key=t.get)
+            lines.append("zc:%s %d %d %d %d   # us per launch: %s" % (
+                name, side, side, side, best, " ".join("%d:%.1f" % (z, t[z]) for z in sorted(t))))
+    print("batch sizes", file=sys.stderr, flush=True)
+    batches = child(args, [l.split("#")[0] for l in lines], "batches")
+    for b in batches:
+        s = b["side"]
+        lines.append("k:%d:%d %d %d %d %d   # probe-iterations/s: %s" % (
+            b["kind"], b["precision"], s, s, s, b["k"], " ".join("%s:%s" % kv for kv in b["rates"].items())))
+    dev = "unknown device"
+    try:
+        import torch
+        dev = torch.cuda.get_device_name(0)
+    except Exception:
+        pass
+    head = ["# sedi_amd tuning table, written by: python -m sedi_amd.autotune --sides %s --iters %d --k-iters %d" % (
+                ",".join(str(s) for s in args.sides), args.iters, args.k_iters),
+            "# device: %s; %s; %.0f s" % (dev, time.strftime("%Y-%m-%d"), time.time() - t0),
+            "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, k:<solver>:<precision> = probes",
+            "# per batch (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
+    with open(args.out, "w") as f:
+        f.write("\n".join(head + lines) + "\n")
+    print("wrote %s (%d entries)" % (args.out, len(lines)), file=sys.stderr, flush=True)
+
+
+if __name__ == "__main__":
+    main()

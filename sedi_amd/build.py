@@ -29,7 +29,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
 # Packed-FP32 instructions are switched off for every device compile.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "h5io.cpp", "plain.hip", "rng.hip", "batched.hip",
+LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "h5io.cpp", "tuning.cpp", "plain.hip", "rng.hip", "batched.hip",
                "batched_pcg.hip", "batched_bicg.hip", "estimator.hip", "serve.hip"]
 DRIVER_SOURCE = os.path.join(CSRC, "driver", "sedi_estimator.cpp")
 

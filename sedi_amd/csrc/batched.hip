@@ -478,7 +478,7 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // ---------------------------------------------------------------------------
 template <class T, bool RHAT>
 static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
-    const KGeom g = bgeom(batch_rep_kernel<T, RHAT, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g = bgeom("batch_rep_kernel", batch_rep_kernel<T, RHAT, 16>, r.shape, SEDI_PREFETCH_BATCH);
     const dim3 grid(g.nblocks, r.nrb);
     const T im = (T)r.im;
     const double* xhi = r.xhi;
@@ -522,8 +522,8 @@ static int cg_iteration(const BRun& r) {
     V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
     V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
     V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
-    const KGeom g1 = bgeom(batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
-    const KGeom g2 = bgeom(batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
+    const KGeom g1 = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
+    const KGeom g2 = bgeom("batch_upd_kernel", batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
                              g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, im));
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, r.nrb), r.stream,

@@ -304,8 +304,8 @@ int bicg_iteration(const BRun& r) {
     using C = compute_t<T>;
     const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
-    const KGeom g1 = bgeom(batch_bicg_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
-    const KGeom g2 = bgeom(batch_bicg_half_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g1 = bgeom("batch_bicg_dir_kernel", batch_bicg_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g2 = bgeom("batch_bicg_half_kernel", batch_bicg_half_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     constexpr bool mixed_t = std::is_same<T, float>::value;  // FP32 storage: x in FP64
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_dir_kernel", batch_bicg_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb),
                              r.stream, g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
@@ -314,13 +314,13 @@ int bicg_iteration(const BRun& r) {
                              r.stream, g2.tg, r.c, a, b, cc, r.n, (const V4<T>*)V(BI_R0), (const V4<T>*)V(BI_R1),
                              (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), im));
     if constexpr (mixed_t) {
-        const KGeom g3 = bgeom(batch_bicg_upd_xd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+        const KGeom g3 = bgeom("batch_bicg_upd_kernel", batch_bicg_upd_xd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
         SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_xd_kernel, T, g3,
                                  dim3(g3.nblocks, r.nrb), r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1),
                                  (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0),
                                  (const V4<T>*)V(BI_P1), (const V4<T>*)V(BI_RH), V(SLOT_XLO), r.xhi, im));
     } else {
-        const KGeom g3 = bgeom(batch_bicg_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+        const KGeom g3 = bgeom("batch_bicg_upd_kernel", batch_bicg_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
         SEDI_HIP(SEDI_BZC_LAUNCH("batch_bicg_upd_kernel", batch_bicg_upd_kernel, T, g3,
                                  dim3(g3.nblocks, r.nrb), r.stream, g3.tg, r.c, a, b, cc, r.n, V(BI_R0), V(BI_R1),
                                  (const V4<T>*)V(BI_V0), (const V4<T>*)V(BI_V1), (const V4<T>*)V(BI_P0),

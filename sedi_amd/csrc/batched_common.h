@@ -12,6 +12,7 @@
 #include "batched.h"
 #include "march.h"
 #include "profiler.h"
+#include "tuning.h"
 
 #ifndef SEDI_PREFETCH_BATCH
 #define SEDI_PREFETCH_BATCH 1
@@ -175,7 +176,7 @@ inline int batch_cus() {
 // batch size, so the order of every reduction -- and with it each RHS's
 // result -- is the same whichever batch an RHS is solved in.
 template <typename Kern>
-KGeom bgeom(Kern kern16, const Shape& s, int D, int ty = TY) {
+KGeom bgeom(const char* name, Kern kern16, const Shape& s, int D, int ty = TY) {
     static std::mutex mu;
     static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
     const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
@@ -192,14 +193,15 @@ KGeom bgeom(Kern kern16, const Shape& s, int D, int ty = TY) {
 #endif
     occ = std::min(occ, 8);
     const int per = std::max(8, occ * batch_cus() / 8 * 8);
-#ifdef SEDI_BATCH_ZC
-    KGeom k;
-    k.tg = make_tile_grid(s, SEDI_BATCH_ZC, ty);
-    k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
-    return cache.emplace(key, k).first->second;
-#else
+    // the autotuner's z chunk for this kernel and grid, else the occupancy plan
+    const int zc = tuned_value(std::string("zc:") + name, s, 0);
+    if (zc == 8 || zc == 16 || zc == 32) {
+        KGeom k;
+        k.tg = make_tile_grid(s, zc, ty);
+        k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
+        return cache.emplace(key, k).first->second;
+    }
     return cache.emplace(key, plan_geometry(s, per, D, ty)).first->second;
-#endif
 }
 
 // Workgroups per RHS block of the elementwise kernels (independent of the

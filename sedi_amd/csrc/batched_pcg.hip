@@ -350,7 +350,7 @@ int pcg_setup(const BRun& r, int rb0, int count) {
     using C = compute_t<T>;
     const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
-    const KGeom g = bgeom(batch_pcg_init_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g = bgeom("batch_pcg_init_kernel", batch_pcg_init_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_init_kernel", batch_pcg_init_kernel, T, g, dim3(g.nblocks, count),
                              r.stream, g.tg, r.c, rb0, a, b, cc, r.n, (const V4<T>*)V(PCG_R), V(PCG_U),
                              V(PCG_W0), V(PCG_Z), V(PCG_Q), V(PCG_S), V(PCG_P), im));
@@ -362,17 +362,17 @@ int pcg_iteration(const BRun& r) {
     using C = compute_t<T>;
     const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
-    const KGeom g = bgeom(batch_pcg_iter_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom g = bgeom("batch_pcg_iter_kernel", batch_pcg_iter_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_iter_kernel", batch_pcg_iter_kernel, T, g, dim3(g.nblocks, r.nrb),
                              r.stream, g.tg, r.c, a, b, cc, r.n, V(PCG_W0), V(PCG_W1), V(SLOT_XLO), V(PCG_R),
                              V(PCG_U), V(PCG_Z), V(PCG_Q), V(PCG_S), V(PCG_P), im));
     if (!r.mixed) return SEDI_OK;
-    const KGeom ga = bgeom(batch_pcg_repa_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom ga = bgeom("batch_pcg_repa_kernel", batch_pcg_repa_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_repa_kernel", batch_pcg_repa_kernel, T, ga, dim3(ga.nblocks, r.nrb),
                              r.stream, ga.tg, r.c, r.a, r.b, r.cc, r.n, (const double*)r.xhi,
                              (const V4<T>*)V(SLOT_XLO), (const V4<T>*)V(PCG_P), (const V4<T>*)V(SLOT_B),
                              V(PCG_R), V(PCG_S), V(PCG_U), V(PCG_Q), r.im));
-    const KGeom gb = bgeom(batch_pcg_repb_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    const KGeom gb = bgeom("batch_pcg_repb_kernel", batch_pcg_repb_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_repb_kernel", batch_pcg_repb_kernel, T, gb, dim3(gb.nblocks, r.nrb),
                              r.stream, gb.tg, r.c, r.a, r.b, r.cc, r.n, (const V4<T>*)V(PCG_U),
                              (const V4<T>*)V(PCG_Q), (const V4<T>*)V(PCG_R), V(PCG_W0), V(PCG_W1), V(PCG_Z)));

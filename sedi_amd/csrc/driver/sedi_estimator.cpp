@@ -71,7 +71,7 @@ const char* progname = "sedi-estimator";
     fprintf(stderr, "\t-S SOLVER : 1 BiCGStab, 2 CG (default), 3 PipelinedCg\n");
     fprintf(stderr, "\t-r SEED : seed of the mt19937 probe stream (default 1234)\n");
     fprintf(stderr, "\t-P PREC : fp64 (default) or mixed (FP32 vectors, FP64 reductions)\n");
-    fprintf(stderr, "\t-k K : probes solved together (default 1)\n");
+    fprintf(stderr, "\t-k K : probes solved together (default 1; 0 = the autotuned batch for this grid)\n");
     fprintf(stderr, "\t-R M : mixed precision: FP64 residual replacement every M iterations (default 50)\n");
     fprintf(stderr, "\t-o FILE : write the diagonal estimate as raw float64\n");
     fprintf(stderr, "\t-X HDF5_FILE : write a validation archive (x_k, v_k of the last sample, diag_inv)\n");
@@ -101,7 +101,7 @@ Options parse(int argc, char** argv) {
             else if (!strcmp(optarg, "mixed")) o.precision = SEDI_FP32_MIXED;
             else usage();
             break;
-        case 'k': if (sscanf(optarg, "%d", &o.batch) != 1 || o.batch < 1) usage(); break;
+        case 'k': if (sscanf(optarg, "%d", &o.batch) != 1 || o.batch < 0) usage(); break;
         case 'R': if (sscanf(optarg, "%d", &o.rep_every) != 1 || o.rep_every < 0) usage(); break;
         case 'o': o.out = optarg; break;
         case 'X': o.xfile = optarg; break;

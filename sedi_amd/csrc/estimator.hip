@@ -5,6 +5,7 @@
 // (rng.hip) runs on its own HIP stream one sample ahead of the solver, into a
 // double buffer guarded by events, so probe generation overlaps the solves.
 #include "estimator.h"
+#include "tuning.h"
 #include "profiler.h"
 
 #include <algorithm>
@@ -70,15 +71,33 @@ int Estimator::create(const sedi_estimate_args& a) {
     SEDI_REQUIRE(a.precision == SEDI_FP64 || a.precision == SEDI_FP32_MIXED, SEDI_ERR_ARG,
                  "unknown precision %d", a.precision);
     SEDI_REQUIRE(a.rep_every >= 0, SEDI_ERR_ARG, "rep_every must be >= 0");
+    SEDI_REQUIRE(a.batch_k >= 0, SEDI_ERR_ARG, "batch_k must be >= 0");
     args_ = a;
     shape_ = Shape{(int)a.grid.n1, (int)a.grid.n2, (int)a.grid.n3};
     const size_t n = (size_t)shape_.n();
-    batched_ = a.batch_k > 1 || a.precision == SEDI_FP32_MIXED;
+    const bool fp32 = a.precision == SEDI_FP32_MIXED;
+    int batch_k = a.batch_k;
+    if (batch_k == 0) {
+        // chosen by the autotuner for this solver, precision and grid (the
+        // entry of nearest cell count); without a table: 64 in the mixed
+        // form, one at a time in FP64 (where the plain solver is as fast)
+        char key[32];
+        std::snprintf(key, sizeof key, "k:%d:%d", a.kind, a.precision);
+        batch_k = tuned_value(key, shape_, fp32 ? 64 : 1, true);
+        // no more than a share of the free device memory holds
+        size_t free_b = 0, total_b = 0;
+        if (batch_k > 4 && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const double per = (double)BatchSolver::bytes_needed(shape_, a.kind, 4, fp32) / 4;
+            const int fit = (int)(0.8 * (double)free_b / per) / 4 * 4;
+            batch_k = std::max(4, std::min(batch_k, fit));
+        }
+    }
+    batched_ = batch_k > 1 || fp32;
     if (batched_) {
         SEDI_REQUIRE(a.kind == SEDI_CG || a.kind == SEDI_PCG || a.kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
                      "unknown solver kind %d", a.kind);
         // blocks of 4 probes, no more than the run needs
-        int K = std::max(4, (a.batch_k + 3) / 4 * 4);
+        int K = std::max(4, (batch_k + 3) / 4 * 4);
         K = std::min(K, std::max(4, (a.samples + 3) / 4 * 4));
         batch_.reset(new BatchSolver());
         int rc = batch_->create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, a.kind, K,

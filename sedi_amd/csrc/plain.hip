@@ -15,6 +15,7 @@
 //   BiCGStab  3 passes: [r,p,v,r^ -> p,v] + [r,v] + [x,p,r,v,r^ -> x,r] = 15 N w
 #include "plain.h"
 #include "profiler.h"
+#include "tuning.h"
 
 #include <cmath>
 #include <cstring>
@@ -584,7 +585,7 @@ constexpr int MAX_BLOCKS_PER_CU = 8;
 // Geometry of a pass on `s`, from the occupancy of its ZC=16 instantiation
 // (the ZC variants differ only in unrolling); cached per kernel and shape.
 template <typename K>
-static const KGeom& geom_for(K kern16, const Shape& s, int D, int ty = PTY) {
+static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, int ty = PTY) {
     static std::mutex mu;
     static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
     const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
@@ -597,6 +598,14 @@ static const KGeom& geom_for(K kern16, const Shape& s, int D, int ty = PTY) {
         occ = 4;
     }
     occ = std::min(occ, MAX_BLOCKS_PER_CU);
+    // the autotuner's z chunk for this kernel and grid, else the occupancy plan
+    const int zc = tuned_value(std::string("zc:") + name, s, 0);
+    if (zc == 8 || zc == 16 || zc == 32) {
+        KGeom k;
+        k.tg = make_tile_grid(s, zc, ty);
+        k.nblocks = (int)std::min<long long>(k.tg.ntiles, (long long)occ * num_cus());
+        return cache.emplace(key, k).first->second;
+    }
     return cache.emplace(key, plan_geometry(s, occ * num_cus(), D, ty)).first->second;
 }
 
@@ -612,7 +621,7 @@ static const KGeom& geom_for(K kern16, const Shape& s, int D, int ty = PTY) {
 
 int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
                       hipStream_t stream) {
-    const KGeom& k = geom_for(stencil_apply_kernel<16>, s, PassApply::D, STY);
+    const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY);
     SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel, k, stream, k.tg, a, b, c, x, y));
     return SEDI_OK;
 }
@@ -662,7 +671,7 @@ int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
     DevState* st = reinterpret_cast<DevState*>(ctl_.p);
     Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const KGeom& kr = geom_for(residual_kernel<PC, 16>, shape_, PassResidual<PC>::D);
+    const KGeom& kr = geom_for("residual_kernel", residual_kernel<PC, 16>, shape_, PassResidual<PC>::D);
     if (kind_ == SEDI_BICGSTAB) {
         // r_1 lives in r[1] (buffers alternate with the iteration's parity)
         SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, stream_, kr.tg, ctl,
@@ -672,7 +681,7 @@ int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
                         a_, b_, c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
     }
     if (kind_ == SEDI_PCG) {
-        const KGeom& kp = geom_for(pcg_init_kernel<PC, 16>, shape_, PassPcgInit<PC>::D);
+        const KGeom& kp = geom_for("pcg_init_kernel", pcg_init_kernel<PC, 16>, shape_, PassPcgInit<PC>::D);
         SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_init_kernel", pcg_init_kernel, PC, kp, stream_, kp.tg, ctl,
                         a_, b_, c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
     }
@@ -686,20 +695,20 @@ int PlainSolver::launch_iterations(int count, PC pc, double* x) {
             reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
     for (int k = 0; k < count; k++) {
         if (kind_ == SEDI_CG) {
-            const KGeom& g1 = geom_for(cg_direction_kernel<PC, 16>, shape_, PassCgDirection<PC>::D);
-            const KGeom& g2 = geom_for(cg_update_kernel<PC, 16>, shape_, PassCgUpdate<PC>::D);
+            const KGeom& g1 = geom_for("cg_direction_kernel", cg_direction_kernel<PC, 16>, shape_, PassCgDirection<PC>::D);
+            const KGeom& g2 = geom_for("cg_update_kernel", cg_update_kernel<PC, 16>, shape_, PassCgUpdate<PC>::D);
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, stream_, g2.tg,
                             ctl, a_, b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
         } else if (kind_ == SEDI_PCG) {
-            const KGeom& g1 = geom_for(pcg_iter_kernel<PC, 16>, shape_, PassPcgIter<PC>::D);
+            const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, shape_, PassPcgIter<PC>::D);
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, stream_, g1.tg,
                             ctl, a_, b_, c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
         } else {
             double* const* vv = reinterpret_cast<double* const*>(static_cast<char*>(ctl_.p) + 384);
-            const KGeom& g1 = geom_for(bicgstab_direction_kernel<PC, 16>, shape_, PassBicgDirection<PC>::D);
-            const KGeom& g2 = geom_for(bicgstab_half_kernel<PC, 16>, shape_, PassBicgHalf<PC>::D);
-            const KGeom& g3 = geom_for(bicgstab_update_kernel<PC, 16>, shape_, PassBicgUpdate<PC>::D);
+            const KGeom& g1 = geom_for("bicgstab_direction_kernel", bicgstab_direction_kernel<PC, 16>, shape_, PassBicgDirection<PC>::D);
+            const KGeom& g2 = geom_for("bicgstab_half_kernel", bicgstab_half_kernel<PC, 16>, shape_, PassBicgHalf<PC>::D);
+            const KGeom& g3 = geom_for("bicgstab_update_kernel", bicgstab_update_kernel<PC, 16>, shape_, PassBicgUpdate<PC>::D);
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_direction_kernel", bicgstab_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, vv, pc));
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_half_kernel", bicgstab_half_kernel, PC, g2, stream_, g2.tg, ctl, a_, b_, c_, vv, pc));
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_update_kernel", bicgstab_update_kernel, PC, g3, stream_, g3.tg, ctl, a_, b_, c_, vv, x, pc));

@@ -12,6 +12,7 @@
 
 #include "profiler.h"
 #include "runtime.h"
+#include "tuning.h"
 #include "sedi_internal.h"
 
 namespace sedi {
@@ -145,6 +146,13 @@ int sedi_init(int device) {
 int sedi_graphs_enable(int on) {
     sedi::set_graphs_enabled(on != 0);
     return SEDI_OK;
+}
+
+int sedi_tuning_lookup(const char* key, const sedi_grid* grid, int fallback) {
+    if (!key || !grid) return fallback;
+    const sedi::Shape sh{(int)grid->n1, (int)grid->n2, (int)grid->n3};
+    const bool nearest = std::strncmp(key, "k:", 2) == 0;
+    return sedi::tuned_value(key, sh, fallback, nearest);
 }
 
 int sedi_profile_enable(const char* kernel_name) {

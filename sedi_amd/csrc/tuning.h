@@ -1,0 +1,23 @@
+// tuning.h -- launch parameters measured by the offline autotuner
+// (sedi_amd/autotune.py), read from a small text table at first use.
+#pragma once
+
+#include <string>
+
+#include "sedi_internal.h"
+
+namespace sedi {
+
+// Value stored for `key` and grid s in the tuning table, or `fallback`.
+// Keys: "zc:<kernel>" -- planes per tile of a march kernel (8, 16 or 32);
+//       "k:<solver>:<precision>" -- probes per batch of the estimator.
+// An entry matches the exact grid, or any grid when written for 0 0 0; the
+// key "zc:*" stands for every kernel.  With `nearest`, an entry of the same
+// key for the grid of nearest cell count also matches.
+int tuned_value(const std::string& key, const Shape& s, int fallback, bool nearest = false);
+
+// The table in use: SEDI_TUNING_FILE if set (empty = none), else tuning.txt
+// next to the library.  "" when no table was found.
+const std::string& tuning_path();
+
+}  // namespace sedi

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -58,3 +59,37 @@ def test_cli_usage_errors():
     assert r.returncode == 1
     r = subprocess.run([CLI, "-g", "8x8", "-p", "1x1x1"], capture_output=True, text=True)
     assert r.returncode == 1
+
+
+TUNING_PROBE = r"""
+import sys
+sys.path.insert(0, %r)
+import sedi_amd
+print(sedi_amd.tuning_lookup("zc:batch_upd_kernel", (256, 256, 256), -1),
+      sedi_amd.tuning_lookup("zc:batch_upd_kernel", (128, 128, 128), -1),
+      sedi_amd.tuning_lookup("zc:cg_update_kernel", (96, 96, 96), -1),
+      sedi_amd.tuning_lookup("zc:pcg_iter_kernel", (256, 256, 256), -1),
+      sedi_amd.tuning_lookup("k:2:1", (200, 200, 200), -1),
+      sedi_amd.tuning_lookup("k:2:1", (40, 40, 40), -1),
+      sedi_amd.tuning_lookup("k:3:1", (256, 256, 256), -1))
+"""
+
+
+def test_tuning_table_lookup(tmp_path):
+    """Exact grid first, then an entry for any grid (0 0 0), then the
+    wildcard kernel; batch sizes also from the nearest grid by cell count."""
+    table = tmp_path / "tuning.txt"
+    table.write_text("# comment line\n"
+                     "zc:batch_upd_kernel 256 256 256 32   # trailing comment\n"
+                     "zc:batch_upd_kernel 0 0 0 8\n"
+                     "zc:* 96 96 96 16\n"
+                     "k:2:1 64 64 64 16\n"
+                     "k:2:1 256 256 256 64\n"
+                     "malformed line\n")
+    env = dict(os.environ, SEDI_TUNING_FILE=str(table))
+    r = subprocess.run([sys.executable, "-c", TUNING_PROBE % ROOT], capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["32", "8", "16", "-1", "64", "16", "-1"]
+    env["SEDI_TUNING_FILE"] = ""  # no table: every lookup falls back
+    r = subprocess.run([sys.executable, "-c", TUNING_PROBE % ROOT], capture_output=True, text=True, env=env)
+    assert r.stdout.split() == ["-1"] * 7

@@ -370,3 +370,16 @@ def test_graph_replay_matches_direct_launches(kind):
     for x, y in zip(runs[False], runs[True]):
         assert np.array_equal(x["D"], y["D"]) and np.array_equal(x["iters"], y["iters"])
     assert np.array_equal(runs[True][0]["D"], runs[True][2]["D"])
+
+
+def test_estimator_autotuned_batch_size_changes_nothing_but_speed():
+    """batch_k = 0 takes the autotuner's k for the grid; a probe's result does
+    not depend on its batch, so D and the iteration counts equal an explicit
+    batch's bitwise."""
+    k = sedi.tuning_lookup("k:%d:%d" % (O.CG, sedi.FP32_MIXED), S32, 64)
+    assert k >= 1
+    auto = sedi.estimate(S32, kind=O.CG, samples=12, maxiter=500, tol=1e-9, seed=1234,
+                         precision=sedi.FP32_MIXED, batch_k=0)
+    fixed = sedi.estimate(S32, kind=O.CG, samples=12, maxiter=500, tol=1e-9, seed=1234,
+                          precision=sedi.FP32_MIXED, batch_k=4)
+    assert np.array_equal(auto["iters"], fixed["iters"]) and np.array_equal(auto["D"], fixed["D"])
