@@ -77,7 +77,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512", "serve"])
     p.add_argument("--jobs", type=int, default=200, help="serve: jobs in the stream")
-    p.add_argument("--rate", type=float, default=5.0, help="serve: Poisson arrival rate (jobs/s)")
+    p.add_argument("--rate", default="5", help="serve: Poisson arrival rate(s) in jobs/s, comma-separated")
     p.add_argument("--serve-jobs", type=int, default=30,
                    help="headline: jobs of the serving sample reported beside it (0 = none)")
     p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
@@ -537,10 +537,14 @@ def main():
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.mode == "serve":
-        line = dict(run_serve(args.jobs, args.rate), mode="serve",
-                    workload="Poisson stream truncated to %d jobs at %.1f jobs/s (sides 64-256, 8-64 probes, "
-                             "tol 1e-6/1e-8/1e-9, CG/PipelinedCG/BiCGStab, seed = job id, mixed, 200 GB "
-                             "budget)" % (args.jobs, args.rate))
+        runs = []
+        for rate in [float(v) for v in args.rate.split(",")]:
+            runs.append(run_serve(args.jobs, rate))
+            print(json.dumps(runs[-1]), file=sys.stderr, flush=True)
+        line = dict(mode="serve", runs=runs,
+                    workload="the serving stream (Poisson arrivals; sides 64/96/128/192/256, 8-64 probes, "
+                             "tol 1e-6/1e-8/1e-9, CG/PipelinedCG/BiCGStab uniform, seed = job id; FP32 "
+                             "storage / FP64 reductions; 200 GB budget), first %d jobs at each rate" % args.jobs)
     elif args.mode == "ksweep":
         line = run_ksweep(args)
     elif args.mode == "c512":

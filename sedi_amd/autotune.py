@@ -5,11 +5,15 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
 
     python -m sedi_amd.autotune [--sides 64,128,256,512] [--iters 40] [--k-iters 200] [--out sedi_amd/tuning.txt]
 
-1. z chunk: for each of 8, 16 and 32 planes, a child process runs every solver
-   (plain FP64 and batched mixed, CG / pipelined CG / BiCGStab) for a fixed
-   number of iterations on each grid with that chunk forced on every kernel
-   (``zc:* 0 0 0 Z``), timing each kernel with HIP events; each kernel then
-   gets the chunk of its least mean launch time on each grid.
+1. z chunk and schedule: for each of 8, 16 and 32 planes and each schedule
+   (0: the resident workgroups walk XCD-contiguous runs of tiles; 1: up to 8
+   workgroups per CU take tiles in launch order), a child process runs every
+   solver (plain FP64 and batched mixed, CG / pipelined CG / BiCGStab) for a
+   fixed number of iterations on each grid, and applies the standalone
+   stencil, with that chunk and schedule forced on every kernel
+   (``zc:* 0 0 0 Z``, ``sched:* 0 0 0 S``), timing each kernel with HIP
+   events; each kernel then gets the pair of least mean launch time on each
+   grid.
 2. batch size: with those chunks, the batched estimator runs k = 8..64 probes
    (as memory allows) for a fixed number of iterations; each (solver, precision,
    grid) gets the k of the highest probe-iterations per second.  In FP64 the
@@ -97,6 +101,17 @@ def measure_chunks(sides, iters):
                         res[name] = 1e3 * ms / launches
                     except sedi_amd.SediError:
                         pass
+        import numpy as np
+        x = np.random.default_rng(0).standard_normal(side ** 3)
+        st = sedi_amd.Stencil()
+        st.apply((side,) * 3, x)
+        sedi_amd.profile_reset()
+        sedi_amd.profile_enable("stencil_apply_kernel")
+        for _ in range(10):
+            st.apply((side,) * 3, x)
+        sedi_amd.profile_enable(None)
+        launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
+        res["stencil_apply_kernel"] = 1e3 * ms / launches
         out[side] = res
         print("  side %d: %s" % (side, json.dumps({k: round(v, 1) for k, v in res.items()})), file=sys.stderr,
               flush=True)
@@ -166,16 +181,18 @@ def main():
     t0 = time.time()
     timings = {}
     for zc in (8, 16, 32):
-        print("z chunk %d" % zc, file=sys.stderr, flush=True)
-        timings[zc] = child(args, ["zc:* 0 0 0 %d" % zc], "chunks")
+        for sched in (0, 1):
+            print("z chunk %d, schedule %d" % (zc, sched), file=sys.stderr, flush=True)
+            timings[(zc, sched)] = child(args, ["zc:* 0 0 0 %d" % zc, "sched:* 0 0 0 %d" % sched], "chunks")
     lines = []
     for side in args.sides:
-        names = sorted(set().union(*(timings[z].get(str(side), {}).keys() for z in timings)))
+        names = sorted(set().union(*(timings[c].get(str(side), {}).keys() for c in timings)))
         for name in names:
-            t = {z: timings[z][str(side)][name] for z in timings if name in timings[z].get(str(side), {})}
-            best = min(t, key=t.get)
-            lines.append("zc:%s %d %d %d %d   # us per launch: %s" % (
-                name, side, side, side, best, " ".join("%d:%.1f" % (z, t[z]) for z in sorted(t))))
+            t = {c: timings[c][str(side)][name] for c in timings if name in timings[c].get(str(side), {})}
+            zc, sched = min(t, key=t.get)
+            note = " ".join("%d/%d:%.1f" % (c[0], c[1], t[c]) for c in sorted(t))
+            lines.append("zc:%s %d %d %d %d   # us per launch (chunk/schedule): %s" % (name, side, side, side, zc, note))
+            lines.append("sched:%s %d %d %d %d" % (name, side, side, side, sched))
     print("batch sizes", file=sys.stderr, flush=True)
     batches = child(args, [l.split("#")[0] for l in lines], "batches")
     for b in batches:
@@ -191,8 +208,9 @@ def main():
     head = ["# sedi_amd tuning table, written by: python -m sedi_amd.autotune --sides %s --iters %d --k-iters %d" % (
                 ",".join(str(s) for s in args.sides), args.iters, args.k_iters),
             "# device: %s; %s; %.0f s" % (dev, time.strftime("%Y-%m-%d"), time.time() - t0),
-            "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, k:<solver>:<precision> = probes",
-            "# per batch (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
+            "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, sched:<kernel> = 1 for tiles in",
+            "# launch order (0: XCD-contiguous runs), k:<solver>:<precision> = probes per batch",
+            "# (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
     with open(args.out, "w") as f:
         f.write("\n".join(head + lines) + "\n")
     print("wrote %s (%d entries)" % (args.out, len(lines)), file=sys.stderr, flush=True)

@@ -193,15 +193,12 @@ KGeom bgeom(const char* name, Kern kern16, const Shape& s, int D, int ty = TY) {
 #endif
     occ = std::min(occ, 8);
     const int per = std::max(8, occ * batch_cus() / 8 * 8);
-    // the autotuner's z chunk for this kernel and grid, else the occupancy plan
+    // the occupancy plan, with the autotuner's z chunk and schedule for this
+    // kernel and grid (partials hold 8 workgroups per CU per RHS block)
+    const KGeom planned = plan_geometry(s, per, D, ty);
     const int zc = tuned_value(std::string("zc:") + name, s, 0);
-    if (zc == 8 || zc == 16 || zc == 32) {
-        KGeom k;
-        k.tg = make_tile_grid(s, zc, ty);
-        k.nblocks = (int)std::min<long long>(k.tg.ntiles, per);
-        return cache.emplace(key, k).first->second;
-    }
-    return cache.emplace(key, plan_geometry(s, per, D, ty)).first->second;
+    const int sched = tuned_value(std::string("sched:") + name, s, 0);
+    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, 8 * batch_cus())).first->second;
 }
 
 // Workgroups per RHS block of the elementwise kernels (independent of the

@@ -57,6 +57,7 @@ constexpr int NXCD = 8;
 struct TileGrid {
     int n1, n2, n3;
     int nbx, nby, nbz, zc, ty;
+    int xcd;  // 1: each XCD takes a contiguous run of tiles; 0: tiles in launch order
     long long s2, s3;
     long long ntiles;
 };
@@ -68,6 +69,7 @@ inline TileGrid make_tile_grid(const Shape& s, int zc, int ty = TY) {
     t.nby = (s.n2 + ty - 1) / ty;
     t.zc = zc;
     t.ty = ty;
+    t.xcd = 1;
     t.nbz = (s.n3 + zc - 1) / zc;
     t.s2 = s.n1;
     t.s3 = (long long)s.n1 * s.n2;
@@ -87,6 +89,23 @@ struct KGeom {
 // one tile per resident workgroup and the prologue costs ~2 + D planes.
 // The march is compiled for chunks of 8, 16 and 32 planes.
 constexpr int ZC_CHOICES[3] = {8, 16, 32};
+
+// The launch plan the autotuner may override per kernel and grid: "zc:<kernel>"
+// the planes per tile, "sched:<kernel>" 1 for as many workgroups as the
+// partial sums allow (up to 8 per CU) in launch order instead of the resident
+// set walking XCD runs.
+inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sched, int max_blocks) {
+    KGeom k = planned;
+    if (zc == 8 || zc == 16 || zc == 32) {
+        k.tg = make_tile_grid(s, zc, planned.tg.ty);
+        k.nblocks = (int)std::min<long long>(k.tg.ntiles, planned.nblocks);
+    }
+    if (sched == 1) {
+        k.tg.xcd = 0;
+        k.nblocks = (int)std::min<long long>(k.tg.ntiles, max_blocks);
+    }
+    return k;
+}
 
 inline KGeom plan_geometry(const Shape& s, int resident, int D, int ty = TY) {
     const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + ty - 1) / ty);
@@ -206,12 +225,16 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Tile handled by step t of the grid-stride loop.  `reverse` walks each
+// Tile handled by step t of the grid-stride loop.  With xcd set, each XCD's
+// workgroups walk a contiguous run of tiles (halo rows shared in its L2);
+// otherwise consecutive tiles go to consecutive workgroups, which keeps the
+// XCDs' concurrent accesses within one region of memory (measured faster on
+// grids much larger than the Infinity Cache; the autotuner decides).  `reverse` walks each
 // XCD's run of tiles backwards: consecutive passes alternate direction, so a
 // pass starts on the tiles the previous pass touched last, whose vectors are
 // still in the 256 MB Infinity Cache.
 __device__ __forceinline__ long long tile_of(const TileGrid& g, long long t, bool reverse) {
-    if ((g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
+    if (g.xcd && (g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
         const long long per = g.ntiles / NXCD;
         const long long k = t / NXCD;
         return (t % NXCD) * per + (reverse ? per - 1 - k : k);

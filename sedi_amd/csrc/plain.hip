@@ -598,15 +598,13 @@ static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, 
         occ = 4;
     }
     occ = std::min(occ, MAX_BLOCKS_PER_CU);
-    // the autotuner's z chunk for this kernel and grid, else the occupancy plan
+    // the occupancy plan, with the autotuner's z chunk and schedule for this
+    // kernel and grid (the partial sums hold MAX_BLOCKS_PER_CU per CU)
+    const KGeom planned = plan_geometry(s, occ * num_cus(), D, ty);
     const int zc = tuned_value(std::string("zc:") + name, s, 0);
-    if (zc == 8 || zc == 16 || zc == 32) {
-        KGeom k;
-        k.tg = make_tile_grid(s, zc, ty);
-        k.nblocks = (int)std::min<long long>(k.tg.ntiles, (long long)occ * num_cus());
-        return cache.emplace(key, k).first->second;
-    }
-    return cache.emplace(key, plan_geometry(s, occ * num_cus(), D, ty)).first->second;
+    const int sched = tuned_value(std::string("sched:") + name, s, 0);
+    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, MAX_BLOCKS_PER_CU * num_cus()))
+        .first->second;
 }
 
 // kernel<..., ZC> for the ZC the geometry chose

@@ -10,9 +10,12 @@ namespace sedi {
 
 // Value stored for `key` and grid s in the tuning table, or `fallback`.
 // Keys: "zc:<kernel>" -- planes per tile of a march kernel (8, 16 or 32);
+//       "sched:<kernel>" -- 1: workgroups take tiles in launch order, up to
+//                           8 per CU; 0 (default): the resident set walks
+//                           XCD-contiguous runs of tiles;
 //       "k:<solver>:<precision>" -- probes per batch of the estimator.
 // An entry matches the exact grid, or any grid when written for 0 0 0; the
-// key "zc:*" stands for every kernel.  With `nearest`, an entry of the same
+// keys "zc:*" and "sched:*" stand for every kernel.  With `nearest`, an entry of the same
 // key for the grid of nearest cell count also matches.
 int tuned_value(const std::string& key, const Shape& s, int fallback, bool nearest = false);
 

@@ -71,7 +71,8 @@ print(sedi_amd.tuning_lookup("zc:batch_upd_kernel", (256, 256, 256), -1),
       sedi_amd.tuning_lookup("zc:pcg_iter_kernel", (256, 256, 256), -1),
       sedi_amd.tuning_lookup("k:2:1", (200, 200, 200), -1),
       sedi_amd.tuning_lookup("k:2:1", (40, 40, 40), -1),
-      sedi_amd.tuning_lookup("k:3:1", (256, 256, 256), -1))
+      sedi_amd.tuning_lookup("k:3:1", (256, 256, 256), -1),
+      sedi_amd.tuning_lookup("sched:cg_update_kernel", (96, 96, 96), -1))
 """
 
 
@@ -85,11 +86,12 @@ def test_tuning_table_lookup(tmp_path):
                      "zc:* 96 96 96 16\n"
                      "k:2:1 64 64 64 16\n"
                      "k:2:1 256 256 256 64\n"
+                     "sched:* 0 0 0 1\n"
                      "malformed line\n")
     env = dict(os.environ, SEDI_TUNING_FILE=str(table))
     r = subprocess.run([sys.executable, "-c", TUNING_PROBE % ROOT], capture_output=True, text=True, env=env)
     assert r.returncode == 0, r.stderr
-    assert r.stdout.split() == ["32", "8", "16", "-1", "64", "16", "-1"]
+    assert r.stdout.split() == ["32", "8", "16", "-1", "64", "16", "-1", "1"]
     env["SEDI_TUNING_FILE"] = ""  # no table: every lookup falls back
     r = subprocess.run([sys.executable, "-c", TUNING_PROBE % ROOT], capture_output=True, text=True, env=env)
-    assert r.stdout.split() == ["-1"] * 7
+    assert r.stdout.split() == ["-1"] * 8
