@@ -650,7 +650,7 @@ int BatchSolver::bind_t(int hist_need) {
     //    FP64's iterations.
     c.rep_every = kind_ == SEDI_PCG ? std::min(rep_every_, 10)
                   : kind_ == SEDI_BICGSTAB ? 1 << 30 : rep_every_;
-    c.drop = 0.1;
+    c.drop = kind_ == SEDI_BICGSTAB ? SEDI_BICG_DROP : 0.1;
     c.restart = 2.0;
     c.bicg_reset = SEDI_BICG_RESET;
     c.mixed = fp32_ ? 1 : 0;

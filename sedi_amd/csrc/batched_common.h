@@ -20,6 +20,9 @@
 #ifndef SEDI_BICG_RESET
 #define SEDI_BICG_RESET 1
 #endif
+#ifndef SEDI_BICG_DROP
+#define SEDI_BICG_DROP 0.1  // BiCGStab (mixed): replace and restart at each tenfold decrease
+#endif
 
 
 namespace sedi {
