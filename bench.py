@@ -63,9 +63,17 @@ BATCH = dict(samples=64, seed=11, tol=1e-8, rep_every=50)
 CMP_SAMPLES = 16
 # dominant kernel per solver and the FP64 vectors it streams per launch
 DOMINANT = {2: ("cg_update_kernel", 5), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
-# batched mixed CG: batch_upd_kernel streams p, x_lo, r (read) and x_lo, r
-# (written): 5 FP32 values per right-hand side and cell
-BATCH_KERNEL, BATCH_BYTES_PER_RHS_CELL = "batch_upd_kernel", 5 * 4
+# batched mixed CG: batch_upd_kernel streams p and r in and r out every
+# iteration, and every second (even) iteration also the previous direction and
+# x_lo in and x_lo out (the two iterations' solution updates are applied
+# together): 3 or 6 FP32 values per right-hand side and cell
+BATCH_KERNEL = "batch_upd_kernel"
+
+
+def upd_values(iters):
+    """FP32 values batch_upd_kernel streams for a right-hand side that ran
+    `iters` iterations: 3 per odd iteration, 6 per even one."""
+    return 3 * ((iters + 1) // 2) + 6 * (iters // 2)
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sedi_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
 
@@ -318,7 +326,7 @@ def run_headline(args):
     # algorithmic bytes: each right-hand side is credited with its own
     # iterations only (a converged one that still rides in an active block of
     # 4 is not counted)
-    rhs_iters = int(iters.sum())
+    rhs_values = int(sum(upd_values(int(m)) for m in iters))
     config = {"workload": "batched mixed-precision CG estimator, 256^3 grid, 64 probes (seed 11) "
                           "as one block of k=%d, FP32 vectors / FP64 reductions, residual "
                           "replacement every 50, rel. residual 1e-8, maxiter 2000" % K,
@@ -331,8 +339,7 @@ def run_headline(args):
                           "mean": round(float(iters.mean()), 2),
                           "unconverged": int(np.count_nonzero(codes))}
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
-    line["roofline"] = roofline(BATCH_KERNEL, launches, kms,
-                                rhs_iters * n * BATCH_BYTES_PER_RHS_CELL)
+    line["roofline"] = roofline(BATCH_KERNEL, launches, kms, rhs_values * n * 4)
     if not args.no_compare:
         # plain FP64 drop-in and the mixed form on the first 16 probes of the
         # same stream: the speed-up and the diagonal error of the mixed form

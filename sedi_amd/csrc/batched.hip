@@ -32,6 +32,18 @@
 // only when its true residual meets the tolerance.  With FP64 storage the
 // recurrences are the plain solvers'.
 //
+// Solution updates of CG come in pairs: an odd iteration leaves x alone and
+// records its alpha as pending (apend); the next even iteration adds
+// apend * p_prev + alpha * p in one read-modify-write of x, reading p_prev
+// from the other direction buffer, which it still holds.  That saves one x
+// read and one x write every other iteration (7.5 instead of 8 vector streams
+// per iteration).  Whatever reads x while an update is pending adds it: the
+// FP64 replacement (before its stencil) and fold, the accumulation and the
+// unpacking of x, and -- for a right-hand side that finished at an odd
+// iteration while its block goes on -- the next direction pass, which reads
+// that p as its old direction.  The pending p is always in the second
+// direction buffer, which odd iterations write.
+//
 // Per-RHS state makes each probe's trajectory independent of the others in
 // its batch: an RHS that is done keeps x and r (alpha = beta = 0),
 // replacements are applied per RHS, and each RHS's sums have a fixed order
@@ -93,6 +105,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         c.ru_old[k] = acc[4 + q];
         c.alpha[k] = 0.0;
         c.beta[k] = 0.0;
+        c.apend[k] = 0.0;
         c.rep[k] = 0;
         c.fold[k] = 0;
         c.sexit[k] = 0;
@@ -126,9 +139,11 @@ struct PassBDir {
     const V* r;
     const V* pold;
     V* pnew;
+    V* x;
     T im;
     V beta;
-    bool first;
+    V flush;  // pending coefficients of lanes that finished (else 0)
+    bool first, any_flush;
     struct Raw { V r, p; };
     struct Aux {};
     __device__ Raw load(cidx_t i) const { return {ldc(r, i), ldc(pold, i)}; }
@@ -137,8 +152,9 @@ struct PassBDir {
         const V u = im * q.r;
         return first ? u : u + beta * q.p;
     }
-    __device__ void finish(cidx_t i, const Raw&, const Aux&, const V& p, const V& s,
+    __device__ void finish(cidx_t i, const Raw& q, const Aux&, const V& p, const V& s,
                            double (&acc)[ND]) const {
+        if (any_flush) stv(x, i, ldc((const V*)x, i) + flush * q.p);
         stv(pnew, i, p);
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q] += (double)s[q] * (double)p[q];
@@ -147,7 +163,7 @@ struct PassBDir {
 
 template <class T, int ZC>
 __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batch_dir_kernel(
-    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r, V4<T>* p0, V4<T>* p1, T im) {
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r, V4<T>* p0, V4<T>* p1, V4<T>* x, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     constexpr int TYK = CgTile<T>::ty;
@@ -156,11 +172,19 @@ __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batc
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
-    V4<T> beta;
+    V4<T> beta, flush;
+    bool any_flush = false;
 #pragma unroll
-    for (int q = 0; q < 4; q++) beta[q] = c.done[4 * rb + q] ? (T)0 : (T)c.beta[4 * rb + q];
-    PassBDir<T> P{a, b, cc, r + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off, im,
-                  beta, it == 1};
+    for (int q = 0; q < 4; q++) {
+        const bool d = c.done[4 * rb + q];
+        beta[q] = d ? (T)0 : (T)c.beta[4 * rb + q];
+        // a lane that finished at an odd iteration: its last update is
+        // pending on p_old, which this pass reads
+        flush[q] = d ? (T)c.apend[4 * rb + q] : (T)0;
+        any_flush |= flush[q] != (T)0;
+    }
+    PassBDir<T> P{a, b, cc, r + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off, x + off, im,
+                  beta, flush, it == 1, any_flush};
     double acc[4];
     run_tiles<PassBDir<T>, ZC, TYK>(P, g, lds, acc, false);
     if (!grid_sum<4, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -169,6 +193,7 @@ __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batc
     for (int q = 0; q < 4; q++) {
         const int k = 4 * rb + q;
         c.alpha[k] = c.done[k] ? 0.0 : c.ru[k] / acc[q];  // Cg.hpp:166
+        if (c.done[k]) c.apend[k] = 0.0;
     }
 }
 
@@ -179,20 +204,29 @@ struct PassBUpd {
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
     T a, b, c;
     const V* p;
+    const V* pprev;  // the previous iteration's direction (even iterations)
     V* xlo;
     V* r;
     T im;
-    V alpha;
+    V alpha, apend;
+    bool xstep;  // even iteration: x += apend p_prev + alpha p
     struct Raw { V p; };
-    struct Aux { V x, r; };
+    struct Aux { V x, pp, r; };
     __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(xlo, i), ldc(r, i)}; }
+    __device__ Aux load_aux(cidx_t i) const {
+        Aux o;
+        if (xstep) {
+            o.x = ldc((const V*)xlo, i);
+            o.pp = ldc(pprev, i);
+        }
+        o.r = ldc(r, i);
+        return o;
+    }
     __device__ V f(const Raw& q) const { return q.p; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V& pv, const V& s,
                            double (&acc)[ND]) const {
-        const V xn = o.x + alpha * pv;
+        if (xstep) stv(xlo, i, (o.x + apend * o.pp) + alpha * pv);
         const V rn = o.r - alpha * s;
-        stv(xlo, i, xn);
         stv(r, i, rn);
         const V u = im * rn;
 #pragma unroll
@@ -215,10 +249,15 @@ __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batc
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
-    V4<T> alpha;
+    V4<T> alpha, apend;
 #pragma unroll
-    for (int q = 0; q < 4; q++) alpha[q] = (T)c.alpha[4 * rb + q];
-    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, xlo + off, r + off, im, alpha};
+    for (int q = 0; q < 4; q++) {
+        alpha[q] = (T)c.alpha[4 * rb + q];
+        apend[q] = (T)c.apend[4 * rb + q];
+    }
+    const bool xstep = (it & 1) == 0;
+    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, ((it & 1) ? p0 : p1) + off, xlo + off, r + off,
+                  im, alpha, apend, xstep};
     double acc[8];
     run_tiles<PassBUpd<T>, ZC, TYK>(P, g, lds, acc, true);
     if (!grid_sum<8, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -226,6 +265,9 @@ __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batc
         return;
     for (int q = 0; q < 4; q++) {
         const int k = 4 * rb + q;
+        // odd iteration: this alpha (0 for a finished lane) waits for the next
+        // even one; even: both updates are in x
+        c.apend[k] = xstep ? 0.0 : c.alpha[k];
         if (c.done[k]) continue;
         const double ru_new = acc[q];
         c.ru_old[k] = c.ru[k];
@@ -254,7 +296,8 @@ __global__ __launch_bounds__(TX * CgTile<T>::ty, CgTile<T>::min_waves) void batc
 
 // ---- FP64 replacement: r = b - A (x_hi + x_lo) -------------------------------
 // The new residual is computed in FP64 from the FP64 sum of the two parts of
-// x and stored rounded to the vector type.  x itself is read at the halo by
+// x (plus CG's pending update, apend * p) and stored rounded to the vector
+// type.  x itself is read at the halo by
 // neighbouring tiles during this pass, so the fold x_hi += x_lo, x_lo = 0 is
 // left to batch_fold_kernel, which runs after it.  The second dot is <r, u>
 // (CG) or <r, r^> (BiCGStab, RHAT).
@@ -268,16 +311,19 @@ struct PassBRep {
     V4<T>* r;
     const V4<T>* rhs;
     V4<T>* rhat;
+    const V4<T>* pend;  // CG: the direction with a pending update (null: none)
+    V apend;
     T im;
     int rep[4];
     bool reset;  // BiCGStab: r^ restarts from the replaced residual
-    struct Raw { V xh; V4<T> xl; };
+    struct Raw { V xh; V4<T> xl, pp; };
     struct Aux { V4<T> b, r, rh; };
     __device__ Raw load(cidx_t i) const {
         Raw q;
 #pragma unroll
         for (int k = 0; k < 4; k++) q.xh[k] = ldc(xh[k], i);
         q.xl = ldc(xlo, i);
+        if (pend) q.pp = ldc(pend, i);
         return q;
     }
     __device__ Aux load_aux(cidx_t i) const {
@@ -291,6 +337,10 @@ struct PassBRep {
         V x;
 #pragma unroll
         for (int k = 0; k < 4; k++) x[k] = q.xh[k] + (double)q.xl[k];
+        if (pend) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[k] = x[k] + apend[k] * (double)q.pp[k];
+        }
         return x;
     }
     __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V&, const V& Ax,
@@ -320,7 +370,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
                                                              double cc, long long n, const double* xhi,
                                                              const V4<T>* xlo, V4<T>* r0, V4<T>* r1,
                                                              const V4<T>* rhs, V4<T>* rhat,
-                                                             T im) {
+                                                             const V4<T>* pend, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     const int* rp = c.rep + 4 * rb;
@@ -343,6 +393,13 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     P.r = r + off;
     P.rhs = rhs + off;
     P.rhat = RHAT ? rhat + off : nullptr;
+    bool pending = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        P.apend[k] = pend ? c.apend[4 * rb + k] : 0.0;
+        pending |= P.apend[k] != 0.0;
+    }
+    P.pend = pending ? pend + off : nullptr;
     P.im = im;
     P.reset = c.bicg_reset != 0;
     double acc[8];
@@ -389,7 +446,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
 // sum, which the final x = x_hi + x_lo would make anyway.
 template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl c, double* xhi,
-                                                              V4<T>* xlo) {
+                                                              V4<T>* xlo, const V4<T>* pend) {
     const int rb = blockIdx.y;
     int* fp = c.fold + 4 * rb;
     const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
@@ -397,15 +454,28 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     __shared__ double red[NWAVES];
     __shared__ int lastflag;
     const int fl[4] = {f0, f1, f2, f3};
+    double ap[4];
+    bool pending = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ap[k] = (pend && fl[k]) ? c.apend[4 * rb + k] : 0.0;
+        pending |= ap[k] != 0.0;
+    }
     V4<T>* xq = xlo + (long long)rb * n;
+    const V4<T>* pq = pending ? pend + (long long)rb * n : nullptr;
     for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
          i += (long long)gridDim.x * NTHREADS) {
         V4<T> xl = xq[i];
+        V4<T> pp{};
+        if (pq) pp = pq[i];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (fl[k]) {
                 double* h = xhi + (long long)(4 * rb + k) * n + i;
-                *h = *h + (double)xl[k];
+                // x_hi + x_lo (+ CG's pending apend * p), as the replacement used
+                double v = *h + (double)xl[k];
+                if (pq) v = v + ap[k] * (double)pp[k];
+                *h = v;
                 xl[k] = (T)0;
             }
         }
@@ -415,19 +485,24 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     if (!grid_sum<1>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
+    for (int k = 0; k < 4; k++)
+        if (fp[k] && pend) c.apend[4 * rb + k] = 0.0;
     fp[0] = fp[1] = fp[2] = fp[3] = 0;
 }
 
 // ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
+// x = x_hi + x_lo (+ CG's pending apend * p: pend non-null)
 template <class T>
 __global__ void batch_accum_kernel(long long n, int k0, int k1, const V4<T>* xlo, const double* xhi,
-                                   const V4<T>* b, double* t, double* q) {
+                                   const V4<T>* b, const V4<T>* pend, const double* apend, double* t,
+                                   double* q) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         double tv = t[i], qv = q[i];
         for (int k = k0; k < k1; k++) {
             const long long cell = (long long)(k >> 2) * n + i;
-            const double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
+            double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
+            if (pend && apend[k] != 0.0) x = x + apend[k] * (double)pend[cell][k & 3];
             const double v = (double)b[cell][k & 3];
             tv = tv + x * v;  // stochastic-estimator.cc:195
             qv = qv + v * v;  // :198
@@ -457,9 +532,11 @@ __global__ void batch_pack_kernel(long long n, int kact, long long row_stride, c
 
 // Rows k0 .. k1-1 of the batch (x = x_hi + x_lo, or the right-hand sides when
 // xhi is null and `src` is b) into [k1-k0][n] FP64.  Grid y = RHS blocks.
+// (plus CG's pending apend * p for x: pend non-null, apend indexed like xhi)
 template <class T>
 __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __restrict__ src,
-                                    const double* __restrict__ xhi, double* __restrict__ x) {
+                                    const double* __restrict__ xhi, const V4<T>* __restrict__ pend,
+                                    const double* __restrict__ apend, double* __restrict__ x) {
     const int rb = blockIdx.y;
     const V4<T>* s = src + (long long)rb * n;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -468,7 +545,11 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const long long k = 4 * rb + q;
-            if (k >= k0 && k < k1) x[(k - k0) * n + i] = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
+            if (k >= k0 && k < k1) {
+                double xv = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
+                if (pend && apend[k] != 0.0) xv = xv + apend[k] * (double)pend[(long long)rb * n + i][q];
+                x[(k - k0) * n + i] = xv;
+            }
         }
     }
 }
@@ -476,6 +557,13 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// The direction buffer that holds CG's pending update (odd iterations write
+// the second one); null for the other solvers, whose x is always current.
+template <class T>
+static const V4<T>* pending_dir(const BRun& r) {
+    return r.c.kind == SEDI_CG ? static_cast<const V4<T>*>(r.v[CG_P1]) : nullptr;
+}
+
 template <class T, bool RHAT>
 static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
     const KGeom g = bgeom("batch_rep_kernel", batch_rep_kernel<T, RHAT, 16>, r.shape, SEDI_PREFETCH_BATCH);
@@ -484,20 +572,21 @@ static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
     const double* xhi = r.xhi;
     const V4<T>* xlo = static_cast<const V4<T>*>(r.v[SLOT_XLO]);
     const V4<T>* b = static_cast<const V4<T>*>(r.v[SLOT_B]);
+    const V4<T>* pend = pending_dir<T>(r);
     if (g.tg.zc == 8)
         return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 8>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
-                      r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+                      r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
     if (g.tg.zc == 32)
         return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 32>, grid, dim3(NTHREADS), r.stream, g.tg,
-                      r.c, r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+                      r.c, r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
     return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 16>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
-                  r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, im);
+                  r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
 }
 
 template <class T>
 int fold_only(const BRun& r) {
     SEDI_HIP(launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS),
-                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO])));
+                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dir<T>(r)));
     return SEDI_OK;
 }
 
@@ -525,7 +614,7 @@ static int cg_iteration(const BRun& r) {
     const KGeom g1 = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
     const KGeom g2 = bgeom("batch_upd_kernel", batch_upd_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgTile<T>::ty);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
-                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, im));
+                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, XL, im));
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_upd_kernel", batch_upd_kernel, T, g2, dim3(g2.nblocks, r.nrb), r.stream,
                              g2.tg, r.c, a, b, cc, r.n, (const V4<T>*)P0, (const V4<T>*)P1, XL, R, im));
     if (r.mixed) return replace_and_fold<T>(r, R, nullptr, true);
@@ -536,7 +625,7 @@ static int cg_iteration(const BRun& r) {
 enum {
     CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
     CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW, CD_TOL,
-    CD_NDOUBLE
+    CD_APEND, CD_NDOUBLE
 };
 enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_MAXIT, CI_NINT };
 
@@ -619,6 +708,7 @@ int BatchSolver::bind_t(int hist_need) {
     c.gamma_prev = d + CD_GAMMAP * K; c.alpha_prev = d + CD_ALPHAP * K;
     c.rho = d + CD_RHO * K; c.rho_prev = d + CD_RHOP * K; c.omega = d + CD_OMEGA * K; c.ss = d + CD_SS * K;
     c.cbw = d + CD_CBW * K;
+    c.apend = d + CD_APEND * K;
     c.tolv = d + CD_TOL * K;
     int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
     c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
@@ -851,7 +941,8 @@ int BatchSolver::accumulate_t(double* t, double* q, int k0, int k1) {
     long long g = (n + 255) / 256;
     if (g > 8192) g = 8192;
     SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n,
-                    k0, k1, (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B], t, q));
+                    k0, k1, (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B],
+                    pending_dir<T>(*run_), (const double*)run_->c.apend, t, q));
     return SEDI_OK;
 }
 
@@ -875,10 +966,12 @@ int BatchSolver::unpack_t(double* x, int k0, int k1, bool rhs) {
     const int rb0 = k0 / 4, rb1 = (k1 + 3) / 4;
     const V4<T>* src = static_cast<const V4<T>*>(rhs ? v_[SLOT_B] : v_[SLOT_XLO]) + (long long)rb0 * n;
     const double* xhi = rhs ? nullptr : xhi_;
+    const V4<T>* pend = rhs ? nullptr : pending_dir<T>(*run_);
     // the kernel indexes RHS blocks from rb0: shift the row range to match
     SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), rb1 - rb0),
                     dim3(NTHREADS), stream_, n, k0 - 4 * rb0, k1 - 4 * rb0, src,
-                    xhi ? xhi + (long long)4 * rb0 * n : nullptr, x));
+                    xhi ? xhi + (long long)4 * rb0 * n : nullptr, pend ? pend + (long long)rb0 * n : nullptr,
+                    run_->c.apend + 4 * rb0, x));
     return SEDI_OK;
 }
 

@@ -125,6 +125,7 @@ struct BCtl {
     double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *elast, *hist;
     double *gamma, *delta, *errp, *gamma_prev, *alpha_prev;  // pipelined CG
     double *rho, *rho_prev, *omega, *ss, *cbw;               // BiCGStab
+    double* apend;  // CG: coefficient of the pending p (the last odd iteration's) not yet in x
     int *fiter, *code, *done, *rep, *fold, *sexit;
     // per RHS block
     int *iter, *rb_done;
