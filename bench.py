@@ -61,8 +61,9 @@ PLAIN = dict(samples=16, seed=7, tol=1e-9)
 # headline batched mixed workload and its FP64 comparison
 BATCH = dict(samples=64, seed=11, tol=1e-8, rep_every=50)
 CMP_SAMPLES = 16
-# dominant kernel per solver and the FP64 vectors it streams per launch
-DOMINANT = {2: ("cg_update_kernel", 5), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
+# dominant kernel per solver and the FP64 vectors it streams per iteration
+# (CG: see upd_values)
+DOMINANT = {2: ("cg_update_kernel", None), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
 # batched mixed CG: batch_upd_kernel streams p and r in and r out every
 # iteration, and every second (even) iteration also the previous direction and
 # x_lo in and x_lo out (the two iterations' solution updates are applied
@@ -71,8 +72,9 @@ BATCH_KERNEL = "batch_upd_kernel"
 
 
 def upd_values(iters):
-    """FP32 values batch_upd_kernel streams for a right-hand side that ran
-    `iters` iterations: 3 per odd iteration, 6 per even one."""
+    """Vectors the CG update pass (batch_upd_kernel, cg_update_kernel) streams
+    for a right-hand side that ran `iters` iterations: 3 per odd iteration, 6
+    per even one."""
     return 3 * ((iters + 1) // 2) + 6 * (iters // 2)
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sedi_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
@@ -293,7 +295,15 @@ def run_plain(args):
     line["iterations"] = {"min": int(iters.min()), "max": int(iters.max()),
                           "mean": round(float(iters.mean()), 2)}
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
-    line["roofline"] = roofline(kname, launches, kms, launches * nvec * n * 8)
+    # algorithmic bytes of the iterations the probes ran (launches queued after
+    # a solve converged return at once and move nothing); plain CG pairs its
+    # solution updates like the batched one: 3 vectors at odd iterations, 6
+    # at even ones
+    if kind == 2:
+        vecs = sum(upd_values(int(m)) for m in iters)
+    else:
+        vecs = int(iters.sum()) * nvec
+    line["roofline"] = roofline(kname, launches, kms, vecs * n * 8)
     line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(kind, PLAIN["seed"], PLAIN["tol"], float(iters.mean()),

@@ -17,6 +17,7 @@ namespace sedi {
 struct DevState {
     double bb, tol, err, final_err;
     double ru, alpha, beta;          // CG (alpha, beta shared with the others)
+    double apend;                    // CG: alpha of the odd iteration whose x update is pending
     double gamma, delta, err_pending;  // pipelined CG
     double rho, rho_prev, omega, ss, cbw;  // BiCGStab
     int maxiter, iter, done, code, final_iter, nhist, sexit, kind;

@@ -78,6 +78,16 @@ __global__ void init_state_kernel(DevState* st, int kind, int maxiter, double to
     }
 }
 
+// CG's last solution update when the solve ended at an odd iteration: x +=
+// apend p (p of an odd iteration is in the second direction buffer).
+__global__ void cg_flush_kernel(const DevState* st, long long n, double* x, const double* p1) {
+    const double ap = st->apend;
+    if (ap == 0.0) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = 1.0 * x[i] + ap * p1[i];
+}
+
 // ---------------------------------------------------------------------------
 // passes
 // ---------------------------------------------------------------------------
@@ -152,24 +162,46 @@ struct PassCgDirection {
 
 // CG update: s = A p recomputed; x += alpha p; r -= alpha s; u = r.*iM;
 // <r,u>, <r,r> (Cg.hpp:168-182).
+//
+// x is write-only in CG's recurrences, so its updates go in pairs: an odd
+// iteration leaves x alone (its alpha becomes DevState::apend), the next even
+// one computes x + apend p_prev, then + alpha p -- the reference's two
+// combine2 updates (SliceOps.hpp:38-40) in their order, so x is bitwise the
+// same -- reading p_prev from the other direction buffer, which still holds
+// it.  A solve that ends at an odd iteration gets its last update from
+// cg_flush_kernel.  The pass streams 3 vectors at odd iterations and 6 at
+// even ones instead of 5 every iteration.
 template <class PC>
 struct PassCgUpdate {
     static constexpr int ND = 2, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* p;
+    const double* pprev;
     double* x;
     double* r;
     PC iM;
-    double alpha;
+    double alpha, apend;
+    bool xstep;
     struct Raw { double p; };
-    struct Aux { double x, r, im; };
+    struct Aux { double x, pp, r, im; };
     __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(x, i), ldc(r, i), iM.at(i)}; }
+    __device__ Aux load_aux(cidx_t i) const {
+        Aux o;
+        if (xstep) {
+            o.x = ldc(x, i);
+            o.pp = ldc(pprev, i);
+        }
+        o.r = ldc(r, i);
+        o.im = iM.at(i);
+        return o;
+    }
     __device__ double f(const Raw& q) const { return q.p; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
-        const double xv = 1.0 * q.x + alpha * pv;
+        if (xstep) {
+            const double xm = 1.0 * q.x + apend * q.pp;
+            stc(x, i, 1.0 * xm + alpha * pv);
+        }
         const double rv = 1.0 * q.r + (-alpha) * s;
-        stc(x, i, xv);
         stc(r, i, rv);
         const double u = rv * q.im;
         acc[0] += rv * u;
@@ -406,11 +438,14 @@ __global__ __launch_bounds__(PTHREADS, 4) void cg_update_kernel(TileGrid g, Ctl 
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, x, r, iM, cst->alpha};
+    const bool xstep = (it & 1) == 0;
+    PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, (it & 1) ? p0 : p1, x, r, iM, cst->alpha, cst->apend,
+                       xstep};
     double acc[2];
     run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, true);
     if (!grid_sum<2, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
+    st->apend = xstep ? 0.0 : st->alpha;
     const double ru_new = acc[0];
     st->beta = ru_new / st->ru;  // Cg.hpp:185
     st->ru = ru_new;
@@ -786,6 +821,12 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
         }
         pending = true;
         slot ^= 1;
+    }
+    if (kind_ == SEDI_CG) {
+        const long long n = shape_.n();
+        const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4LL * num_cus());
+        SEDI_HIP(launch("cg_flush_kernel", cg_flush_kernel, dim3(std::max(1u, blocks)), dim3(256), stream_,
+                        (const DevState*)st, n, x, (const double*)v_[2]));
     }
     SEDI_HIP(hipStreamSynchronize(stream_));
     profile_flush();
