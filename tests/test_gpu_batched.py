@@ -113,6 +113,37 @@ def test_batch_eigenvector_one_iteration():
     assert np.allclose(x[0], phi / lam, atol=1e-14)
 
 
+@pytest.mark.parametrize("prec", [sedi.FP64, sedi.FP32_MIXED])
+def test_update_pending_when_finishing_at_odd_iteration(prec):
+    """CG applies solution updates in pairs; a right-hand side that finishes at
+    an odd iteration has one pending.  Here the eigenvector finishes at
+    iteration 1 while its block-mates go on (the next direction pass applies
+    it), and alone in its block (the read-out applies it); the block-mates'
+    results equal their solves alone bitwise."""
+    shape = (8, 8, 8)
+    i = np.arange(1, 9)
+    s = np.sin(np.pi * 2 * i / 9)
+    phi = (s[:, None, None] * s[None, :, None] * s[None, None, :]).reshape(-1)
+    lam = 6.0 - 6 * np.cos(np.pi * 2 / 9)
+    others = probes(9, shape, 0, 2)
+    b = np.vstack([phi, others])
+    bs = sedi.Batch(shape, 4, prec)
+    x, iters, errs, codes = bs.solve(b, maxiter=200, tol=1e-10)
+    assert np.all(codes == 0) and np.all(iters[1:] > 2)
+    if prec == sedi.FP64:
+        # the eigenvector is solved exactly by one step (FP32 storage rounds
+        # phi, so the mixed form takes a few more, ending where it may)
+        assert iters[0] == 1
+        assert np.max(np.abs(x[0] - phi / lam)) < 1e-13 * np.max(np.abs(phi / lam))
+    else:  # the mixed form solves with b rounded to FP32
+        assert np.max(np.abs(x[0] - phi / lam)) < 1e-7 * np.max(np.abs(phi / lam))
+    for k in (1, 2):
+        x1, it1, e1, c1 = bs.solve(b[k], maxiter=200, tol=1e-10)
+        assert np.array_equal(x1[0], x[k]) and (it1[0], c1[0]) == (iters[k], codes[k])
+    x0, it0, e0, c0 = bs.solve(phi, maxiter=200, tol=1e-10)
+    assert np.array_equal(x0[0], x[0]) and it0[0] == iters[0]
+
+
 # ------------------------------------------- pipelined CG and BiCGStab ------
 KINDS = [O.CG, O.PCG, O.BICGSTAB]
 
