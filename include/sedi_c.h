@@ -94,8 +94,8 @@ int sedi_solve(sedi_solver *solver, const double *b, double *x,
 
 /* The err values that the reference prints after each iteration
  * ("Cg: after iter=%d, err=%.10e", Cg.hpp:192; PipelinedCg.hpp:228;
- * BiCgStab.hpp:228) for the last solve.  Copies min(cap, count) values and
- * returns count. */
+ * BiCgStab.hpp:228) for the last solve.  Copies min(cap, count) values
+ * (none if out is NULL) and returns count. */
 int sedi_solver_err_history(sedi_solver *solver, double *out, int cap);
 
 void sedi_solver_destroy(sedi_solver *solver);
@@ -217,7 +217,8 @@ int sedi_batch_create(const sedi_grid *grid, const sedi_stencil *stencil, int ki
 int sedi_batch_solve(sedi_batch *batch, const double *b, int nrhs, double *x, int maxiter,
                      double tol, int *iters, double *errs, int *codes);
 /* The per-iteration error of right-hand side k in the last solve; returns the
- * number of entries (written up to cap), or a negative error code. */
+ * number of entries (written up to cap; none if out is NULL), or a negative
+ * error code. */
 int sedi_batch_err_history(sedi_batch *batch, int k, double *out, int cap);
 void sedi_batch_destroy(sedi_batch *batch);
 

@@ -238,7 +238,8 @@ int sedi_batch_err_history(sedi_batch* h, int k, double* out, int cap) {
         const std::vector<double> hist = h->impl.history(k);
         const int code = h->last.codes[k];
         const int n = (code == 0 || code == 1) ? std::min<int>(h->last.iters[k], (int)hist.size()) : 0;
-        for (int i = 0; i < n && i < cap; i++) out[i] = hist[i];
+        if (out)
+            for (int i = 0; i < n && i < cap; i++) out[i] = hist[i];
         return n;
     });
 }
