@@ -223,12 +223,21 @@ inline unsigned ew_blocks(long long n) {
      : (geo).tg.zc == 32 ? launch(name, K<TT, 32>, grid, dim3(TX * (geo).tg.ty), __VA_ARGS__)    \
                          : launch(name, K<TT, 16>, grid, dim3(TX * (geo).tg.ty), __VA_ARGS__))
 
-// Tile height of the batched CG passes: tall tiles in FP32 storage, where
-// the registers allow it; FP64 blocks keep 4 rows.
+// Tile heights of the batched CG passes: the direction pass takes 8 rows in
+// FP32 storage (16 measured no faster); the update pass 8 or 16 rows by grid
+// (batched.hip, upd_rows).  FP64 blocks, whose registers do not fit two
+// 8-wave workgroups per CU, keep 4 rows.
 template <class T>
 struct CgTile {
-    static constexpr int ty = sizeof(T) == 4 ? TY_TALL : TY;
     static constexpr int min_waves = sizeof(T) == 4 ? 4 : 1;
+};
+#ifndef SEDI_CG_DIR_TY
+#define SEDI_CG_DIR_TY TY_TALL
+#endif
+template <class T>
+struct CgDirTile {
+    static constexpr int ty = sizeof(T) == 4 ? SEDI_CG_DIR_TY : TY;
+    static constexpr int min_waves = CgTile<T>::min_waves;
 };
 
 // Vector slots of a batch (each [K/4][N] of V4<T>).

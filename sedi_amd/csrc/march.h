@@ -8,8 +8,9 @@
 //
 // Tiling (gfx950, wave64): a workgroup has one wave per tile row; a tile is
 // TX=64 cells along x (one wave row, 512 contiguous bytes per FP64 vector) by
-// TYK rows along y (4, or 8 for the passes whose registers allow two such
-// workgroups per CU), and marches over a chunk of ZC planes along z.  For
+// TYK rows along y (4; 8 for the passes whose registers allow two such
+// workgroups per CU; 16 for the standalone stencil and, on large grids, the
+// batched CG update pass), and marches over a chunk of ZC planes along z.  For
 // each plane it evaluates f at its cells plus the 2*64 + 2*TYK in-plane halo
 // cells (lanes 0..31 of the first four waves take half a y-halo row, lanes
 // 32/33 of every wave the two x-halo cells of its row), stores them in a
@@ -145,12 +146,16 @@ __device__ __forceinline__ void block_sum(double (&acc)[ND], double* red /* LDS 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        // wave by wave, all ND sums at once: each sum keeps its order
+        // (wave 0, 1, ..., NW-1) while at most 2*ND values are live (a fully
+        // unrolled loop loads all ND*NW at once, which sets the register
+        // peak of kernels with many dot products)
 #pragma unroll
-        for (int d = 0; d < ND; d++) {
-            double t = red[d * NW];
+        for (int d = 0; d < ND; d++) acc[d] = red[d * NW];
+#pragma unroll 1
+        for (int v = 1; v < NW; v++) {
 #pragma unroll
-            for (int v = 1; v < NW; v++) t += red[d * NW + v];
-            acc[d] = t;
+            for (int d = 0; d < ND; d++) acc[d] += red[d * NW + v];
         }
     }
 }
