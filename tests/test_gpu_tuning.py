@@ -2,9 +2,12 @@
 
 The tuning table (sedi_amd/tuning.txt) sets per kernel and grid the planes per
 tile (8, 16, 32) and the schedule (0: resident workgroups walk XCD-contiguous
-runs of tiles; 1: tiles in launch order, more workgroups).  The test grids have
-no entries of their own, so here a child process forces one plan on every
-kernel (SEDI_TUNING_FILE, read once per process) and the results are checked
+runs of tiles; 1: tiles in launch order, more workgroups), and the rows per
+tile of the FP32 batched CG update pass (8, or 16 by default from 256 rows up).
+The test grids have no entries of their own, so here a child process forces
+one plan on every kernel (SEDI_TUNING_FILE, read once per process; one plan
+takes 16-row update tiles on the batched grid of 20 rows, a full tile and a
+partial one) and the results are checked
 against the reference fixtures with the bars of the parity suite: stencil
 bit-exact, plain estimates within +-1 iteration and 1e-10 (1e-8 when the
 counts differ; BiCGStab 20% and 1e-6), fixed-iteration runs exact in counts
@@ -48,7 +51,7 @@ for kind in (1, 2, 3):
 print(json.dumps(out))
 """
 
-PLANS = [(8, 1), (32, 0), (16, 1)]
+PLANS = [(8, 1, 0), (32, 0, 16), (16, 1, 0)]  # z chunk, schedule, update-pass rows (0: default)
 
 
 @pytest.fixture(scope="module")
@@ -59,13 +62,14 @@ def batched_default():
                              precision=sedi.FP32_MIXED, batch_k=8)["D"] for k in (1, 2, 3)}
 
 
-@pytest.mark.parametrize("zc,sched", PLANS)
-def test_forced_launch_plan_keeps_parity(tmp_path, ref32, batched_default, zc, sched):
+@pytest.mark.parametrize("zc,sched,ty", PLANS)
+def test_forced_launch_plan_keeps_parity(tmp_path, ref32, batched_default, zc, sched, ty):
     table = tmp_path / "tuning.txt"
-    table.write_text("zc:* 0 0 0 %d\nsched:* 0 0 0 %d\n" % (zc, sched))
+    table.write_text("zc:* 0 0 0 %d\nsched:* 0 0 0 %d\n" % (zc, sched) +
+                     ("ty:batch_upd_kernel 0 0 0 %d\n" % ty if ty else ""))
     env = dict(os.environ, SEDI_TUNING_FILE=str(table))
     code = CHILD % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), tmp=str(tmp_path))
-    print("plan zc=%d sched=%d" % (zc, sched), flush=True)
+    print("plan zc=%d sched=%d ty=%d" % (zc, sched, ty), flush=True)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
