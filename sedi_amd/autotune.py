@@ -43,8 +43,8 @@ BATCH_KERNELS = {2: ["batch_dir_kernel", "batch_upd_kernel"], 3: ["batch_pcg_ite
                  1: ["batch_bicg_dir_kernel", "batch_bicg_half_kernel", "batch_bicg_upd_kernel"]}
 # device bytes per right-hand side and cell of a batch (FP32 storage + FP64
 # solution for the mixed form), for sizing k to the memory
-BYTES_MIXED = {2: 28, 3: 48, 1: 44}
-BYTES_FP64 = {2: 40, 3: 80, 1: 72}
+BYTES_MIXED = {2: 32, 3: 48, 1: 44}
+BYTES_FP64 = {2: 48, 3: 80, 1: 72}
 K_CHOICES = (8, 16, 32, 64)
 
 

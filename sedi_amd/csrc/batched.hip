@@ -163,7 +163,8 @@ struct PassBDir {
 
 template <class T, int ZC>
 __global__ __launch_bounds__(TX * CgDirTile<T>::ty, CgDirTile<T>::min_waves) void batch_dir_kernel(
-    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r, V4<T>* p0, V4<T>* p1, V4<T>* x, T im) {
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r0, const V4<T>* r1, V4<T>* p0, V4<T>* p1,
+    V4<T>* x, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     constexpr int TYK = CgDirTile<T>::ty;
@@ -183,8 +184,8 @@ __global__ __launch_bounds__(TX * CgDirTile<T>::ty, CgDirTile<T>::min_waves) voi
         flush[q] = d ? (T)c.apend[4 * rb + q] : (T)0;
         any_flush |= flush[q] != (T)0;
     }
-    PassBDir<T> P{a, b, cc, r + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off, x + off, im,
-                  beta, flush, it == 1, any_flush};
+    PassBDir<T> P{a, b, cc, ((it & 1) ? r1 : r0) + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off,
+                  x + off, im, beta, flush, it == 1, any_flush};
     double acc[4];
     run_tiles<PassBDir<T>, ZC, TYK>(P, g, lds, acc, false);
     if (!grid_sum<4, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -206,7 +207,9 @@ struct PassBUpd {
     const V* p;
     const V* pprev;  // the previous iteration's direction (even iterations)
     V* xlo;
-    V* r;
+    const V* r;
+    V* rnext;  // the next iteration's residual (the other buffer: stores stream
+               // to fresh lines instead of back into the lines being read)
     T im;
     V alpha, apend;
     bool xstep;  // even iteration: x += apend p_prev + alpha p
@@ -227,7 +230,7 @@ struct PassBUpd {
                            double (&acc)[ND]) const {
         if (xstep) stv(xlo, i, (o.x + apend * o.pp) + alpha * pv);
         const V rn = o.r - alpha * s;
-        stv(r, i, rn);
+        stv(rnext, i, rn);
         const V u = im * rn;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -240,7 +243,7 @@ struct PassBUpd {
 template <class T, int ZC, int TYK>
 __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kernel(
     TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* p0, const V4<T>* p1, V4<T>* xlo,
-    V4<T>* r, T im) {
+    V4<T>* r0, V4<T>* r1, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<T> lds[2][TYK + 2][TX + 2];
@@ -255,8 +258,8 @@ __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kern
         apend[q] = (T)c.apend[4 * rb + q];
     }
     const bool xstep = (it & 1) == 0;
-    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, ((it & 1) ? p0 : p1) + off, xlo + off, r + off,
-                  im, alpha, apend, xstep};
+    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, ((it & 1) ? p0 : p1) + off, xlo + off,
+                  ((it & 1) ? r1 : r0) + off, ((it & 1) ? r0 : r1) + off, im, alpha, apend, xstep};
     double acc[8];
     run_tiles<PassBUpd<T>, ZC, TYK>(P, g, lds, acc, true);
     if (!grid_sum<8, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -378,8 +381,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     __shared__ double red[8 * NWAVES];
     __shared__ int lastflag;
     const long long off = (long long)rb * n;
-    // the residual the next iteration reads: CG has one, BiCGStab alternates
-    // with the parity of the iteration (r of iteration it+1 = iter[rb])
+    // the residual the next iteration reads: it alternates with the parity of
+    // the iteration (r of iteration it+1 = iter[rb]: r1 when odd)
     V4<T>* r = (r1 && (c.iter[rb] & 1)) ? r1 : r0;
     PassBRep<T, RHAT> P;
     P.a = a; P.b = b; P.c = cc;
@@ -589,13 +592,13 @@ int fold_only(const BRun& r) {
     return SEDI_OK;
 }
 
-// CG: res is r and rhat is null.  BiCGStab: res is r0 and rhat r^; the kernel
-// picks r0 / r1 by the parity of the next iteration.
+// CG: res is r0 and rhat is null.  BiCGStab: res is r0 and rhat r^.  The
+// kernel picks r0 / r1 by the parity of the next iteration.
 template <class T>
 int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
     hipError_t e;
     if (rhat) e = launch_rep<T, true>(r, res, static_cast<V4<T>*>(r.v[BI_R1]), rhat);
-    else e = launch_rep<T, false>(r, res, nullptr, nullptr);
+    else e = launch_rep<T, false>(r, res, static_cast<V4<T>*>(r.v[CG_R1]), nullptr);
     if (e != hipSuccess) {
         set_error("replacement launch failed: %s", hipGetErrorString(e));
         return SEDI_ERR_HIP;
@@ -607,12 +610,13 @@ int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
 // kernel for the geometry's z chunk.
 template <class T, int TYK>
 static hipError_t launch_upd(const BRun& r, T a, T b, T cc, T im, const V4<T>* P0, const V4<T>* P1, V4<T>* XL,
-                             V4<T>* R) {
+                             V4<T>* R0, V4<T>* R1) {
     const KGeom g = bgeom("batch_upd_kernel", batch_upd_kernel<T, 16, TYK>, r.shape, SEDI_PREFETCH_BATCH, TYK);
     const dim3 grid(g.nblocks, r.nrb), block(TX * TYK);
     auto kern = g.tg.zc == 8 ? batch_upd_kernel<T, 8, TYK>
                              : (g.tg.zc == 32 ? batch_upd_kernel<T, 32, TYK> : batch_upd_kernel<T, 16, TYK>);
-    return launch("batch_upd_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, P0, P1, XL, R, im);
+    return launch("batch_upd_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, P0, P1, XL, R0, R1,
+                  im);
 }
 
 // Rows per tile of the update pass in FP32 storage: 16 (one 16-wave
@@ -629,22 +633,23 @@ static int upd_rows(const Shape& s) {
 template <class T>
 static int cg_iteration(const BRun& r) {
     const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
-    V4<T>* R = static_cast<V4<T>*>(r.v[CG_R]);
+    V4<T>* R0 = static_cast<V4<T>*>(r.v[CG_R]);
+    V4<T>* R1 = static_cast<V4<T>*>(r.v[CG_R1]);
     V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
     V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
     V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
     const KGeom g1 = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgDirTile<T>::ty);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
-                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R, P0, P1, XL, im));
+                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R0, (const V4<T>*)R1, P0, P1, XL, im));
     if constexpr (sizeof(T) == 4) {
         if (upd_rows(r.shape) == 2 * TY_TALL)
-            SEDI_HIP((launch_upd<T, 2 * TY_TALL>(r, a, b, cc, im, P0, P1, XL, R)));
+            SEDI_HIP((launch_upd<T, 2 * TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
         else
-            SEDI_HIP((launch_upd<T, TY_TALL>(r, a, b, cc, im, P0, P1, XL, R)));
+            SEDI_HIP((launch_upd<T, TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
     } else {
-        SEDI_HIP((launch_upd<T, TY>(r, a, b, cc, im, P0, P1, XL, R)));
+        SEDI_HIP((launch_upd<T, TY>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
     }
-    if (r.mixed) return replace_and_fold<T>(r, R, nullptr, true);
+    if (r.mixed) return replace_and_fold<T>(r, R0, nullptr, true);
     return SEDI_OK;
 }
 
@@ -793,8 +798,8 @@ int BatchSolver::start_t(int rb0, int count, double tol, int maxiter) {
     const T im = (T)(1.0 / a_);
     V4<T>* B = static_cast<V4<T>*>(v_[SLOT_B]);
     V4<T>* XL = static_cast<V4<T>*>(v_[SLOT_XLO]);
-    // the first residual: r (CG, pipelined CG) or r1, read by iteration 1 (BiCGStab)
-    V4<T>* R = static_cast<V4<T>*>(kind_ == SEDI_BICGSTAB ? v_[BI_R1] : v_[CG_R]);
+    // the first residual, read by iteration 1: r1 (CG, BiCGStab) or r (pipelined CG)
+    V4<T>* R = static_cast<V4<T>*>(kind_ == SEDI_BICGSTAB ? v_[BI_R1] : (kind_ == SEDI_CG ? v_[CG_R1] : v_[PCG_R]));
     V4<T>* RH = kind_ == SEDI_BICGSTAB ? static_cast<V4<T>*>(v_[BI_RH]) : nullptr;
     SEDI_HIP(launch("batch_init_kernel", batch_init_kernel<T>, dim3(ew_blocks(n), count), dim3(NTHREADS),
                     stream_, n, run.c, rb0, tol, maxiter, (const V4<T>*)B, R, RH, XL, xhi_, im));

@@ -242,8 +242,9 @@ struct CgDirTile {
 
 // Vector slots of a batch (each [K/4][N] of V4<T>).
 enum { SLOT_B = 0, SLOT_XLO = 1, SLOT_K0 = 2 };
-// CG: r, p0, p1
-enum { CG_R = SLOT_K0, CG_P0, CG_P1, CG_NVEC };
+// CG: r0, p0, p1, r1 (the residual of iteration it is in r1 when it is odd,
+// in r0 when even: the update pass writes its successor to the other buffer)
+enum { CG_R = SLOT_K0, CG_P0, CG_P1, CG_R1, CG_NVEC };
 // pipelined CG: r, u, w0, w1, z, q, s, p
 enum { PCG_R = SLOT_K0, PCG_U, PCG_W0, PCG_W1, PCG_Z, PCG_Q, PCG_S, PCG_P, PCG_NVEC };
 // BiCGStab: r0, r1, r^, p0, p1, v0, v1
