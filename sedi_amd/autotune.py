@@ -17,7 +17,13 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
 2. batch size: with those chunks, the batched estimator runs k = 8..64 probes
    (as memory allows) for a fixed number of iterations; each (solver, precision,
    grid) gets the k of the highest probe-iterations per second.  In FP64 the
-   plain solver (k = 1) competes too.
+   plain solver (k = 1) competes too, over at least PLAIN_ITERS iterations: it
+   generates each probe sequentially on a side stream while the previous probe
+   solves, which real solves hide but short timing runs would not (at 256^3 a
+   probe takes ~48 ms, 240 plain iterations).
+
+``--k-only`` re-measures the batch sizes alone, with the chunk and schedule
+entries of the existing table (--out), which it keeps.
 
 The table is data: a run's results depend only on it and the grid, so the
 same table gives the same results on every run.
@@ -46,6 +52,7 @@ BATCH_KERNELS = {2: ["batch_dir_kernel", "batch_upd_kernel"], 3: ["batch_pcg_ite
 BYTES_MIXED = {2: 32, 3: 48, 1: 44}
 BYTES_FP64 = {2: 48, 3: 80, 1: 72}
 K_CHOICES = (8, 16, 32, 64)
+PLAIN_ITERS = 600
 
 
 def device_bytes():
@@ -134,7 +141,8 @@ def measure_batches(sides, iters):
                     run_est(side, kind, max(k, 4), 2, prec, k)
                     best = 0.0
                     for _ in range(2):
-                        secs, its = run_est(side, kind, max(k, 4), iters, prec, k)
+                        secs, its = run_est(side, kind, max(k, 4), max(iters, PLAIN_ITERS) if k == 1 else iters,
+                                            prec, k)
                         best = max(best, its / secs)
                     rates[k] = best
                 if rates:
@@ -169,6 +177,8 @@ def main():
                    help="iterations per batch-size timing (long enough that probe generation, which the "
                         "plain FP64 path does one probe at a time, is a small part)")
     p.add_argument("--out", default=os.path.join(PKG, "tuning.txt"))
+    p.add_argument("--k-only", action="store_true",
+                   help="re-measure the batch sizes only, keeping the table's chunk and schedule entries")
     p.add_argument("--child", choices=["chunks", "batches"], help=argparse.SUPPRESS)
     args = p.parse_args()
     if args.child == "chunks":
@@ -179,6 +189,23 @@ def main():
         return
 
     t0 = time.time()
+    if args.k_only:
+        with open(args.out) as f:
+            kept = [l.rstrip("\n") for l in f if l.strip() and not l.startswith("k:")]
+        head = [l for l in kept if l.startswith("#")]
+        lines = [l for l in kept if not l.startswith("#")]
+        print("batch sizes", file=sys.stderr, flush=True)
+        batches = child(args, [l.split("#")[0] for l in lines], "batches")
+        for b in batches:
+            s = b["side"]
+            lines.append("k:%d:%d %d %d %d %d   # probe-iterations/s: %s" % (
+                b["kind"], b["precision"], s, s, s, b["k"], " ".join("%s:%s" % kv for kv in b["rates"].items())))
+        head.append("# k entries re-measured by --k-only --k-iters %d; %s; %.0f s" % (
+            args.k_iters, time.strftime("%Y-%m-%d"), time.time() - t0))
+        with open(args.out, "w") as f:
+            f.write("\n".join(head + lines) + "\n")
+        print("wrote %s (%d entries)" % (args.out, len(lines)), file=sys.stderr, flush=True)
+        return
     timings = {}
     for zc in (8, 16, 32):
         for sched in (0, 1):
