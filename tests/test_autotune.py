@@ -1,0 +1,76 @@
+"""CPU: the autotuner's table writing, with the GPU measurements replaced by
+fixed timings: per kernel and grid the (chunk, schedule) of least launch time,
+per solver/precision/grid the batch size of most probe-iterations per second,
+and --k-only re-measuring the batch sizes while keeping every other entry."""
+import sys
+
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, ROOT)
+autotune = pytest.importorskip("sedi_amd.autotune")
+
+# launch times (us) per (chunk, schedule), grid side and kernel
+TIMES = {
+    (8, 0): {"64": {"batch_upd_kernel": 70.0, "cg_update_kernel": 9.0}},
+    (8, 1): {"64": {"batch_upd_kernel": 69.0, "cg_update_kernel": 12.0}},
+    (16, 0): {"64": {"batch_upd_kernel": 60.0, "cg_update_kernel": 10.0}},
+    (16, 1): {"64": {"batch_upd_kernel": 61.0, "cg_update_kernel": 11.0}},
+    (32, 0): {"64": {"batch_upd_kernel": 65.0, "cg_update_kernel": 13.0}},
+    (32, 1): {"64": {"batch_upd_kernel": 58.5, "cg_update_kernel": 14.0}},
+}
+BATCHES = [dict(side=64, kind=2, precision=1, k=32, rates={"8": 1.0, "32": 3.0, "64": 2.0}),
+           dict(side=64, kind=2, precision=0, k=1, rates={"1": 5.0, "8": 4.0})]
+
+
+def fake_child(calls):
+    def child(args, table_lines, mode):
+        calls.append((mode, list(table_lines)))
+        if mode == "batches":
+            return BATCHES
+        zc = int(table_lines[0].split()[-1])
+        sched = int(table_lines[1].split()[-1])
+        return TIMES[(zc, sched)]
+    return child
+
+
+def entries(path):
+    out = {}
+    for line in open(path):
+        line = line.split("#")[0].split()
+        if line:
+            out[(line[0], int(line[1]))] = int(line[4])
+    return out
+
+
+def test_table_takes_fastest_plan_and_best_batch(tmp_path, monkeypatch):
+    calls = []
+    monkeypatch.setattr(autotune, "child", fake_child(calls))
+    out = tmp_path / "tuning.txt"
+    monkeypatch.setattr(sys, "argv", ["autotune", "--sides", "64", "--out", str(out)])
+    autotune.main()
+    e = entries(out)
+    assert e[("zc:batch_upd_kernel", 64)] == 32 and e[("sched:batch_upd_kernel", 64)] == 1
+    assert e[("zc:cg_update_kernel", 64)] == 8 and e[("sched:cg_update_kernel", 64)] == 0
+    assert e[("k:2:1", 64)] == 32 and e[("k:2:0", 64)] == 1
+    # six plans measured, then the batch sizes under the chosen plans
+    assert [c[0] for c in calls] == ["chunks"] * 6 + ["batches"]
+    assert "zc:batch_upd_kernel 64 64 64 32" in [l.strip() for l in calls[-1][1]]
+
+
+def test_k_only_keeps_plans(tmp_path, monkeypatch):
+    out = tmp_path / "tuning.txt"
+    out.write_text("# header\nzc:batch_upd_kernel 64 64 64 16   # note\nsched:batch_upd_kernel 64 64 64 0\n"
+                   "k:2:1 64 64 64 8   # old\n")
+    calls = []
+    monkeypatch.setattr(autotune, "child", fake_child(calls))
+    monkeypatch.setattr(sys, "argv", ["autotune", "--sides", "64", "--k-only", "--out", str(out)])
+    autotune.main()
+    e = entries(out)
+    assert e[("zc:batch_upd_kernel", 64)] == 16 and e[("sched:batch_upd_kernel", 64)] == 0
+    assert e[("k:2:1", 64)] == 32 and e[("k:2:0", 64)] == 1
+    assert [c[0] for c in calls] == ["batches"]
+    text = out.read_text()
+    assert text.startswith("# header\n") and "k entries re-measured by --k-only" in text
+    assert text.count("k:2:1 ") == 1
