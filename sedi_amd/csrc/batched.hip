@@ -876,7 +876,11 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
             break;
         }
         if (pending) {
-            (void)hipEventSynchronize(ev[slot ^ 1]);
+            if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
+                set_error("batched status wait failed: %s", hipGetErrorString(hipGetLastError()));
+                rc = SEDI_ERR_HIP;
+                break;
+            }
             bool all = true;
             for (int q = 0; q < nrb_; q++) all = all && pinned_[(slot ^ 1) * nrb_ + q];
             if (all) break;
@@ -884,11 +888,15 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
         pending = true;
         slot ^= 1;
     }
-    (void)hipStreamSynchronize(stream_);
+    const hipError_t se = hipStreamSynchronize(stream_);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     profile_flush();
     if (rc) return rc;
+    if (se != hipSuccess) {
+        set_error("batched solve failed: %s", hipGetErrorString(se));
+        return SEDI_ERR_HIP;
+    }
     if (res) {
         res->iters.assign(K, 0);
         res->codes.assign(K, 0);
