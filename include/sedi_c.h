@@ -235,8 +235,10 @@ int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double *v
  * are mid-solve, and pools of different grids run on their own streams.  Jobs
  * are admitted in arrival order against a device memory budget.  A job's D
  * is accumulated in probe order and equals sedi_estimate's batched result for
- * the same job bitwise, whatever else is in flight.  Thread-safe submit and
- * wait; one engine thread owns the device. */
+ * the same job with ignore_errors = 1 bitwise, whatever else is in flight:
+ * like the reference's -i, every probe is accumulated whether or not its
+ * solve converged, and the per-probe codes tell which did not.  Thread-safe
+ * submit and wait; one engine thread owns the device. */
 typedef struct sedi_server sedi_server;
 typedef struct sedi_server_config {
     double hbm_budget_bytes; /* device memory for pools and jobs             */
@@ -260,7 +262,9 @@ int sedi_server_create(const sedi_server_config *cfg, sedi_server **out);
 int64_t sedi_server_submit(sedi_server *srv, const sedi_job *job);
 /* Waits for a job (timeout_s < 0: no limit).  Returns 0 when it finished
  * (then *status is 0, or a negative error code if the job failed, and its
- * result is released), 1 on timeout, < 0 on a bad call.  D (n cells),
+ * result is released; a probe that reached maxiter or broke down does not
+ * fail the job, its code does not change *status), 1 on timeout, < 0 on a
+ * bad call.  D (n cells),
  * iters/errs/codes (per probe) and times4 = {submitted, admitted, first
  * block started, done} in seconds since the server started may be NULL. */
 int sedi_server_wait(sedi_server *srv, int64_t job, double timeout_s, double *D, int *iters,

@@ -63,6 +63,13 @@ int check_grid(const sedi_grid* g, sedi::Shape* s) {
     return SEDI_OK;
 }
 
+// The plain FP64 kernels index each vector with 32-bit byte offsets.
+int check_plain_size(const sedi::Shape& s) {
+    SEDI_REQUIRE(sedi::offsets_fit_32(s.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
+                 "grid of %lld cells too large: an FP64 vector must be < 4 GiB (32-bit offsets)", s.n());
+    return SEDI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -73,6 +80,7 @@ int sedi_stencil_apply(const sedi_grid* grid, const sedi_stencil* st, const doub
         int rc = check_grid(grid, &s);
         if (rc) return rc;
         SEDI_REQUIRE(st && x && y, SEDI_ERR_ARG, "stencil, x and y must not be NULL");
+        if ((rc = check_plain_size(s))) return rc;
         if ((rc = sedi::ensure_device())) return rc;
         const size_t bytes = sizeof(double) * (size_t)s.n();
         sedi::DevBuf xs, ys;
@@ -104,6 +112,7 @@ int sedi_solver_create(const sedi_grid* grid, const sedi_stencil* st, int kind, 
         sedi::Shape s;
         int rc = check_grid(grid, &s);
         if (rc) return rc;
+        if ((rc = check_plain_size(s))) return rc;
         if ((rc = sedi::ensure_device())) return rc;
         sedi_solver* h = new sedi_solver();
         rc = h->impl.create(s, st->a, st->b, st->c, kind, mode);
@@ -408,6 +417,13 @@ int sedi_estimator_create(const sedi_estimate_args* a, sedi_estimator** out) {
     return guarded([&]() -> int {
         SEDI_REQUIRE(a && out, SEDI_ERR_ARG, "args and out must not be NULL");
         *out = nullptr;
+        if (a->precision == SEDI_FP64 && a->batch_k == 1) {
+            // the plain path, chosen explicitly
+            sedi::Shape s;
+            int rc = check_grid(&a->grid, &s);
+            if (rc) return rc;
+            if ((rc = check_plain_size(s))) return rc;
+        }
         int rc = sedi::ensure_device();
         if (rc) return rc;
         sedi_estimator* h = new sedi_estimator();

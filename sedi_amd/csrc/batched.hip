@@ -876,8 +876,9 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
             break;
         }
         if (pending) {
-            if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
-                set_error("batched status wait failed: %s", hipGetErrorString(hipGetLastError()));
+            const hipError_t we = hipEventSynchronize(ev[slot ^ 1]);
+            if (we != hipSuccess) {
+                set_error("batched status wait failed: %s", hipGetErrorString(we));
                 rc = SEDI_ERR_HIP;
                 break;
             }

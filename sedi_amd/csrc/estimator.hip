@@ -108,6 +108,9 @@ int Estimator::create(const sedi_estimate_args& a) {
         q_ = t_ + n;
         return SEDI_OK;
     }
+    SEDI_REQUIRE(offsets_fit_32(shape_.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
+                 "grid of %lld cells too large for the plain FP64 path (a vector must be < 4 GiB); "
+                 "the batched path takes grids up to 4 GiB per block of 4 FP32 probes", shape_.n());
     int rc = solver_.create(shape_, a.stencil.a, a.stencil.b, a.stencil.c, a.kind, a.mode);
     if (rc) return rc;
     SEDI_HIP(buf_.reserve(sizeof(double) * n * 5));

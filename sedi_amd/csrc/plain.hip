@@ -654,6 +654,8 @@ static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, 
 
 int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
                       hipStream_t stream) {
+    SEDI_REQUIRE(offsets_fit_32(s.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
+                 "grid of %lld cells too large: an FP64 vector must be < 4 GiB (32-bit offsets)", s.n());
     const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY);
     SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel, k, stream, k.tg, a, b, c, x, y));
     return SEDI_OK;
@@ -672,6 +674,8 @@ int PlainSolver::create(const Shape& s, double a, double b, double c, int kind, 
     SEDI_REQUIRE(s.n1 > 0 && s.n2 > 0 && s.n3 > 0, SEDI_ERR_ARG, "grid dimensions must be positive");
     SEDI_REQUIRE(kind == SEDI_CG || kind == SEDI_PCG || kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
                  "unknown solver kind %d", kind);
+    SEDI_REQUIRE(offsets_fit_32(s.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
+                 "grid of %lld cells too large: an FP64 vector must be < 4 GiB (32-bit offsets)", s.n());
     shape_ = s;
     a_ = a; b_ = b; c_ = c;
     kind_ = kind;

@@ -38,6 +38,13 @@ struct Shape {
     long long n() const { return (long long)n1 * n2 * n3; }
 };
 
+// The kernels address a vector with 32-bit byte offsets from its base, so
+// every vector they touch must be smaller than 4 GiB: `cells` values of
+// `bytes_per_cell` bytes each.
+inline bool offsets_fit_32(long long cells, double bytes_per_cell) {
+    return (double)cells * bytes_per_cell < 4294967296.0;
+}
+
 // Inverse diagonal of the Jacobi preconditioner.  The estimator's is the
 // constant 1/a (precond.all_cells.fill(1.0/stencil.a),
 // stochastic-estimator.cc:102-103), which kernels instantiated with

@@ -252,7 +252,17 @@ int Server::engine_step(bool* progress) {
         }
     }
     for (auto& p : pools_) {
-        while (p->inflight > 0 && hipEventQuery(p->ev[p->head]) == hipSuccess) {
+        while (p->inflight > 0) {
+            // not ready: poll again on the next step; any other result is
+            // an error that no later query would report (two chunks are
+            // already in flight, so nothing new is launched), so the engine
+            // stops and fails every job
+            const hipError_t q = hipEventQuery(p->ev[p->head]);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) {
+                set_error("serving pool chunk failed: %s", hipGetErrorString(q));
+                return SEDI_ERR_HIP;
+            }
             if ((rc = harvest(*p))) return rc;
             p->head ^= 1;
             p->inflight--;
@@ -499,6 +509,8 @@ int Server::complete(Job& j) {
     j.tq.release();
     std::lock_guard<std::mutex> lk(mu_);
     bytes_ -= j.bytes;
+    // every probe was accumulated, converged or not (the reference's -i);
+    // the per-probe codes carry the solver outcomes
     j.res.status = 0;
     j.res.t_done = now();
     j.done = true;

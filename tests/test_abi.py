@@ -49,6 +49,63 @@ def test_library_reports_errors_without_gpu_work():
     assert lib.sedi_version() == 1
 
 
+class Grid(ctypes.Structure):
+    _fields_ = [("n1", ctypes.c_int64), ("n2", ctypes.c_int64), ("n3", ctypes.c_int64)]
+
+
+class Stencil(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double)]
+
+
+class EstimateArgs(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("stencil", Stencil), ("kind", ctypes.c_int), ("mode", ctypes.c_int),
+                ("samples", ctypes.c_int), ("maxiter", ctypes.c_int), ("tol", ctypes.c_double),
+                ("ignore_errors", ctypes.c_int), ("seed", ctypes.c_uint32), ("precision", ctypes.c_int),
+                ("batch_k", ctypes.c_int), ("rep_every", ctypes.c_int)]
+
+
+@pytest.mark.parametrize("dims", [(1024, 1024, 512), (1024, 1024, 1024), (2048, 2048, 128)])
+def test_plain_path_rejects_vectors_of_4gib(dims):
+    """The plain FP64 kernels use 32-bit byte offsets: a grid of 2^29 cells or
+    more (an FP64 vector of 4 GiB) is refused before any allocation or device
+    call, by the stencil, the solver and the plain estimator."""
+    lib = ctypes.CDLL(LIB)
+    lib.sedi_last_error.restype = ctypes.c_char_p
+    g = Grid(*dims)
+    st = Stencil(6.0, 1.0, 1.0)
+    x = (ctypes.c_double * 1)()
+    lib.sedi_stencil_apply.argtypes = [ctypes.c_void_p] * 4
+    assert lib.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), x, x) == -5
+    assert b"4 GiB" in lib.sedi_last_error()
+    out = ctypes.c_void_p()
+    lib.sedi_solver_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]
+    assert lib.sedi_solver_create(ctypes.byref(g), ctypes.byref(st), 2, 0, ctypes.byref(out)) == -5
+    assert not out.value
+    args = EstimateArgs(g, st, 2, 0, 4, 100, 1e-9, 0, 1, 0, 1, 0)
+    lib.sedi_estimator_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.sedi_estimator_create(ctypes.byref(args), ctypes.byref(out)) == -5
+    assert b"4 GiB" in lib.sedi_last_error()
+
+
+def test_plain_path_size_limit_is_exact():
+    """One cell below 2^29 passes the size check (and then needs a device)."""
+    lib = ctypes.CDLL(LIB)
+    lib.sedi_last_error.restype = ctypes.c_char_p
+    lib.sedi_solver_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]
+    st = Stencil(6.0, 1.0, 1.0)
+    out = ctypes.c_void_p()
+    rc = lib.sedi_solver_create(ctypes.byref(Grid(1 << 29, 1, 1)), ctypes.byref(st), 2, 0, ctypes.byref(out))
+    assert rc == -5
+    rc = lib.sedi_solver_create(ctypes.byref(Grid((1 << 29) - 1, 1, 1)), ctypes.byref(st), 2, 0,
+                                ctypes.byref(out))
+    assert rc != -5 and b"4 GiB" not in (lib.sedi_last_error() or b"")
+    if rc == 0:
+        lib.sedi_solver_destroy.argtypes = [ctypes.c_void_p]
+        lib.sedi_solver_destroy(out)
+
+
 def test_cli_usage_errors():
     if not os.path.exists(CLI):
         pytest.fail("sedi-estimator is not built")
