@@ -92,6 +92,24 @@ int sedi_solve(sedi_solver *solver, const double *b, double *x,
                const double *inv_diag, int maxiter, double tol,
                double *final_err, int *final_iter);
 
+/* No counterpart in the reference: an allocation-free sedi_solve for callers
+ * that cannot allocate device memory (code under HIP graph capture set-up,
+ * tight serving loops, harnesses that forbid hipMalloc).  b, x and inv_diag
+ * (or NULL for 1/a) must be device arrays; every temporary -- the solver's
+ * vectors, its control block and err history -- lives in one fixed device
+ * array of SEDI_NOALLOC_SCRATCH_BYTES that the library holds on each device
+ * it runs on until the process ends.  Same kernels, launch plan, result
+ * codes and final_err/final_iter as sedi_solver_create + sedi_solve on the
+ * same grid (so the same x, bitwise); no err history is kept.  A solve
+ * needing more (vectors: 3 for CG, 8 for pipelined CG, 7 for BiCGStab, each
+ * 8*n1*n2*n3 bytes, plus 8*maxiter bytes and ~50 KB) returns SEDI_ERR_ARG.
+ * Launches on the null stream and returns when the solve is done.  Calls on
+ * one device share the array: calls that overlap on one device race. */
+#define SEDI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sedi_solve_noalloc(const sedi_grid *grid, const sedi_stencil *stencil, int kind,
+                       const double *b, double *x, const double *inv_diag, int maxiter,
+                       double tol, double *final_err, int *final_iter);
+
 /* The err values that the reference prints after each iteration
  * ("Cg: after iter=%d, err=%.10e", Cg.hpp:192; PipelinedCg.hpp:228;
  * BiCgStab.hpp:228) for the last solve.  Copies min(cap, count) values

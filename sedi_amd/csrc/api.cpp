@@ -133,6 +133,31 @@ int sedi_solve(sedi_solver* h, const double* b, double* x, const double* inv_dia
     });
 }
 
+int sedi_solve_noalloc(const sedi_grid* grid, const sedi_stencil* st, int kind, const double* b, double* x,
+                       const double* inv_diag, int maxiter, double tol, double* final_err, int* final_iter) {
+    return guarded([&]() -> int {
+        sedi::Shape s;
+        int rc = check_grid(grid, &s);
+        if (rc) return rc;
+        SEDI_REQUIRE(st && b && x, SEDI_ERR_ARG, "stencil, b and x must not be NULL");
+        SEDI_REQUIRE(kind == SEDI_CG || kind == SEDI_PCG || kind == SEDI_BICGSTAB, SEDI_ERR_ARG,
+                     "unknown solver kind %d", kind);
+        SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
+        const size_t need = sedi::plain_noalloc_bytes(s, kind, maxiter);
+        SEDI_REQUIRE(need <= (size_t)SEDI_NOALLOC_SCRATCH_BYTES, SEDI_ERR_ARG,
+                     "solve needs %zu bytes of working memory, above the allocation-free limit of %zu "
+                     "(SEDI_NOALLOC_SCRATCH_BYTES); use sedi_solver_create/sedi_solve",
+                     need, (size_t)SEDI_NOALLOC_SCRATCH_BYTES);
+        if ((rc = sedi::ensure_device())) return rc;
+        // no staging: host arrays would need device copies
+        SEDI_REQUIRE(sedi::is_device_pointer(b) && sedi::is_device_pointer(x) &&
+                         (!inv_diag || sedi::is_device_pointer(inv_diag)),
+                     SEDI_ERR_ARG, "sedi_solve_noalloc takes device arrays only");
+        return sedi::plain_solve_noalloc(s, st->a, st->b, st->c, kind, b, x, inv_diag, maxiter, tol, final_err,
+                                         final_iter);
+    });
+}
+
 int sedi_solver_err_history(sedi_solver* h, double* out, int cap) {
     return guarded([&]() -> int {
         SEDI_REQUIRE(h, SEDI_ERR_ARG, "solver is NULL");

@@ -33,6 +33,38 @@ struct Ctl {
 int stencil_apply_dev(const Shape& s, double a, double b, double c, const double* x, double* y,
                       hipStream_t stream);
 
+// Where a solve's device working set lives and where it is launched: shape,
+// stencil and kind, the control block (state | arrival counter at +256 |
+// BiCGStab's vector table at +384 | partial sums at +512), the err history
+// and the vectors.  PlainSolver owns the memory behind one; the
+// allocation-free solve points one into a fixed device array.
+struct PlainCore {
+    Shape shape{};
+    double a = 6.0, b = 1.0, c = 1.0;
+    int kind = SEDI_CG;
+    hipStream_t stream = nullptr;
+    Ctl ctl{};
+    double* const* vv = nullptr;  // device table of v[0..6] (BiCGStab)
+    double* v[8] = {};
+};
+
+// Vectors a solver of `kind` keeps: CG r, p x2; pipelined CG r, u, w x2, z,
+// q, s, p; BiCGStab r x2, r^, p x2, v x2.
+inline int plain_vectors(int kind) { return kind == SEDI_CG ? 3 : (kind == SEDI_PCG ? 8 : 7); }
+
+// Bytes of the control block for `nblocks` workgroups (3 partial sums each).
+inline size_t plain_ctl_bytes(int nblocks) { return 512 + sizeof(double) * 3 * (size_t)nblocks; }
+
+// The allocation-free solve (sedi_solve_noalloc): b, x, iM device arrays,
+// every temporary in this library's fixed device array.  Launches on the
+// null stream and returns when the solve is done.
+int plain_solve_noalloc(const Shape& s, double a, double b, double c, int kind, const double* rhs, double* x,
+                        const double* iM, int maxiter, double tol, double* final_err, int* final_iter);
+// Bytes of that array a solve needs, and its size.
+size_t plain_noalloc_bytes(const Shape& s, int kind, int maxiter);
+// Looks up this device's address of the fixed array (once per device).
+int plain_noalloc_setup();
+
 class PlainSolver {
   public:
     ~PlainSolver();
@@ -48,20 +80,17 @@ class PlainSolver {
     const Shape& shape() const { return shape_; }
 
   private:
-    template <class PC> int launch_setup(PC pc, const double* b, double* x);
-    template <class PC> int launch_iterations(int count, PC pc, double* x);
     // count iterations: direct launches, or the replay of a captured graph
     int iterate(int count, const double* iM, double* x);
     Shape shape_{};
     double a_ = 6.0, b_ = 1.0, c_ = 1.0;
     int kind_ = 2, mode_ = 0;
-    int nblocks_max_ = 0;  // partials are sized for this many workgroups
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     DevState* pinned_ = nullptr;
     DevBuf ctl_, hist_, vec_;
     DevBuf stage_b_, stage_x_, stage_m_;
-    double* v_[10] = {};        // the solver's device vectors, named in create()
+    PlainCore core_;            // launch view of ctl_, hist_ and vec_
     std::vector<double> hist_host_;
     int nhist_ = 0;
     // captured chunk of iterations and what its launches were bound to

@@ -661,6 +661,78 @@ int stencil_apply_dev(const Shape& s, double a, double b, double c, const double
     return SEDI_OK;
 }
 
+// ---- launch sequences over a PlainCore -------------------------------------
+// Set-up of a solve: the residual r = b - A x with <b,b>, <r,r>, <r,u> (and
+// r^ = r for BiCGStab); pipelined CG's first u, w and dots.
+template <class PC>
+static int core_setup(const PlainCore& k, PC pc, const double* b, double* x) {
+    const KGeom& kr = geom_for("residual_kernel", residual_kernel<PC, 16>, k.shape, PassResidual<PC>::D);
+    if (k.kind == SEDI_BICGSTAB) {
+        // r_1 lives in r[1] (buffers alternate with the iteration's parity)
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, k.stream, kr.tg, k.ctl,
+                        k.a, k.b, k.c, (const double*)x, b, k.v[1], k.v[2], pc));
+    } else {
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, k.stream, kr.tg, k.ctl,
+                        k.a, k.b, k.c, (const double*)x, b, k.v[0], (double*)nullptr, pc));
+    }
+    if (k.kind == SEDI_PCG) {
+        const KGeom& kp = geom_for("pcg_init_kernel", pcg_init_kernel<PC, 16>, k.shape, PassPcgInit<PC>::D);
+        SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_init_kernel", pcg_init_kernel, PC, kp, k.stream, kp.tg, k.ctl,
+                        k.a, k.b, k.c, (const double*)k.v[0], k.v[1], k.v[2], k.v[4], k.v[5], k.v[6], k.v[7], pc));
+    }
+    return SEDI_OK;
+}
+
+// `count` iterations of the solver, launched directly.
+template <class PC>
+static int core_iterations(const PlainCore& k, int count, PC pc, double* x) {
+    for (int it = 0; it < count; it++) {
+        if (k.kind == SEDI_CG) {
+            const KGeom& g1 = geom_for("cg_direction_kernel", cg_direction_kernel<PC, 16>, k.shape, PassCgDirection<PC>::D);
+            const KGeom& g2 = geom_for("cg_update_kernel", cg_update_kernel<PC, 16>, k.shape, PassCgUpdate<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, k.stream, g1.tg, k.ctl, k.a, k.b, k.c, (const double*)k.v[0], k.v[1], k.v[2], pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, k.stream, g2.tg,
+                            k.ctl, k.a, k.b, k.c, (const double*)k.v[1], (const double*)k.v[2], x, k.v[0], pc));
+        } else if (k.kind == SEDI_PCG) {
+            const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, k.shape, PassPcgIter<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, k.stream, g1.tg,
+                            k.ctl, k.a, k.b, k.c, k.v[2], k.v[3], x, k.v[0], k.v[1], k.v[4], k.v[5], k.v[6], k.v[7], pc));
+        } else {
+            const KGeom& g1 = geom_for("bicgstab_direction_kernel", bicgstab_direction_kernel<PC, 16>, k.shape, PassBicgDirection<PC>::D);
+            const KGeom& g2 = geom_for("bicgstab_half_kernel", bicgstab_half_kernel<PC, 16>, k.shape, PassBicgHalf<PC>::D);
+            const KGeom& g3 = geom_for("bicgstab_update_kernel", bicgstab_update_kernel<PC, 16>, k.shape, PassBicgUpdate<PC>::D);
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_direction_kernel", bicgstab_direction_kernel, PC, g1, k.stream, g1.tg, k.ctl, k.a, k.b, k.c, k.vv, pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_half_kernel", bicgstab_half_kernel, PC, g2, k.stream, g2.tg, k.ctl, k.a, k.b, k.c, k.vv, pc));
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_update_kernel", bicgstab_update_kernel, PC, g3, k.stream, g3.tg, k.ctl, k.a, k.b, k.c, k.vv, x, pc));
+        }
+    }
+    return SEDI_OK;
+}
+
+// CG's pending solution update of a solve that ended at an odd iteration.
+static int core_flush(const PlainCore& k, double* x) {
+    if (k.kind != SEDI_CG) return SEDI_OK;
+    const long long n = k.shape.n();
+    const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4LL * num_cus());
+    SEDI_HIP(launch("cg_flush_kernel", cg_flush_kernel, dim3(std::max(1u, blocks)), dim3(256), k.stream,
+                    (const DevState*)k.ctl.st, n, x, (const double*)k.v[2]));
+    return SEDI_OK;
+}
+
+static int core_iterate_any(const PlainCore& k, int count, const double* iM, double* x) {
+    return iM ? core_iterations(k, count, PrecondVec{iM}, x) : core_iterations(k, count, PrecondScalar{1.0 / k.a}, x);
+}
+
+// Points a PlainCore's control block and vectors into `base` (device memory
+// laid out as plain_ctl_bytes(nblocks) | vectors).
+static void core_bind(PlainCore& k, char* base, int nblocks, double* hist) {
+    k.ctl = Ctl{reinterpret_cast<DevState*>(base), reinterpret_cast<unsigned*>(base + 256),
+                reinterpret_cast<double*>(base + 512), hist};
+    k.vv = reinterpret_cast<double* const*>(base + 384);
+    (void)nblocks;
+}
+
+// ---- PlainSolver -----------------------------------------------------------
 PlainSolver::~PlainSolver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_) (void)hipGraphExecDestroy(graph_);
@@ -680,76 +752,30 @@ int PlainSolver::create(const Shape& s, double a, double b, double c, int kind, 
     a_ = a; b_ = b; c_ = c;
     kind_ = kind;
     mode_ = mode;
-    nblocks_max_ = MAX_BLOCKS_PER_CU * num_cus();
+    const int nblocks_max = MAX_BLOCKS_PER_CU * num_cus();  // partials are sized for this many workgroups
     SEDI_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SEDI_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
     SEDI_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
     SEDI_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(DevState), hipHostMallocDefault));
-    // control block: state | counter | partials
-    const size_t ctl_bytes = 512 + sizeof(double) * 3 * (size_t)nblocks_max_;
+    const size_t ctl_bytes = plain_ctl_bytes(nblocks_max);
     SEDI_HIP(ctl_.reserve(ctl_bytes));
     SEDI_HIP(hipMemset(ctl_.p, 0, ctl_bytes));
-    const int nv = kind == SEDI_CG ? 3 : (kind == SEDI_PCG ? 8 : 7);
+    const int nv = plain_vectors(kind);
     const size_t n = (size_t)s.n();
     SEDI_HIP(vec_.reserve(nv * n * sizeof(double)));
     SEDI_HIP(hipMemset(vec_.p, 0, nv * n * sizeof(double)));
-    for (int k = 0; k < nv; k++) v_[k] = vec_.as<double>() + k * n;
+    core_ = PlainCore{};
+    core_.shape = s;
+    core_.a = a; core_.b = b; core_.c = c;
+    core_.kind = kind;
+    core_.stream = stream_;
+    core_bind(core_, static_cast<char*>(ctl_.p), nblocks_max, nullptr);
+    for (int k = 0; k < nv; k++) core_.v[k] = vec_.as<double>() + k * n;
     if (kind == SEDI_BICGSTAB) {
         // BiCGStab's vector table (r0 r1 r^ p0 p1 v0 v1) sits in the control
         // block after the counter; its kernels pick buffers by parity.
-        double* table[7] = {v_[0], v_[1], v_[2], v_[3], v_[4], v_[5], v_[6]};
+        double* table[7] = {core_.v[0], core_.v[1], core_.v[2], core_.v[3], core_.v[4], core_.v[5], core_.v[6]};
         SEDI_HIP(hipMemcpy(static_cast<char*>(ctl_.p) + 384, table, sizeof(table), hipMemcpyHostToDevice));
-    }
-    return SEDI_OK;
-}
-
-template <class PC>
-int PlainSolver::launch_setup(PC pc, const double* b, double* x) {
-    DevState* st = reinterpret_cast<DevState*>(ctl_.p);
-    Ctl ctl{st, reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
-            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    const KGeom& kr = geom_for("residual_kernel", residual_kernel<PC, 16>, shape_, PassResidual<PC>::D);
-    if (kind_ == SEDI_BICGSTAB) {
-        // r_1 lives in r[1] (buffers alternate with the iteration's parity)
-        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, stream_, kr.tg, ctl,
-                        a_, b_, c_, (const double*)x, b, v_[1], v_[2], pc));
-    } else {
-        SEDI_HIP(SEDI_ZC_LAUNCH_PC("residual_kernel", residual_kernel, PC, kr, stream_, kr.tg, ctl,
-                        a_, b_, c_, (const double*)x, b, v_[0], (double*)nullptr, pc));
-    }
-    if (kind_ == SEDI_PCG) {
-        const KGeom& kp = geom_for("pcg_init_kernel", pcg_init_kernel<PC, 16>, shape_, PassPcgInit<PC>::D);
-        SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_init_kernel", pcg_init_kernel, PC, kp, stream_, kp.tg, ctl,
-                        a_, b_, c_, (const double*)v_[0], v_[1], v_[2], v_[4], v_[5], v_[6], v_[7], pc));
-    }
-    return SEDI_OK;
-}
-
-template <class PC>
-int PlainSolver::launch_iterations(int count, PC pc, double* x) {
-    Ctl ctl{reinterpret_cast<DevState*>(ctl_.p),
-            reinterpret_cast<unsigned*>(static_cast<char*>(ctl_.p) + 256),
-            reinterpret_cast<double*>(static_cast<char*>(ctl_.p) + 512), hist_.as<double>()};
-    for (int k = 0; k < count; k++) {
-        if (kind_ == SEDI_CG) {
-            const KGeom& g1 = geom_for("cg_direction_kernel", cg_direction_kernel<PC, 16>, shape_, PassCgDirection<PC>::D);
-            const KGeom& g2 = geom_for("cg_update_kernel", cg_update_kernel<PC, 16>, shape_, PassCgUpdate<PC>::D);
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, (const double*)v_[0], v_[1], v_[2], pc));
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, stream_, g2.tg,
-                            ctl, a_, b_, c_, (const double*)v_[1], (const double*)v_[2], x, v_[0], pc));
-        } else if (kind_ == SEDI_PCG) {
-            const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, shape_, PassPcgIter<PC>::D);
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, stream_, g1.tg,
-                            ctl, a_, b_, c_, v_[2], v_[3], x, v_[0], v_[1], v_[4], v_[5], v_[6], v_[7], pc));
-        } else {
-            double* const* vv = reinterpret_cast<double* const*>(static_cast<char*>(ctl_.p) + 384);
-            const KGeom& g1 = geom_for("bicgstab_direction_kernel", bicgstab_direction_kernel<PC, 16>, shape_, PassBicgDirection<PC>::D);
-            const KGeom& g2 = geom_for("bicgstab_half_kernel", bicgstab_half_kernel<PC, 16>, shape_, PassBicgHalf<PC>::D);
-            const KGeom& g3 = geom_for("bicgstab_update_kernel", bicgstab_update_kernel<PC, 16>, shape_, PassBicgUpdate<PC>::D);
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_direction_kernel", bicgstab_direction_kernel, PC, g1, stream_, g1.tg, ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_half_kernel", bicgstab_half_kernel, PC, g2, stream_, g2.tg, ctl, a_, b_, c_, vv, pc));
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("bicgstab_update_kernel", bicgstab_update_kernel, PC, g3, stream_, g3.tg, ctl, a_, b_, c_, vv, x, pc));
-        }
     }
     return SEDI_OK;
 }
@@ -759,9 +785,7 @@ int PlainSolver::launch_iterations(int count, PC pc, double* x) {
 // Every per-iteration value lives in device memory, so the graph only depends
 // on the pointers its launches were given, which are checked here.
 int PlainSolver::iterate(int count, const double* iM, double* x) {
-    const PrecondScalar pcs{1.0 / a_};  // stochastic-estimator.cc:103
-    const PrecondVec pcv{iM};
-    auto direct = [&]() { return iM ? launch_iterations(count, pcv, x) : launch_iterations(count, pcs, x); };
+    auto direct = [&]() { return core_iterate_any(core_, count, iM, x); };
     bool& ready = geometry_ready_[iM ? 1 : 0];
     if (!graphs_enabled() || profile_any() || !ready) {
         const int rc = direct();
@@ -798,13 +822,12 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
                            double* final_err, int* final_iter) {
     SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
     SEDI_HIP(hist_.reserve(sizeof(double) * (size_t)std::max(maxiter, 1)));
-    DevState* st = reinterpret_cast<DevState*>(ctl_.p);
-    const PrecondScalar pcs{1.0 / a_};  // stochastic-estimator.cc:103
-    const PrecondVec pcv{iM};
+    core_.ctl.hist = hist_.as<double>();
+    DevState* st = core_.ctl.st;
 
     SEDI_HIP(launch("init_state_kernel", init_state_kernel, dim3(1), dim3(64), stream_, st, kind_,
                     maxiter, tol));
-    int rc = iM ? launch_setup(pcv, b, x) : launch_setup(pcs, b, x);
+    int rc = iM ? core_setup(core_, PrecondVec{iM}, b, x) : core_setup(core_, PrecondScalar{1.0 / a_}, b, x);
     if (rc) return rc;
 
     // Iterations go out in chunks; the host looks at the device's done flag
@@ -826,12 +849,7 @@ int PlainSolver::solve_dev(const double* b, double* x, const double* iM, int max
         pending = true;
         slot ^= 1;
     }
-    if (kind_ == SEDI_CG) {
-        const long long n = shape_.n();
-        const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4LL * num_cus());
-        SEDI_HIP(launch("cg_flush_kernel", cg_flush_kernel, dim3(std::max(1u, blocks)), dim3(256), stream_,
-                        (const DevState*)st, n, x, (const double*)v_[2]));
-    }
+    if ((rc = core_flush(core_, x))) return rc;
     SEDI_HIP(hipStreamSynchronize(stream_));
     profile_flush();
     DevState hs;
@@ -879,6 +897,112 @@ int PlainSolver::err_history(double* out, int cap) const {
     if (out)
         for (int k = 0; k < std::min(cap, nhist_); k++) out[k] = hist_host_[k];
     return nhist_;
+}
+
+// ---------------------------------------------------------------------------
+// the allocation-free solve
+// ---------------------------------------------------------------------------
+// Its working set -- control block, err history and vectors -- lives in this
+// fixed device array, which the code object brings along: each device that
+// loads the library holds its own copy until the process ends.  Host code
+// never names the array directly (its name is a host-side stand-in, not a
+// device address); it uses the address hipGetSymbolAddress returns for the
+// current device, looked up once per device.  Calls on one device share the
+// array, so calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[SEDI_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static char* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int plain_noalloc_setup() {
+    int dev = 0;
+    SEDI_HIP(hipGetDevice(&dev));
+    SEDI_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, SEDI_ERR_UNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[dev]) return SEDI_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+        return SEDI_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = static_cast<char*>(p);
+    return SEDI_OK;
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t plain_noalloc_bytes(const Shape& s, int kind, int maxiter) {
+    const int nblocks_max = MAX_BLOCKS_PER_CU * num_cus();
+    return round256(plain_ctl_bytes(nblocks_max)) + round256(sizeof(double) * (size_t)std::max(maxiter, 1)) +
+           (size_t)plain_vectors(kind) * round256(sizeof(double) * (size_t)s.n());
+}
+
+// Zeroes the arrival counter (a call that stopped part-way may have left it
+// armed) and writes BiCGStab's vector table, in place of the memset and copy
+// of PlainSolver::create.
+struct VTable { double* v[7]; };
+__global__ void noalloc_ctl_kernel(unsigned* counter, double** table, VTable t) {
+    const int i = threadIdx.x;
+    if (i == 0) *counter = 0u;
+    if (i < 7) table[i] = t.v[i];
+}
+
+int plain_solve_noalloc(const Shape& s, double a, double b, double c, int kind, const double* rhs, double* x,
+                        const double* iM, int maxiter, double tol, double* final_err, int* final_iter) {
+    SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
+    const size_t need = plain_noalloc_bytes(s, kind, maxiter);
+    SEDI_REQUIRE(need <= (size_t)SEDI_NOALLOC_SCRATCH_BYTES, SEDI_ERR_ARG,
+                 "solve needs %zu bytes of working memory, above the allocation-free limit of %zu "
+                 "(SEDI_NOALLOC_SCRATCH_BYTES); use sedi_solver_create/sedi_solve",
+                 need, (size_t)SEDI_NOALLOC_SCRATCH_BYTES);
+    int rc = plain_noalloc_setup();
+    if (rc) return rc;
+    int dev = 0;
+    SEDI_HIP(hipGetDevice(&dev));
+    char* base = g_noalloc_ptr[dev];
+    ProfileSuppress no_events;
+
+    // layout: control block | err history | vectors
+    const int nblocks_max = MAX_BLOCKS_PER_CU * num_cus();
+    PlainCore k;
+    k.shape = s;
+    k.a = a; k.b = b; k.c = c;
+    k.kind = kind;
+    k.stream = nullptr;  // the null stream: creating one would allocate
+    char* p = base + round256(plain_ctl_bytes(nblocks_max));
+    core_bind(k, base, nblocks_max, reinterpret_cast<double*>(p));
+    p += round256(sizeof(double) * (size_t)std::max(maxiter, 1));
+    const size_t vbytes = round256(sizeof(double) * (size_t)s.n());
+    for (int q = 0; q < plain_vectors(kind); q++, p += vbytes) k.v[q] = reinterpret_cast<double*>(p);
+
+    VTable t{};
+    for (int q = 0; q < 7; q++) t.v[q] = k.v[q];
+    SEDI_HIP(launch("noalloc_ctl_kernel", noalloc_ctl_kernel, dim3(1), dim3(64), k.stream, k.ctl.counter,
+                    reinterpret_cast<double**>(base + 384), t));
+    SEDI_HIP(launch("init_state_kernel", init_state_kernel, dim3(1), dim3(64), k.stream, k.ctl.st, kind, maxiter,
+                    tol));
+    rc = iM ? core_setup(k, PrecondVec{iM}, rhs, x) : core_setup(k, PrecondScalar{1.0 / a}, rhs, x);
+    if (rc) return rc;
+    // chunks of iterations; after each the host reads the done flag with a
+    // synchronous copy into its own stack (no pinned buffer to allocate)
+    const int chunk = 32;
+    for (int launched = 0; launched < maxiter;) {
+        const int cnt = std::min(chunk, maxiter - launched);
+        if ((rc = core_iterate_any(k, cnt, iM, x))) return rc;
+        launched += cnt;
+        int done = 0;
+        SEDI_HIP(hipMemcpy(&done, &k.ctl.st->done, sizeof(int), hipMemcpyDeviceToHost));
+        if (done) break;
+    }
+    if ((rc = core_flush(k, x))) return rc;
+    DevState hs;
+    SEDI_HIP(hipMemcpy(&hs, k.ctl.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    SEDI_REQUIRE(hs.done, SEDI_ERR_STATE, "solver stopped without a result (iter %d)", hs.iter);
+    if (final_err) *final_err = hs.final_err;
+    if (final_iter) *final_iter = hs.final_iter;
+    return hs.code;
 }
 
 }  // namespace sedi

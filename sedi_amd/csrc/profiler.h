@@ -14,6 +14,13 @@ bool profile_any();
 // replayed (default on; sedi_graphs_enable).
 bool graphs_enabled();
 void set_graphs_enabled(bool on);
+// While one of these lives, launches on this thread are not timed (timing
+// creates HIP events; the allocation-free solve must not).
+struct ProfileSuppress {
+    ProfileSuppress();
+    ~ProfileSuppress();
+    bool prev;
+};
 // Records the start event (returns a token, -1 if not profiling this kernel).
 int profile_begin(const char* name, hipStream_t stream);
 void profile_end(int token, hipStream_t stream);

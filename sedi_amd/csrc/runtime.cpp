@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "plain.h"
 #include "profiler.h"
 #include "runtime.h"
 #include "tuning.h"
@@ -88,8 +89,12 @@ static std::atomic<bool> g_graphs{true};
 bool graphs_enabled() { return g_graphs.load(); }
 void set_graphs_enabled(bool on) { g_graphs.store(on); }
 
+static thread_local bool g_suppress = false;
+ProfileSuppress::ProfileSuppress() : prev(g_suppress) { g_suppress = true; }
+ProfileSuppress::~ProfileSuppress() { g_suppress = prev; }
+
 bool profile_wants(const char* name) {
-    if (g_target.empty()) return false;
+    if (g_suppress || g_target.empty()) return false;
     return g_target == "*" || g_target == name;
 }
 
@@ -140,7 +145,8 @@ int sedi_init(int device) {
     if (rc) return rc;
     SEDI_HIP(hipSetDevice(device));
     SEDI_HIP(hipFree(nullptr));
-    return SEDI_OK;
+    // this device's address of the allocation-free solve's working array
+    return sedi::plain_noalloc_setup();
 }
 
 int sedi_graphs_enable(int on) {

@@ -152,3 +152,24 @@ def test_tuning_table_lookup(tmp_path):
     env["SEDI_TUNING_FILE"] = ""  # no table: every lookup falls back
     r = subprocess.run([sys.executable, "-c", TUNING_PROBE % ROOT], capture_output=True, text=True, env=env)
     assert r.stdout.split() == ["-1"] * 8
+
+
+def test_noalloc_solve_refuses_what_its_array_cannot_hold():
+    """sedi_solve_noalloc checks the working set against its fixed array
+    before any device call: 256^3 CG needs 3 x 128 MiB > 64 MiB."""
+    lib = ctypes.CDLL(LIB)
+    lib.sedi_last_error.restype = ctypes.c_char_p
+    lib.sedi_solve_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+    st = Stencil(6.0, 1.0, 1.0)
+    b = (ctypes.c_double * 1)()
+    fe, fi = ctypes.c_double(), ctypes.c_int()
+    for dims, kind in [((256, 256, 256), 2), ((128, 128, 128), 3), ((128, 128, 128), 1)]:
+        rc = lib.sedi_solve_noalloc(ctypes.byref(Grid(*dims)), ctypes.byref(st), kind, b, b, None, 100, 1e-9,
+                                    ctypes.byref(fe), ctypes.byref(fi))
+        assert rc == -1 and b"allocation-free limit" in lib.sedi_last_error(), (dims, kind)
+    # maxiter counts too (8 bytes of err history per iteration)
+    rc = lib.sedi_solve_noalloc(ctypes.byref(Grid(16, 16, 16)), ctypes.byref(st), 2, b, b, None, 10 ** 7, 1e-9,
+                                ctypes.byref(fe), ctypes.byref(fi))
+    assert rc == -1 and b"allocation-free limit" in lib.sedi_last_error()
