@@ -60,3 +60,36 @@ def wsum(words):
     i = np.arange(len(words), dtype=np.uint64)
     with np.errstate(over="ignore"):
         return int(np.sum(words * (2 * i + 1), dtype=np.uint64))
+
+
+@pytest.fixture(scope="session")
+def ref_spread():
+    return load_npz("ref_spread.npz")
+
+
+def order_spread(spread, tag, kind):
+    """How far the reference's own results move when only its summation order
+    changes (tests/golden/ref_spread.npz: other -p decompositions and 4- and
+    8-lane SIMD builds of the same solver code, against its -p 1x1x1 2-lane
+    run): the largest |iteration count - the 1x1x1 run's| over every variant
+    and probe, and the largest normwise distance of D.  CG and pipelined CG
+    do not move (0 iterations, D within ~1e-12); BiCGStab does."""
+    its = spread[f"{tag}_s{kind}_iters"]
+    return int(np.max(np.abs(its[1:] - its[0]))), float(np.max(spread[f"{tag}_s{kind}_Ddist"]))
+
+
+def iteration_bar(spread, tag, kind):
+    """Allowed |GPU count - reference count| per probe: the reference's own
+    spread under summation-order changes, and at least 1 (the GPU sums in yet
+    another order)."""
+    return max(1, order_spread(spread, tag, kind)[0])
+
+
+def d_bar(spread, tag, kind, counts_equal):
+    """Allowed normwise distance of D from the reference's: 1e-10 where every
+    count agrees and the solver is order-stable (CG, pipelined CG); else twice
+    the reference's own largest distance under summation-order changes."""
+    dit, dD = order_spread(spread, tag, kind)
+    if counts_equal and dit == 0:
+        return 1e-10
+    return max(1e-10, 2 * dD) if dit > 0 else 1e-8

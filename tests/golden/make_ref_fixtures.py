@@ -17,7 +17,22 @@ prints and writes as .npz files next to this script:
                     final errors, and D's summary (sum, sum of squares, min, max)
                     with D at 4096 strided cells.
 
-Usage: python tests/golden/make_ref_fixtures.py [--big]
+  ref_spread.npz    (--spread) how far the reference's own results move when
+                    only its summation order changes: the three solvers at 32^3
+                    (seed 1234, 8 samples, tol 1e-9) and 13x7x5 (seed 5, 3
+                    samples, tol 1e-8) under other decompositions (every
+                    p1 x p2 x p3 on 2, 4 and 8 ranks, 32^3 only) and wider-SIMD builds of the same driver (4-lane AVX2, 8-lane
+                    AVX-512 dot products; oracle/Makefile ref_variants):
+                    per-variant iteration counts and D's normwise distance
+                    from the -p 1x1x1 SSE2 run.
+  ref_256cube_extra.npz (--big-extra) at 256^3, seed 7: BiCGStab in
+                    fixed-iteration mode (-m 100 -i, 16 samples, -p 2x2x2),
+                    with D's summary and 4096 strided cells; and the first 4
+                    probes of each solver at -p 1x1x1 (SSE2, and AVX-512 for
+                    BiCGStab): iterations and final errors, beside the -p 2x2x2
+                    run of ref_256cube.npz.
+
+Usage: python tests/golden/make_ref_fixtures.py [--big | --spread | --big-extra]
 """
 import os
 import re
@@ -37,10 +52,10 @@ MPIRUN = "/opt/conda/bin/mpirun"
 NAMES = {1: "BiCgStab", 2: "Cg", 3: "PipelinedCg"}
 
 
-def run_ref(grid, procs, samples, tol, maxiter, solver, seed, ignore=False, nranks=1):
+def run_ref(grid, procs, samples, tol, maxiter, solver, seed, ignore=False, nranks=1, binary=None):
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, "out.bin")
-        cmd = [REF, "-g", "x".join(map(str, grid)), "-p", "x".join(map(str, procs)),
+        cmd = [binary or REF, "-g", "x".join(map(str, grid)), "-p", "x".join(map(str, procs)),
                "-s", str(samples), "-e", repr(tol), "-m", str(maxiter), "-S", str(solver),
                "-r", str(seed), "-o", out]
         if ignore:
@@ -104,8 +119,92 @@ def big():
     np.savez_compressed(os.path.join(HERE, "ref_256cube.npz"), **d)
 
 
+def normwise(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def decompositions(n, ranks):
+    """Every p1 x p2 x p3 with p1 p2 p3 = ranks dividing n^3 into sub-domains
+    of at least 2 cells per side (what the reference's halo needs)."""
+    out = []
+    for p1 in range(1, ranks + 1):
+        for p2 in range(1, ranks + 1):
+            if ranks % (p1 * p2):
+                continue
+            p3 = ranks // (p1 * p2)
+            if all(n % p == 0 and n // p >= 2 for p in (p1, p2, p3)):
+                out.append((p1, p2, p3))
+    return out
+
+
+# (label, binary suffix, decomposition, ranks): the -p 1x1x1 SSE2 run first,
+# then every decomposition into 2, 4 and 8 ranks with each SIMD width, and
+# the 1x1x1 run with the wider SIMD
+SIMD = [("sse2", ""), ("avx2", "_avx2"), ("avx512", "_avx512")]
+VARIANTS_32 = [("sse2_p111", "", (1, 1, 1), 1)] + \
+    [(f"{w}_p{p[0]}{p[1]}{p[2]}", suf, p, r) for r in (2, 4, 8) for p in decompositions(32, r) for w, suf in SIMD] + \
+    [("avx2_p111", "_avx2", (1, 1, 1), 1), ("avx512_p111", "_avx512", (1, 1, 1), 1)]
+# 13x7x5 has no decomposition into sub-domains of two or more cells per side
+# (which the reference's halo needs), so only the SIMD width varies there
+VARIANTS_ODD = [("sse2_p111", "", (1, 1, 1), 1), ("avx2_p111", "_avx2", (1, 1, 1), 1),
+                ("avx512_p111", "_avx512", (1, 1, 1), 1)]
+
+
+def spread():
+    d = {}
+    for tag, grid, samples, tol, maxiter, seed, variants in [
+            ("c32", (32, 32, 32), 8, 1e-9, 500, 1234, VARIANTS_32),
+            ("odd", (13, 7, 5), 3, 1e-8, 200, 5, VARIANTS_ODD)]:
+        d[f"{tag}_variants"] = np.array([v[0] for v in variants])
+        for s in (1, 2, 3):
+            its, dist = [], []
+            base = None
+            for label, suffix, procs, nranks in variants:
+                r = run_ref(grid, procs, samples, tol, maxiter, s, seed, nranks=nranks, binary=REF + suffix)
+                assert np.all(r["codes"] == 0), (tag, s, label)
+                base = base or r
+                its.append(r["iters"])
+                dist.append(normwise(r["D"], base["D"]))
+                print(tag, "solver", s, label, r["iters"].tolist(), "D %.2e" % dist[-1], flush=True)
+            d[f"{tag}_s{s}_iters"] = np.array(its, dtype=np.int32)
+            d[f"{tag}_s{s}_Ddist"] = np.array(dist)
+    np.savez_compressed(os.path.join(HERE, "ref_spread.npz"), **d)
+
+
+def big_extra():
+    d = {}
+    n = 256 ** 3
+    idx = np.arange(0, n, n // 4096, dtype=np.int64)[:4096]
+    r = run_ref((256, 256, 256), (2, 2, 2), 16, 1e-9, 100, 1, 7, ignore=True, nranks=8)
+    D = r["D"]
+    d.update({"s1_fixed100_iters": r["iters"], "s1_fixed100_codes": r["codes"], "s1_fixed100_errs": r["errs"],
+              "s1_fixed100_Dsum": np.array(D.sum()), "s1_fixed100_Dsq": np.array((D * D).sum()),
+              "s1_fixed100_Dmin": np.array(D.min()), "s1_fixed100_Dmax": np.array(D.max()),
+              "s1_fixed100_Dsample": D[idx]})
+    print("fixed100", r["iters"].tolist(), flush=True)
+    # four single-rank runs at once (one core each)
+    jobs = [(1, ""), (1, "_avx512"), (2, ""), (3, "")]
+
+    def one(job):
+        s, suffix = job
+        return job, run_ref((256, 256, 256), (1, 1, 1), 4, 1e-9, 2000, s, 7, ignore=True, binary=REF + suffix)
+
+    import concurrent.futures
+    with concurrent.futures.ThreadPoolExecutor(len(jobs)) as ex:
+        for (s, suffix), r in ex.map(one, jobs):
+            tag = f"s{s}_p111{suffix}"
+            d.update({f"{tag}_iters": r["iters"], f"{tag}_codes": r["codes"], f"{tag}_errs": r["errs"]})
+            print(tag, r["iters"].tolist(), flush=True)
+    d["sample_idx"] = idx
+    np.savez_compressed(os.path.join(HERE, "ref_256cube_extra.npz"), **d)
+
+
 if __name__ == "__main__":
     if "--big" in sys.argv:
         big()
+    elif "--spread" in sys.argv:
+        spread()
+    elif "--big-extra" in sys.argv:
+        big_extra()
     else:
         small()

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from conftest import normwise
+from conftest import d_bar, iteration_bar, normwise
 
 sedi = pytest.importorskip("sedi_amd")
 
@@ -164,7 +164,7 @@ def test_batch_fp64_other_solvers_fixed_iterations_vs_oracle(kind):
 
 
 @pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
-def test_batch_fp64_other_solvers_converged_vs_oracle(kind):
+def test_batch_fp64_other_solvers_converged_vs_oracle(ref_spread, kind):
     b = probes(1234, S32, 0, 8)
     x, iters, errs, codes = sedi.Batch(S32, 8, sedi.FP64, kind=kind).solve(b, maxiter=500, tol=1e-9)
     assert np.all(codes == 0)
@@ -172,8 +172,8 @@ def test_batch_fp64_other_solvers_converged_vs_oracle(kind):
         code, ox, ofe, ofi, oh = O.solve(kind, S32, b[k], maxiter=500, tol=1e-9)
         if kind == O.PCG:
             assert abs(int(iters[k]) - ofi) <= 1
-        else:  # rounding differences grow along BiCGStab's iterations
-            assert abs(int(iters[k]) - ofi) <= max(3, ofi // 5)
+        else:  # the reference's own spread under summation-order changes
+            assert abs(int(iters[k]) - ofi) <= iteration_bar(ref_spread, "c32", kind), (k, iters[k], ofi)
         assert true_rel_residual(S32, b[k], x[k]) < 3e-9
 
 
@@ -225,16 +225,15 @@ def test_result_same_alone_and_in_the_mix_other_solvers(kind):
 
 
 @pytest.mark.parametrize("kind", [O.PCG, O.BICGSTAB])
-def test_estimator_batched_other_solvers_vs_reference(ref32, kind):
+def test_estimator_batched_other_solvers_vs_reference(ref32, ref_spread, kind):
     r = sedi.estimate(S32, kind=kind, samples=8, maxiter=500, tol=1e-9, seed=1234, batch_k=8)
     ref_it = ref32[f"s{kind}_iters"]
     assert r["status"] == 0 and np.all(r["codes"] == 0)
-    if kind == O.PCG:
-        assert np.all(np.abs(r["iters"] - ref_it) <= 1), (r["iters"], ref_it)
-        assert normwise(r["D"], ref32[f"s{kind}_D"]) < (1e-10 if np.array_equal(r["iters"], ref_it) else 1e-8)
-    else:
-        assert np.all(np.abs(r["iters"] - ref_it) <= np.maximum(3, ref_it // 5)), (r["iters"], ref_it)
-        assert normwise(r["D"], ref32[f"s{kind}_D"]) < 1e-6
+    bar = iteration_bar(ref_spread, "c32", kind)
+    print("kind", kind, "iters", r["iters"].tolist(), "ref", ref_it.tolist(), "bar", bar, flush=True)
+    assert np.all(np.abs(r["iters"] - ref_it) <= bar), (r["iters"], ref_it)
+    assert normwise(r["D"], ref32[f"s{kind}_D"]) < d_bar(ref_spread, "c32", kind,
+                                                         np.array_equal(r["iters"], ref_it))
     f = sedi.estimate(S32, kind=kind, samples=8, maxiter=20, tol=1e-9, seed=1234, batch_k=8,
                       ignore_errors=True)
     assert np.array_equal(f["iters"], ref32[f"s{kind}_fixed20_iters"])

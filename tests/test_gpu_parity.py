@@ -8,17 +8,21 @@ Bars (see the module docstrings of the oracle and sedi_amd):
   * CG / pipelined CG in convergence mode: iteration counts within +-1 of the
     reference and D within 1e-10 (normwise) when the counts agree;
   * all solvers in fixed-iteration mode (-m 20 -i): D within 1e-10 (normwise);
-  * BiCGStab in convergence mode: rounding-level differences grow
-    exponentially along its iterations (the reference itself shows this
-    between -p 1x1x1 and other summation orders), so parity there is: every
-    solve converges, the true residual meets the tolerance, and D agrees with
-    the reference's within the error the tolerance admits.
+  * BiCGStab in convergence mode: rounding-level differences grow along its
+    iterations, and the reference itself moves when only its summation order
+    changes (tests/golden/ref_spread.npz: at 32^3 its counts move by up to 9
+    of ~95 iterations and D by up to 4.5e-8 normwise across 59 variants --
+    every decomposition into 2, 4 and 8 ranks and 2-, 4- and 8-lane SIMD dot
+    products).  The GPU sums in one more order, so its bars are that measured
+    spread: counts within 9 of the -p 1x1x1 run's per probe, D within twice
+    the largest D distance; on the 13x7x5 grid, where the reference does not
+    move, +-1 and 1e-10 as for the other solvers.
 """
 import numpy as np
 import pytest
 
 import oracle_lib as O
-from conftest import normwise, sign_words, wsum
+from conftest import d_bar, iteration_bar, normwise, sign_words, wsum
 
 sedi = pytest.importorskip("sedi_amd")
 
@@ -92,7 +96,7 @@ def gpu_solve(kind, shape, b, x0=None, iM=None, maxiter=500, tol=1e-9, abc=(6.0,
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_single_solve_vs_oracle_history(kind):
+def test_single_solve_vs_oracle_history(ref_spread, kind):
     shape = (32, 32, 32)
     b = O.rademacher(1234, 0, 32768)
     code, x, fe, fi, hist = gpu_solve(kind, shape, b, maxiter=500, tol=1e-9)
@@ -105,25 +109,30 @@ def test_single_solve_vs_oracle_history(kind):
         assert abs(fi - ofi) <= 1
         assert normwise(x, ox) < 1e-9
     else:
-        assert abs(fi - ofi) <= max(3, ofi // 5)
+        # probe 0 of seed 1234 at 32^3: the reference's own spread
+        assert abs(fi - ofi) <= iteration_bar(ref_spread, "c32", kind), (fi, ofi)
     true = np.linalg.norm(b - O.stencil_apply(shape, x)) / np.linalg.norm(b)
     assert true < 2e-9
 
 
+def check_vs_reference(r, ref, ref_spread, tag, kind):
+    """Counts and D of an estimate against the reference's run, within the
+    reference's own summation-order spread (+-1 and 1e-10 where it has none)."""
+    ref_it = ref[f"s{kind}_iters"]
+    bar = iteration_bar(ref_spread, tag, kind)
+    dev = np.abs(r["iters"] - ref_it)
+    dist = normwise(r["D"], ref[f"s{kind}_D"])
+    print("kind", kind, "iters", r["iters"].tolist(), "ref", ref_it.tolist(), "max dev", int(dev.max()),
+          "bar", bar, "D %.2e" % dist, flush=True)
+    assert np.all(dev <= bar), (r["iters"], ref_it, bar)
+    assert dist < d_bar(ref_spread, tag, kind, np.array_equal(r["iters"], ref_it)), dist
+
+
 @pytest.mark.parametrize("kind", KINDS)
-def test_estimator_32cube_vs_reference(ref32, kind):
+def test_estimator_32cube_vs_reference(ref32, ref_spread, kind):
     r = sedi.estimate((32, 32, 32), kind=kind, samples=8, maxiter=500, tol=1e-9, seed=1234)
-    ref_it = ref32[f"s{kind}_iters"]
     assert r["status"] == 0 and np.all(r["codes"] == 0)
-    if kind != O.BICGSTAB:
-        assert np.all(np.abs(r["iters"] - ref_it) <= 1), (r["iters"], ref_it)
-        if np.array_equal(r["iters"], ref_it):
-            assert normwise(r["D"], ref32[f"s{kind}_D"]) < 1e-10
-        else:
-            assert normwise(r["D"], ref32[f"s{kind}_D"]) < 1e-8
-    else:
-        assert np.all(np.abs(r["iters"] - ref_it) <= np.maximum(3, ref_it // 5)), (r["iters"], ref_it)
-        assert normwise(r["D"], ref32[f"s{kind}_D"]) < 1e-6
+    check_vs_reference(r, ref32, ref_spread, "c32", kind)
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -136,12 +145,9 @@ def test_estimator_fixed_iterations_vs_reference(ref32, kind):
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_estimator_odd_grid_vs_reference(ref_odd, kind):
+def test_estimator_odd_grid_vs_reference(ref_odd, ref_spread, kind):
     r = sedi.estimate((13, 7, 5), kind=kind, samples=3, maxiter=200, tol=1e-8, seed=5)
-    ref_it = ref_odd[f"s{kind}_iters"]
-    assert np.all(np.abs(r["iters"] - ref_it) <= 1), (r["iters"], ref_it)
-    if np.array_equal(r["iters"], ref_it):
-        assert normwise(r["D"], ref_odd[f"s{kind}_D"]) < 1e-10
+    check_vs_reference(r, ref_odd, ref_spread, "odd", kind)
 
 
 @pytest.mark.parametrize("kind", KINDS)

@@ -235,3 +235,31 @@ def test_decomposed_reference_agrees_to_rounding(ref32):
     p8 = load_npz("ref_32cube_p8.npz")
     assert np.array_equal(p8["s2_iters"], ref32["s2_iters"])
     assert normwise(p8["s2_D"], ref32["s2_D"]) < 1e-10
+
+
+@pytest.mark.parametrize("tag", ["c32", "odd"])
+def test_reference_summation_order_spread(ref32, ref_odd, ref_spread, tag):
+    """The evidence behind the BiCGStab bars: the reference's own solvers,
+    changed only in their summation order (other decompositions, 4- and
+    8-lane SIMD dot products), stop CG and pipelined CG at the same iteration
+    for every probe with D equal to rounding, while BiCGStab's counts move by
+    several iterations and D by ~1e-8 at 32^3.  Variant 0 is the -p 1x1x1
+    2-lane run of ref_32cube.npz / ref_odd.npz, which the oracle equals
+    bitwise."""
+    ref = ref32 if tag == "c32" else ref_odd
+    for kind in (1, 2, 3):
+        its = ref_spread[f"{tag}_s{kind}_iters"]
+        assert np.array_equal(its[0], ref[f"s{kind}_iters"])
+        assert ref_spread[f"{tag}_s{kind}_Ddist"][0] == 0.0
+        if kind != 1:
+            assert np.all(its == its[0])
+            assert np.max(ref_spread[f"{tag}_s{kind}_Ddist"]) < 1e-11
+    if tag == "c32":
+        spread = np.abs(ref_spread["c32_s1_iters"][1:] - ref_spread["c32_s1_iters"][0])
+        assert spread.max() == 9 and 1e-8 < np.max(ref_spread["c32_s1_Ddist"]) < 1e-7
+        # the decompositions alone, and the SIMD width alone, each move it
+        labels = list(ref_spread["c32_variants"])
+        simd_only = [labels.index("avx2_p111"), labels.index("avx512_p111")]
+        decomp_only = [i for i, l in enumerate(labels) if l.startswith("sse2_") and i > 0]
+        assert len(labels) == 60 and len(decomp_only) == 19
+        assert spread[[i - 1 for i in simd_only]].max() > 0 and spread[[i - 1 for i in decomp_only]].max() > 0
