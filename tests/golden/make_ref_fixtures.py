@@ -32,7 +32,22 @@ prints and writes as .npz files next to this script:
                     BiCGStab): iterations and final errors, beside the -p 2x2x2
                     run of ref_256cube.npz.
 
-Usage: python tests/golden/make_ref_fixtures.py [--big | --spread | --big-extra]
+  ref_breakdown.json (--breakdown) BiCGStab on small systems built so that
+                    its breakdowns happen in exact arithmetic (oracle/_ref/
+                    sedi_ref_cases: the reference's own BiCgStab on one rank):
+                    omega = <t,s>/<t,t> = 0 (code 3) and rho = <r,r^> = 0 at
+                    the second iteration (code 2), along each axis, and omega
+                    = 0 with err <= tol (code 0: the outcome order of
+                    BiCgStab.hpp:237-259); code, final iteration, final error,
+                    the err lines and x.
+
+  ref_spread_256cube.npz (--spread256) BiCGStab at 256^3 (seed 7, 16
+                    samples, tol 1e-9) under seven other summation orders
+                    (-p 1x1x8, 1x8x1, 8x1x1, 1x2x4, 4x2x1 with 2-lane SIMD;
+                    -p 2x2x2 with 4- and 8-lane SIMD): counts and D at the
+                    4096 strided cells, beside ref_256cube.npz's -p 2x2x2.
+
+Usage: python tests/golden/make_ref_fixtures.py [--big | --spread | --big-extra | --breakdown | --spread256]
 """
 import os
 import re
@@ -171,6 +186,30 @@ def spread():
     np.savez_compressed(os.path.join(HERE, "ref_spread.npz"), **d)
 
 
+VARIANTS_256 = [("sse2_p118", "", (1, 1, 8)), ("sse2_p181", "", (1, 8, 1)), ("sse2_p811", "", (8, 1, 1)),
+                ("sse2_p124", "", (1, 2, 4)), ("sse2_p421", "", (4, 2, 1)), ("avx2_p222", "_avx2", (2, 2, 2)),
+                ("avx512_p222", "_avx512", (2, 2, 2))]
+
+
+def spread256():
+    """BiCGStab at 256^3 (seed 7, 16 samples, tol 1e-9) under other
+    decompositions into 8 ranks and SIMD widths, beside the -p 2x2x2 2-lane
+    run of ref_256cube.npz: iteration counts and D at its 4096 strided cells."""
+    d = {"variants": np.array([v[0] for v in VARIANTS_256])}
+    n = 256 ** 3
+    idx = np.arange(0, n, n // 4096, dtype=np.int64)[:4096]
+    its, samples = [], []
+    for label, suffix, procs in VARIANTS_256:
+        r = run_ref((256, 256, 256), procs, 16, 1e-9, 2000, 1, 7, ignore=True, nranks=8, binary=REF + suffix)
+        its.append(r["iters"])
+        samples.append(r["D"][idx])
+        print(label, r["iters"].tolist(), flush=True)
+    d["s1_iters"] = np.array(its, dtype=np.int32)
+    d["s1_Dsample"] = np.array(samples)
+    d["sample_idx"] = idx
+    np.savez_compressed(os.path.join(HERE, "ref_spread_256cube.npz"), **d)
+
+
 def big_extra():
     d = {}
     n = 256 ** 3
@@ -199,6 +238,39 @@ def big_extra():
     np.savez_compressed(os.path.join(HERE, "ref_256cube_extra.npz"), **d)
 
 
+# (grid, stencil (a, b, c), right-hand side, tol, maxiter): found by a search
+# over 1-4 cell grids, stencils and right-hand sides of small integers with
+# the oracle, then run on the reference
+BREAKDOWN_CASES = [
+    ((2, 1, 1), (1.0, 2.0, 0.0), (1.0, -1.0), 1e-12, 6),   # omega = 0 at iteration 1
+    ((1, 2, 1), (1.0, 2.0, 0.0), (1.0, -1.0), 1e-12, 6),
+    ((1, 1, 2), (1.0, 2.0, 0.0), (1.0, -1.0), 1e-12, 6),
+    ((2, 1, 1), (1.0, 0.0, -2.0), (2.0, 2.0), 1e-12, 6),
+    ((3, 1, 1), (1.0, 2.0, 1.0), (1.0, 2.0, 2.0), 1e-12, 6),  # rho = 0 at iteration 2
+    ((1, 3, 1), (1.0, 2.0, 1.0), (1.0, 2.0, 2.0), 1e-12, 6),
+    ((1, 1, 3), (1.0, 2.0, 1.0), (1.0, 2.0, 2.0), 1e-12, 6),
+    ((3, 1, 1), (2.0, -2.0, -1.0), (1.0, 2.0, 2.0), 1e-12, 6),
+    ((2, 1, 1), (1.0, 2.0, 0.0), (1.0, -1.0), 0.5, 6),     # omega = 0 but err <= tol: converged
+]
+
+
+def breakdown():
+    import json
+    out = []
+    for grid, abc, rhs, tol, maxiter in BREAKDOWN_CASES:
+        cmd = [REF + "_cases", "-g", "x".join(map(str, grid)), "-c", ",".join(repr(v) for v in abc),
+               "-b", ",".join(repr(v) for v in rhs), "-e", repr(tol), "-m", str(maxiter), "-S", "1", "-f", "1"]
+        log = subprocess.run(cmd, check=True, capture_output=True, text=True).stdout
+        m = re.search(r"result code=(\d+) iter=(-?\d+) err=(\S+)", log)
+        x = [float(v) for v in re.search(r"^x (.*)$", log, re.M).group(1).split()]
+        hist = [float(v) for v in re.findall(r"BiCgStab: after iter=\d+, err=(\S+)", log)]
+        out.append(dict(grid=grid, abc=abc, rhs=rhs, tol=tol, maxiter=maxiter, code=int(m.group(1)),
+                        iter=int(m.group(2)), err=float(m.group(3)), hist=hist, x=x))
+        print(out[-1], flush=True)
+    with open(os.path.join(HERE, "ref_breakdown.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
     if "--big" in sys.argv:
         big()
@@ -206,5 +278,9 @@ if __name__ == "__main__":
         spread()
     elif "--big-extra" in sys.argv:
         big_extra()
+    elif "--breakdown" in sys.argv:
+        breakdown()
+    elif "--spread256" in sys.argv:
+        spread256()
     else:
         small()

@@ -223,3 +223,48 @@ def test_estimate_matches_exact_diagonal_statistically():
     r = sedi.estimate(shape, kind=sedi.CG, samples=400, maxiter=500, tol=1e-12, seed=5)
     st, D, iters, errs, codes = O.estimate(O.CG, shape, 5, 400, 500, 1e-12)
     assert normwise(r["D"], D) < 1e-12
+
+
+def _breakdown_cases():
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "ref_breakdown.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", range(9))
+def test_bicgstab_breakdown_outcomes_vs_reference(case):
+    """BiCGStab's codes 3 (omega = 0) and 2 (rho = 0), along each axis, and
+    omega = 0 with err <= tol (code 0: the outcome order of
+    BiCgStab.hpp:237-259), on systems where they happen in exact arithmetic:
+    code, final iteration, final error, err lines and x equal the reference's
+    own solver's (tests/golden/ref_breakdown.json)."""
+    c = _breakdown_cases()[case]
+    shape = tuple(c["grid"])
+    print("breakdown case", case, shape, c["abc"], c["rhs"], c["tol"], flush=True)
+    code, x, fe, fi, hist = gpu_solve(O.BICGSTAB, shape, np.array(c["rhs"]), maxiter=c["maxiter"], tol=c["tol"],
+                                      abc=tuple(c["abc"]))
+    assert (code, fi, fe) == (c["code"], c["iter"], c["err"])
+    assert np.array_equal(x, np.array(c["x"]))
+    assert [float("%.10e" % h) for h in hist] == c["hist"]
+
+
+@pytest.mark.parametrize("case", [0, 2, 4, 6, 8])
+def test_bicgstab_breakdown_outcomes_batched_fp64(case):
+    """The same outcomes from the batched FP64 BiCGStab, per right-hand side,
+    with a regular probe beside it in the block that must not be disturbed."""
+    c = _breakdown_cases()[case]
+    shape = tuple(c["grid"])
+    n = int(np.prod(shape))
+    b = np.stack([np.array(c["rhs"]), np.zeros(n), np.array(c["rhs"])[::-1].copy()])
+    bs = sedi.Batch(shape, 4, sedi.FP64, kind=sedi.BICGSTAB, stencil=sedi.Stencil(*c["abc"]))
+    try:
+        x, iters, errs, codes = bs.solve(b, maxiter=c["maxiter"], tol=c["tol"])
+    finally:
+        bs.close()
+    assert (codes[0], iters[0], errs[0]) == (c["code"], c["iter"], c["err"])
+    assert np.array_equal(x[0], np.array(c["x"]))
+    assert (codes[1], iters[1], errs[1]) == (0, 0, 0.0)
+    oc, ox, ofe, ofi, _ = O.solve(O.BICGSTAB, shape, b[2], maxiter=c["maxiter"], tol=c["tol"], abc=tuple(c["abc"]))
+    assert (codes[2], iters[2]) == (oc, ofi) and np.array_equal(x[2], ox)

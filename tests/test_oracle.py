@@ -263,3 +263,52 @@ def test_reference_summation_order_spread(ref32, ref_odd, ref_spread, tag):
         decomp_only = [i for i, l in enumerate(labels) if l.startswith("sse2_") and i > 0]
         assert len(labels) == 60 and len(decomp_only) == 19
         assert spread[[i - 1 for i in simd_only]].max() > 0 and spread[[i - 1 for i in decomp_only]].max() > 0
+
+
+def _breakdown_cases():
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "ref_breakdown.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", range(9))
+def test_oracle_bicgstab_breakdowns_equal_reference(case):
+    """BiCGStab's breakdown outcomes (BiCgStab.hpp:162-163,229-230,237-259) on
+    systems where they happen in exact arithmetic: omega = 0 (code 3), rho = 0
+    at the second iteration (code 2), each along x, y and z, and omega = 0
+    with err <= tol (code 0, the outcome order).  The reference's own solver
+    (tests/golden/ref_breakdown.json) and the oracle agree bitwise."""
+    c = _breakdown_cases()[case]
+    code, x, fe, fi, hist = O.solve(O.BICGSTAB, tuple(c["grid"]), np.array(c["rhs"]), maxiter=c["maxiter"],
+                                    tol=c["tol"], abc=tuple(c["abc"]))
+    assert (code, fi, fe) == (c["code"], c["iter"], c["err"])
+    assert np.array_equal(x, np.array(c["x"]))
+    nh = len(c["hist"])
+    assert [float("%.10e" % h) for h in hist[:nh]] == c["hist"]
+    assert code in (0, 2, 3)
+
+
+def test_breakdown_cases_cover_every_outcome():
+    codes = [c["code"] for c in _breakdown_cases()]
+    assert codes.count(3) == 4 and codes.count(2) == 4 and codes.count(0) == 1
+
+
+def test_exact_diag_fixture_256cube():
+    """tests/golden/exact_diag_256cube.npz (diag(A^-1) at 8192 cells of the
+    256^3 grid, from oracle_exact_diag_inv): re-computed cells agree, the
+    corner is the known 0.185577 and every cell lies between it and Watson's
+    limit 0.2527 (the infinite lattice's value, which interior cells approach
+    from below)."""
+    from conftest import load_npz
+    ex = load_npz("exact_diag_256cube.npz")
+    assert abs(ex["diag_strided"][0] - 0.185577) < 1e-6
+    for name in ("strided", "random"):
+        d = ex["diag_" + name]
+        assert len(d) == 4096 and d.min() >= 0.185577 - 1e-6 and d.max() < 0.25273
+        pick = np.array([0, 1000, 4095])
+        again = O.exact_diag_inv((256, 256, 256), ex["idx_" + name][pick])
+        assert np.allclose(again, d[pick], rtol=0, atol=1e-15)
+    n = 256 ** 3
+    assert np.array_equal(ex["idx_strided"], np.arange(0, n, n // 4096)[:4096])
