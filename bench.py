@@ -88,7 +88,10 @@ def parse():
     p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512", "serve"])
     p.add_argument("--jobs", type=int, default=200, help="serve: jobs in the stream")
     p.add_argument("--rate", default="5", help="serve: Poisson arrival rate(s) in jobs/s, comma-separated")
-    p.add_argument("--serve-jobs", type=int, default=30,
+    p.add_argument("--serve-sub-jobs", type=int, default=40,
+                   help="headline: jobs of the serving sample below saturation (0 = none)")
+    p.add_argument("--serve-sub-rate", type=float, default=0.5, help="headline: its arrival rate, jobs/s")
+    p.add_argument("--serve-jobs", type=int, default=100,
                    help="headline: jobs of the serving sample reported beside it (0 = none)")
     p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
     p.add_argument("--solver", default="cg", choices=sorted(NAMES), help="--mode plain only")
@@ -101,6 +104,8 @@ def parse():
     p.add_argument("--stencil-side", type=int, default=0, help="--stencil: grid side (default 256)")
     p.add_argument("--stencil", action="store_true",
                    help="time the standalone 7-point stencil kernel (y = A x, FP64) at 256^3 and exit")
+    p.add_argument("--cold", action="store_true",
+                   help="--stencil: rotate through (x, y) pairs spanning 1 GiB so x comes from HBM")
     p.add_argument("--pmc-workload", action="store_true",
                    help="run a short fixed workload for rocprofv3 --pmc (1 probe per plain solver "
                         "and one batch of 64 mixed CG probes, 10 iterations each, at 256^3) and exit")
@@ -127,43 +132,55 @@ def pmc_workload():
     print("pmc workload done", flush=True)
 
 
-def stencil_only(reps=20, side=None):
+# footprint of a cold measurement: the (x, y) pairs it rotates through span
+# at least four times the 256 MB Infinity Cache
+COLD_BYTES = 1 << 30
+
+
+def stencil_only(reps=20, side=None, cold=False):
     """y = A x at 256^3 (or side^3) through sedi_stencil_apply with
-    device-resident x, y (torch allocations), kernel time from HIP events
+    device-resident x, y (sedi_amd.DeviceArray), kernel time from HIP events
     around each launch; algorithmic bytes 2*N*8 per launch (read x, write y).
-    At 256^3 x fits the 256 MB Infinity Cache between launches; 512^3
-    (1 GiB vectors) streams from HBM."""
+    Warm: back-to-back launches on one (x, y), so at 256^3 part of x is still
+    in the 256 MB Infinity Cache from the previous launch.  Cold: launches
+    rotate through enough (x, y) pairs to span 1 GiB (4 at 256^3, 2 at
+    512^3), so each launch reads an x untouched since three launches back
+    (evicted) while the cache holds the previous launch's dirty y, as in a
+    solver's sweep over vectors larger than the cache."""
     import ctypes
     import numpy as np
-    import torch
     import sedi_amd
     grid = (side,) * 3 if side else GRID
     n = grid[0] * grid[1] * grid[2]
-    x = torch.from_numpy(np.random.default_rng(0).standard_normal(n)).to("cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
+    npairs = max(2, -(-COLD_BYTES // (16 * n))) if cold else 1
+    x0 = np.random.default_rng(0).standard_normal(n)
+    pairs = [(sedi_amd.DeviceArray(x0), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
     g, _ = sedi_amd._grid(grid)
     st = sedi_amd.Stencil()._c()
     L = sedi_amd.lib()
 
-    def apply():
-        rc = L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), ctypes.c_void_p(x.data_ptr()),
-                                  ctypes.c_void_p(y.data_ptr()))
+    def apply(k):
+        x, y = pairs[k % npairs]
+        rc = L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), x.ptr, y.ptr)
         if rc < 0:
             raise RuntimeError(L.sedi_last_error().decode())
 
-    apply()  # warm-up
+    for k in range(npairs):  # warm-up: every pair touched once
+        apply(k)
     sedi_amd.profile_reset()
     sedi_amd.profile_enable("stencil_apply_kernel")
-    for _ in range(reps):
-        apply()
+    for r in range(reps):
+        apply(r)
     sedi_amd.profile_enable(None)
     launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
     avg = ms / 1e3 / launches
     gbs = 2 * n * 8 / avg / 1e9
-    return {"kernel": "stencil_apply_kernel", "grid": list(grid), "launches": launches,
-            "avg_launch_us": round(avg * 1e6, 2), "achieved_GBs": round(gbs, 1),
-            "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+    for x, y in pairs:
+        x.free()
+        y.free()
+    return {"kernel": "stencil_apply_kernel", "grid": list(grid), "cache": "cold" if cold else "warm",
+            "pairs": npairs, "launches": launches, "avg_launch_us": round(avg * 1e6, 2),
+            "achieved_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def traffic_ratio_from_profiles(kernel):
@@ -377,12 +394,18 @@ def run_headline(args):
                                        iterations_mean_mixed=round(float(rm["iters"].mean()), 2))
     if args.serve_jobs > 0:
         # the metric's third part: job latency of the serving stream, on a
-        # bounded sample of it
+        # bounded sample of it: saturated (5 jobs/s, several times what the
+        # GPU completes of this mix) and below saturation (0.5 jobs/s)
         line["serving"] = dict(run_serve(args.serve_jobs, 5.0),
                                workload="the serving stream's first %d jobs at 5 jobs/s (Poisson; sides "
                                         "64-256, 8-64 probes, tol 1e-6/1e-8/1e-9, CG/PipelinedCG/BiCGStab, "
                                         "seed = job id, mixed, 200 GB budget); the GPU is saturated, so "
                                         "latency includes queueing" % args.serve_jobs)
+    if args.serve_sub_jobs > 0:
+        line["serving_below_saturation"] = dict(
+            run_serve(args.serve_sub_jobs, args.serve_sub_rate),
+            workload="the same stream's first %d jobs at %g jobs/s (below what the GPU completes of this "
+                     "mix): latency is mostly service time" % (args.serve_sub_jobs, args.serve_sub_rate))
     line["cpu_baseline"] = None
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(2, BATCH["seed"], BATCH["tol"], float(iters.mean()),
@@ -390,18 +413,37 @@ def run_headline(args):
     return line
 
 
+# diag(A^-1) of the (6, 1, 1) operator is smallest at a corner cell,
+# 0.185577 on every grid of side >= 6
+DIAG_MIN = 0.185577
+EXACT_DIAG = os.path.join(ROOT, "tests", "golden", "exact_diag_256cube.npz")
+
+
 def d_errors(D, ref):
-    """Errors of a diagonal estimate against a reference one: the normwise max
-    max|dD| / max|D| and RMS ||dD||_2 / ||D||_2 (the parity metrics: with a
-    few dozen probes the estimate crosses zero at some cells, where per-cell
-    relative errors are ill-conditioned), and for completeness the per-cell
-    max and RMS relative errors."""
+    """Errors of a diagonal estimate against a reference one on the same
+    probes.  Per cell relative to the quantity estimated, diag(A^-1)_i:
+    max and RMS of |dD_i| / diag(A^-1)_i at the 8192 cells of
+    tests/golden/exact_diag_256cube.npz (exact diagonal; 256^3 only), and over
+    every cell the bound max|dD| / min diag(A^-1).  Normwise: max|dD| / max|D|
+    and ||dD||_2 / ||D||_2.  And the per-cell error relative to the estimate
+    itself, |dD_i| / |D_i|, which is ill-posed: with a few dozen probes D
+    crosses zero at some cells, so it is large there whatever the precision."""
     import numpy as np
     d = np.abs(D - ref)
     rel = d / np.abs(ref)
-    return {"normwise_max": float(d.max() / np.abs(ref).max()),
-            "normwise_rms": float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
-            "cell_max_rel": float(rel.max()), "cell_rms_rel": float(np.sqrt(np.mean(rel * rel)))}
+    out = {"cell_max_rel_bound_all_cells": float(d.max() / DIAG_MIN)}
+    if len(D) == 256 ** 3 and os.path.exists(EXACT_DIAG):
+        with np.load(EXACT_DIAG) as z:
+            idx = np.concatenate([z["idx_strided"], z["idx_random"]])
+            dg = np.concatenate([z["diag_strided"], z["diag_random"]])
+        r = d[idx] / dg
+        out.update({"cell_max_rel_vs_exact_diag": float(r.max()),
+                    "cell_rms_rel_vs_exact_diag": float(np.sqrt(np.mean(r * r))), "exact_diag_cells": len(idx)})
+    out.update({"normwise_max": float(d.max() / np.abs(ref).max()),
+                "normwise_rms": float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
+                "cell_max_rel_to_estimate": float(rel.max()),
+                "cell_rms_rel_to_estimate": float(np.sqrt(np.mean(rel * rel)))})
+    return out
 
 
 def estimator_run(grid, progress=False, maxiter=MAXITER, **kw):
@@ -547,7 +589,7 @@ def main():
         pmc_workload()
         return
     if args.stencil:
-        print(json.dumps(stencil_only(side=args.stencil_side or None)), flush=True)
+        print(json.dumps(stencil_only(side=args.stencil_side or None, cold=args.cold)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
