@@ -27,7 +27,7 @@ __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEG
            "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
            "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
            "Estimator", "estimate", "Batch", "probe_batch", "Server", "h5_available", "h5_read", "h5_write",
-           "HOOK_X", "HOOK_V", "HOOK_D", "graphs_enable", "profile_enable", "profile_get",
+           "HOOK_X", "HOOK_V", "HOOK_D", "DeviceArray", "graphs_enable", "profile_enable", "profile_get",
            "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -156,6 +156,63 @@ def _ptr(a):
 def _grid(grid):
     n1, n2, n3 = (int(v) for v in grid)
     return _Grid(n1, n2, n3), n1 * n2 * n3
+
+
+_hip = None
+
+
+def _hiprt():
+    """The HIP runtime libsedi_amd.so is linked to (loaded with it): device
+    memory for DeviceArray comes from the same runtime the kernels run on, so
+    the library sees it as device memory whatever else the process loaded."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = ctypes.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        _hip = H
+    return _hip
+
+
+class DeviceArray:
+    """A float64 array in device memory (hipMalloc), for the entry points that
+    take device pointers: keeps inputs and outputs resident on the GPU.
+    `ptr` is the device address; get() copies it back."""
+
+    def __init__(self, a=None, n=None):
+        H = _hiprt()
+        host = None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self.n = host.size if host is not None else int(n)
+        self.ptr = ctypes.c_void_p()
+        if H.hipMalloc(ctypes.byref(self.ptr), max(8, 8 * self.n)) != 0:
+            raise SediError(-3, "hipMalloc of %d doubles failed" % self.n)
+        if host is not None and H.hipMemcpy(self.ptr, host.ctypes.data, 8 * self.n, 1) != 0:
+            raise SediError(-2, "hipMemcpy to the device failed")
+
+    def get(self):
+        out = np.empty(self.n)
+        if _hiprt().hipMemcpy(out.ctypes.data, self.ptr, 8 * self.n, 2) != 0:
+            raise SediError(-2, "hipMemcpy from the device failed")
+        return out
+
+    def fill_bytes(self, value=0):
+        """hipMemset of the whole array (synchronous)."""
+        if _hiprt().hipMemset(self.ptr, int(value), 8 * self.n) != 0:
+            raise SediError(-2, "hipMemset failed")
+
+    def free(self):
+        if self.ptr:
+            _hiprt().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Stencil:

@@ -94,16 +94,21 @@ constexpr int ZC_CHOICES[3] = {8, 16, 32};
 // The launch plan the autotuner may override per kernel and grid: "zc:<kernel>"
 // the planes per tile, "sched:<kernel>" 1 for as many workgroups as the
 // partial sums allow (up to 8 per CU) in launch order instead of the resident
-// set walking XCD runs.
-inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sched, int max_blocks) {
+// set walking XCD runs, 2 for one workgroup per tile in launch order (no
+// grid-stride loop; the hardware dispatcher hands out tiles as workgroups
+// retire) where the kernel has no partial sums to bound its workgroups
+// (`one_per_tile` false: as 1).
+inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sched, int max_blocks,
+                            bool one_per_tile = false) {
     KGeom k = planned;
     if (zc == 8 || zc == 16 || zc == 32) {
         k.tg = make_tile_grid(s, zc, planned.tg.ty);
         k.nblocks = (int)std::min<long long>(k.tg.ntiles, planned.nblocks);
     }
-    if (sched == 1) {
+    if (sched == 1 || sched == 2) {
         k.tg.xcd = 0;
-        k.nblocks = (int)std::min<long long>(k.tg.ntiles, max_blocks);
+        const long long cap = (sched == 2 && one_per_tile) ? k.tg.ntiles : (long long)max_blocks;
+        k.nblocks = (int)std::min<long long>(k.tg.ntiles, cap);
     }
     return k;
 }

@@ -91,9 +91,13 @@ __global__ void cg_flush_kernel(const DevState* st, long long n, double* x, cons
 // ---------------------------------------------------------------------------
 // passes
 // ---------------------------------------------------------------------------
-// y = A x (Stencil.hpp:38-54).
-struct PassApply {
-    static constexpr int ND = 1, D = SEDI_PREFETCH_APPLY;
+// y = A x (Stencil.hpp:38-54).  DD planes of loads in flight (the
+// standalone stencil's only tuning beyond the tile: each wave streams one
+// 512-byte row per plane, so a few planes ahead keep enough bytes in flight
+// per CU to cover HBM latency).
+template <int DD = SEDI_PREFETCH_APPLY>
+struct PassApplyT {
+    static constexpr int ND = 1, D = DD;
     double a, b, c;
     const double* x;
     double* y;
@@ -106,6 +110,7 @@ struct PassApply {
         stc_nt(y, i, Af);
     }
 };
+using PassApply = PassApplyT<>;
 
 // r = b - A x  (Cg.hpp:119-123; PipelinedCg.hpp:113-117; BiCgStab.hpp:131-135)
 // with the dots <b,b>, <r,r> and <r, r.*iM>; optionally r_hat = r
@@ -365,12 +370,31 @@ constexpr int PTHREADS = TX * PTY;
 // faster than 8 rows, with y written non-temporally.
 constexpr int STY = 16;
 
-template <int ZC>
+template <int ZC, int PF = SEDI_PREFETCH_APPLY>
 __global__ __launch_bounds__(TX * STY) void stencil_apply_kernel(TileGrid g, double a, double b,
                                                                  double c, const double* x,
                                                                  double* y) {
     __shared__ double lds[2][STY + 2][TX + 2];
-    PassApply P{a, b, c, x, y};
+    PassApplyT<PF> P{a, b, c, x, y};
+    double acc[1];
+    run_tiles<decltype(P), ZC, STY>(P, g, lds, acc);
+}
+
+// the stencil with 4 and 8 planes of loads in flight, by chunk (for the
+// launch macro, which takes a template over the chunk alone)
+template <int ZC>
+__global__ __launch_bounds__(TX * STY) void stencil_apply_kernel_pf4(TileGrid g, double a, double b, double c,
+                                                                     const double* x, double* y) {
+    __shared__ double lds[2][STY + 2][TX + 2];
+    PassApplyT<4> P{a, b, c, x, y};
+    double acc[1];
+    run_tiles<decltype(P), ZC, STY>(P, g, lds, acc);
+}
+template <int ZC>
+__global__ __launch_bounds__(TX * STY) void stencil_apply_kernel_pf8(TileGrid g, double a, double b, double c,
+                                                                     const double* x, double* y) {
+    __shared__ double lds[2][STY + 2][TX + 2];
+    PassApplyT<8> P{a, b, c, x, y};
     double acc[1];
     run_tiles<decltype(P), ZC, STY>(P, g, lds, acc);
 }
@@ -620,7 +644,8 @@ constexpr int MAX_BLOCKS_PER_CU = 8;
 // Geometry of a pass on `s`, from the occupancy of its ZC=16 instantiation
 // (the ZC variants differ only in unrolling); cached per kernel and shape.
 template <typename K>
-static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, int ty = PTY) {
+static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, int ty = PTY,
+                             bool one_per_tile = false) {
     static std::mutex mu;
     static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
     const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
@@ -638,7 +663,7 @@ static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, 
     const KGeom planned = plan_geometry(s, occ * num_cus(), D, ty);
     const int zc = tuned_value(std::string("zc:") + name, s, 0);
     const int sched = tuned_value(std::string("sched:") + name, s, 0);
-    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, MAX_BLOCKS_PER_CU * num_cus()))
+    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, MAX_BLOCKS_PER_CU * num_cus(), one_per_tile))
         .first->second;
 }
 
@@ -656,8 +681,27 @@ int stencil_apply_dev(const Shape& s, double a, double b, double c, const double
                       hipStream_t stream) {
     SEDI_REQUIRE(offsets_fit_32(s.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
                  "grid of %lld cells too large: an FP64 vector must be < 4 GiB (32-bit offsets)", s.n());
-    const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY);
-    SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel, k, stream, k.tg, a, b, c, x, y));
+    // no partial sums: one workgroup per tile is a schedule choice too
+    const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY, true);
+    // planes of loads in flight: the autotuner's "pf:stencil_apply_kernel" (2, 4 or 8)
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, int> pf_cache;
+    int pf;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = pf_cache.find(std::make_tuple(s.n1, s.n2, s.n3));
+        if (it == pf_cache.end())
+            it = pf_cache.emplace(std::make_tuple(s.n1, s.n2, s.n3),
+                                  tuned_value("pf:stencil_apply_kernel", s, SEDI_PREFETCH_APPLY)).first;
+        pf = it->second;
+    }
+    if (pf == 8) {
+        SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel_pf8, k, stream, k.tg, a, b, c, x, y));
+    } else if (pf == 4) {
+        SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel_pf4, k, stream, k.tg, a, b, c, x, y));
+    } else {
+        SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel, k, stream, k.tg, a, b, c, x, y));
+    }
     return SEDI_OK;
 }
 
