@@ -22,8 +22,16 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
    solves, which real solves hide but short timing runs would not (at 256^3 a
    probe takes ~48 ms, 240 plain iterations).
 
+3. the standalone stencil's plan: chunk (8, 16, 32) x schedule (0, 1, and 2:
+   one workgroup per tile, which only a kernel without partial sums can use)
+   x planes of loads in flight (2, 4, 8), each forced in a child process and
+   timed cold -- launches rotate through (x, y) pairs spanning 1 GiB of device
+   memory, so x comes from HBM as in a sweep over vectors larger than the
+   Infinity Cache -- replacing step 1's stencil entries.
+
 ``--k-only`` re-measures the batch sizes alone, with the chunk and schedule
-entries of the existing table (--out), which it keeps.
+entries of the existing table (--out), which it keeps; ``--stencil-only``
+re-measures step 3 alone and keeps everything else.
 
 The table is data: a run's results depend only on it and the grid, so the
 same table gives the same results on every run.
@@ -125,6 +133,62 @@ def measure_chunks(sides, iters):
     return out
 
 
+STENCIL_PLANS = [(zc, sched, pf) for zc in (8, 16, 32) for sched in (0, 1, 2) for pf in (2, 4, 8)]
+
+
+def measure_stencil(sides, reps):
+    """Child: per grid, the mean launch time of the standalone stencil, cold
+    (launches rotate through (x, y) pairs spanning 1 GiB)."""
+    import ctypes
+    import numpy as np
+    import sedi_amd
+    out = {}
+    L = sedi_amd.lib()
+    st = sedi_amd.Stencil()._c()
+    for side in sides:
+        n = side ** 3
+        g, _ = sedi_amd._grid((side,) * 3)
+        npairs = max(2, -(-(1 << 30) // (16 * n)))
+        x0 = np.random.default_rng(0).standard_normal(n)
+        pairs = [(sedi_amd.DeviceArray(x0), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
+        for x, y in pairs:
+            if L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), x.ptr, y.ptr) < 0:
+                raise RuntimeError(L.sedi_last_error().decode())
+        sedi_amd.profile_reset()
+        sedi_amd.profile_enable("stencil_apply_kernel")
+        for r in range(reps):
+            x, y = pairs[r % npairs]
+            L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), x.ptr, y.ptr)
+        sedi_amd.profile_enable(None)
+        launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
+        out[side] = 1e3 * ms / launches
+        for x, y in pairs:
+            x.free()
+            y.free()
+    return out
+
+
+def stencil_lines(args):
+    """Step 3: the stencil's best (chunk, schedule, prefetch) per grid."""
+    t = {}
+    for zc, sched, pf in STENCIL_PLANS:
+        print("stencil: z chunk %d, schedule %d, prefetch %d" % (zc, sched, pf), file=sys.stderr, flush=True)
+        t[(zc, sched, pf)] = child(args, ["zc:stencil_apply_kernel 0 0 0 %d" % zc,
+                                          "sched:stencil_apply_kernel 0 0 0 %d" % sched,
+                                          "pf:stencil_apply_kernel 0 0 0 %d" % pf], "stencil")
+    lines = []
+    for side in args.sides:
+        tt = {c: t[c][str(side)] for c in t}
+        zc, sched, pf = min(tt, key=tt.get)
+        best = sorted(tt, key=tt.get)[:4]
+        note = " ".join("%d/%d/%d:%.1f" % (c[0], c[1], c[2], tt[c]) for c in best)
+        lines.append("zc:stencil_apply_kernel %d %d %d %d   # cold us per launch (chunk/schedule/prefetch), "
+                     "best four: %s" % (side, side, side, zc, note))
+        lines.append("sched:stencil_apply_kernel %d %d %d %d" % (side, side, side, sched))
+        lines.append("pf:stencil_apply_kernel %d %d %d %d" % (side, side, side, pf))
+    return lines
+
+
 def measure_batches(sides, iters):
     """Child: per (solver, precision, grid), probe-iterations/s for each k."""
     import sedi_amd
@@ -179,16 +243,32 @@ def main():
     p.add_argument("--out", default=os.path.join(PKG, "tuning.txt"))
     p.add_argument("--k-only", action="store_true",
                    help="re-measure the batch sizes only, keeping the table's chunk and schedule entries")
-    p.add_argument("--child", choices=["chunks", "batches"], help=argparse.SUPPRESS)
+    p.add_argument("--stencil-only", action="store_true",
+                   help="re-measure the standalone stencil's plan only, keeping every other entry")
+    p.add_argument("--child", choices=["chunks", "batches", "stencil"], help=argparse.SUPPRESS)
     args = p.parse_args()
     if args.child == "chunks":
         print(json.dumps(measure_chunks(args.sides, args.iters)))
+        return
+    if args.child == "stencil":
+        print(json.dumps(measure_stencil(args.sides, 12)))
         return
     if args.child == "batches":
         print(json.dumps(measure_batches(args.sides, args.iters)))
         return
 
     t0 = time.time()
+    if args.stencil_only:
+        with open(args.out) as f:
+            kept = [l.rstrip("\n") for l in f if l.strip() and ":stencil_apply_kernel " not in l]
+        head = [l for l in kept if l.startswith("#")]
+        lines = [l for l in kept if not l.startswith("#")] + stencil_lines(args)
+        head.append("# stencil entries re-measured by --stencil-only (cold, sides %s); %s; %.0f s" % (
+            ",".join(str(v) for v in args.sides), time.strftime("%Y-%m-%d"), time.time() - t0))
+        with open(args.out, "w") as f:
+            f.write("\n".join(head + lines) + "\n")
+        print("wrote %s (%d entries)" % (args.out, len(lines)), file=sys.stderr, flush=True)
+        return
     if args.k_only:
         with open(args.out) as f:
             kept = [l.rstrip("\n") for l in f if l.strip() and not l.startswith("k:")]
@@ -215,6 +295,8 @@ def main():
     for side in args.sides:
         names = sorted(set().union(*(timings[c].get(str(side), {}).keys() for c in timings)))
         for name in names:
+            if name == "stencil_apply_kernel":
+                continue  # step 3
             t = {c: timings[c][str(side)][name] for c in timings if name in timings[c].get(str(side), {})}
             zc, sched = min(t, key=t.get)
             note = " ".join("%d/%d:%.1f" % (c[0], c[1], t[c]) for c in sorted(t))
@@ -226,6 +308,7 @@ def main():
         s = b["side"]
         lines.append("k:%d:%d %d %d %d %d   # probe-iterations/s: %s" % (
             b["kind"], b["precision"], s, s, s, b["k"], " ".join("%s:%s" % kv for kv in b["rates"].items())))
+    lines += stencil_lines(args)
     dev = "unknown device"
     try:
         import torch
@@ -236,7 +319,8 @@ def main():
                 ",".join(str(s) for s in args.sides), args.iters, args.k_iters),
             "# device: %s; %s; %.0f s" % (dev, time.strftime("%Y-%m-%d"), time.time() - t0),
             "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, sched:<kernel> = 1 for tiles in",
-            "# launch order (0: XCD-contiguous runs), k:<solver>:<precision> = probes per batch",
+            "# launch order (0: XCD-contiguous runs; 2: one workgroup per tile, stencil only), pf:<kernel> =",
+            "# planes of loads in flight (stencil), k:<solver>:<precision> = probes per batch",
             "# (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
     with open(args.out, "w") as f:
         f.write("\n".join(head + lines) + "\n")

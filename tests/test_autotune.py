@@ -24,11 +24,19 @@ BATCHES = [dict(side=64, kind=2, precision=1, k=32, rates={"8": 1.0, "32": 3.0, 
            dict(side=64, kind=2, precision=0, k=1, rates={"1": 5.0, "8": 4.0})]
 
 
+def stencil_time(zc, sched, pf):
+    # fastest plan: chunk 32, one workgroup per tile, prefetch 4
+    return 100.0 + abs(zc - 32) + 3 * abs(sched - 2) + abs(pf - 4)
+
+
 def fake_child(calls):
     def child(args, table_lines, mode):
         calls.append((mode, list(table_lines)))
         if mode == "batches":
             return BATCHES
+        if mode == "stencil":
+            zc, sched, pf = (int(l.split()[-1]) for l in table_lines)
+            return {"64": stencil_time(zc, sched, pf)}
         zc = int(table_lines[0].split()[-1])
         sched = int(table_lines[1].split()[-1])
         return TIMES[(zc, sched)]
@@ -54,9 +62,14 @@ def test_table_takes_fastest_plan_and_best_batch(tmp_path, monkeypatch):
     assert e[("zc:batch_upd_kernel", 64)] == 32 and e[("sched:batch_upd_kernel", 64)] == 1
     assert e[("zc:cg_update_kernel", 64)] == 8 and e[("sched:cg_update_kernel", 64)] == 0
     assert e[("k:2:1", 64)] == 32 and e[("k:2:0", 64)] == 1
-    # six plans measured, then the batch sizes under the chosen plans
-    assert [c[0] for c in calls] == ["chunks"] * 6 + ["batches"]
-    assert "zc:batch_upd_kernel 64 64 64 32" in [l.strip() for l in calls[-1][1]]
+    # the stencil's own plan: chunk, schedule (incl. one workgroup per tile)
+    # and prefetch, from its cold timings
+    assert e[("zc:stencil_apply_kernel", 64)] == 32 and e[("sched:stencil_apply_kernel", 64)] == 2
+    assert e[("pf:stencil_apply_kernel", 64)] == 4
+    # six plans measured, then the batch sizes under the chosen plans, then
+    # the 27 stencil plans
+    assert [c[0] for c in calls] == ["chunks"] * 6 + ["batches"] + ["stencil"] * 27
+    assert "zc:batch_upd_kernel 64 64 64 32" in [l.strip() for l in calls[6][1]]
 
 
 def test_k_only_keeps_plans(tmp_path, monkeypatch):
@@ -74,3 +87,19 @@ def test_k_only_keeps_plans(tmp_path, monkeypatch):
     text = out.read_text()
     assert text.startswith("# header\n") and "k entries re-measured by --k-only" in text
     assert text.count("k:2:1 ") == 1
+
+
+def test_stencil_only_keeps_everything_else(tmp_path, monkeypatch):
+    out = tmp_path / "tuning.txt"
+    out.write_text("# header\nzc:batch_upd_kernel 64 64 64 16   # note\nk:2:1 64 64 64 8\n"
+                   "zc:stencil_apply_kernel 64 64 64 8\nsched:stencil_apply_kernel 64 64 64 0\n")
+    calls = []
+    monkeypatch.setattr(autotune, "child", fake_child(calls))
+    monkeypatch.setattr(sys, "argv", ["autotune", "--sides", "64", "--stencil-only", "--out", str(out)])
+    autotune.main()
+    e = entries(out)
+    assert e[("zc:batch_upd_kernel", 64)] == 16 and e[("k:2:1", 64)] == 8
+    assert (e[("zc:stencil_apply_kernel", 64)], e[("sched:stencil_apply_kernel", 64)],
+            e[("pf:stencil_apply_kernel", 64)]) == (32, 2, 4)
+    assert [c[0] for c in calls] == ["stencil"] * 27
+    assert out.read_text().count("zc:stencil_apply_kernel") == 1
