@@ -80,6 +80,23 @@ def build(verbose=True, extra=(), lib=LIB, build_dir=BUILD, driver=True):
     return LIB
 
 
+VARIANTS = {
+    # the cross-workgroup reduction published with a release fence and read
+    # with an acquire fence (the C++ memory model's own guarantee) instead of
+    # agent-scope atomic stores and a wait on their completion
+    "fence": ["-DSEDI_GRID_SUM_FENCE"],
+}
+
+
+def build_variant(name, verbose=False):
+    """Builds sedi_amd/variants/libsedi_amd_<name>.so with the flags of
+    VARIANTS[name] (no CLI driver); tests load it through SEDI_AMD_LIB."""
+    out_dir = os.path.join(PKG, "variants")
+    os.makedirs(out_dir, exist_ok=True)
+    return build(verbose=verbose, extra=VARIANTS[name], lib=os.path.join(out_dir, "libsedi_amd_%s.so" % name),
+                 build_dir=os.path.join(BUILD, "variant_" + name), driver=False)
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
 
