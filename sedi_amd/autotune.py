@@ -7,7 +7,9 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
 
 1. z chunk and schedule: for each of 8, 16 and 32 planes and each schedule
    (0: the resident workgroups walk XCD-contiguous runs of tiles; 1: up to 8
-   workgroups per CU take tiles in launch order), a child process runs every
+   workgroups per CU take tiles in launch order; 3 and 4: as 1, with the tiles
+   dealt to the XCDs in chunks of 4 or 8 rows of tiles, so a tile's y
+   neighbours run on its XCD and its halo rows hit that XCD's L2), a child process runs every
    solver (plain FP64 and batched mixed, CG / pipelined CG / BiCGStab) for a
    fixed number of iterations on each grid, and applies the standalone
    stencil, with that chunk and schedule forced on every kernel
@@ -134,6 +136,10 @@ def measure_chunks(sides, iters):
 
 
 STENCIL_PLANS = [(zc, sched, pf) for zc in (8, 16, 32) for sched in (0, 1, 2) for pf in (2, 4, 8)]
+# schedules of the solver kernels (march.h tuned_geometry): 0 XCD-contiguous
+# runs, 1 launch order, 3 and 4 launch order with tiles dealt to the XCDs in
+# chunks of 4 or 8 rows of tiles
+SCHEDS = (0, 1, 3, 4)
 
 
 def measure_stencil(sides, reps):
@@ -288,7 +294,7 @@ def main():
         return
     timings = {}
     for zc in (8, 16, 32):
-        for sched in (0, 1):
+        for sched in SCHEDS:
             print("z chunk %d, schedule %d" % (zc, sched), file=sys.stderr, flush=True)
             timings[(zc, sched)] = child(args, ["zc:* 0 0 0 %d" % zc, "sched:* 0 0 0 %d" % sched], "chunks")
     lines = []
@@ -319,7 +325,8 @@ def main():
                 ",".join(str(s) for s in args.sides), args.iters, args.k_iters),
             "# device: %s; %s; %.0f s" % (dev, time.strftime("%Y-%m-%d"), time.time() - t0),
             "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, sched:<kernel> = 1 for tiles in",
-            "# launch order (0: XCD-contiguous runs; 2: one workgroup per tile, stencil only), pf:<kernel> =",
+            "# launch order (0: XCD-contiguous runs; 2: one workgroup per tile, stencil only; 3/4: launch order",
+            "# with tiles dealt to the XCDs in chunks of 4/8 tile rows), pf:<kernel> =",
             "# planes of loads in flight (stencil), k:<solver>:<precision> = probes per batch",
             "# (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
     with open(args.out, "w") as f:

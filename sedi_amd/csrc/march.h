@@ -58,7 +58,9 @@ constexpr int NXCD = 8;
 struct TileGrid {
     int n1, n2, n3;
     int nbx, nby, nbz, zc, ty;
-    int xcd;  // 1: each XCD takes a contiguous run of tiles; 0: tiles in launch order
+    int xcd;  // 1: each XCD takes a contiguous run of tiles; 2: chunks of xg tiles
+              // dealt round-robin to the XCDs; 0: tiles in launch order
+    int xg;   // xcd == 2: tiles per chunk (whole rows of tiles along x)
     long long s2, s3;
     long long ntiles;
 };
@@ -71,6 +73,7 @@ inline TileGrid make_tile_grid(const Shape& s, int zc, int ty = TY) {
     t.zc = zc;
     t.ty = ty;
     t.xcd = 1;
+    t.xg = t.nbx;
     t.nbz = (s.n3 + zc - 1) / zc;
     t.s2 = s.n1;
     t.s3 = (long long)s.n1 * s.n2;
@@ -97,7 +100,11 @@ constexpr int ZC_CHOICES[3] = {8, 16, 32};
 // set walking XCD runs, 2 for one workgroup per tile in launch order (no
 // grid-stride loop; the hardware dispatcher hands out tiles as workgroups
 // retire) where the kernel has no partial sums to bound its workgroups
-// (`one_per_tile` false: as 1).
+// (`one_per_tile` false: as 1), 3 and 4 as 1 but with the tiles dealt to the
+// XCDs in chunks of 4 or 8 rows of tiles: a tile and its y neighbours then
+// run on one XCD at about the same time, so the halo rows one of them loads
+// are in that XCD's L2 for the others, while the eight XCDs still work on
+// one region of memory.
 inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sched, int max_blocks,
                             bool one_per_tile = false) {
     KGeom k = planned;
@@ -105,10 +112,17 @@ inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sc
         k.tg = make_tile_grid(s, zc, planned.tg.ty);
         k.nblocks = (int)std::min<long long>(k.tg.ntiles, planned.nblocks);
     }
-    if (sched == 1 || sched == 2) {
+    if (sched >= 1 && sched <= 4) {
         k.tg.xcd = 0;
         const long long cap = (sched == 2 && one_per_tile) ? k.tg.ntiles : (long long)max_blocks;
         k.nblocks = (int)std::min<long long>(k.tg.ntiles, cap);
+        if (sched >= 3) {
+            k.tg.xcd = 2;
+            k.tg.xg = k.tg.nbx * std::min(k.tg.nby, sched == 3 ? 4 : 8);
+            // the dispatcher deals workgroup b to XCD b % 8: the chunked order
+            // needs every XCD to take whole rounds
+            k.nblocks = std::max(NXCD, k.nblocks / NXCD * NXCD);
+        }
     }
     return k;
 }
@@ -244,12 +258,25 @@ __device__ __forceinline__ void lds_barrier() {
 // pass starts on the tiles the previous pass touched last, whose vectors are
 // still in the 256 MB Infinity Cache.
 __device__ __forceinline__ long long tile_of(const TileGrid& g, long long t, bool reverse) {
-    if (g.xcd && (g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
+    if (g.xcd == 1 && (g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
         const long long per = g.ntiles / NXCD;
         const long long k = t / NXCD;
         return (t % NXCD) * per + (reverse ? per - 1 - k : k);
     }
-    return reverse ? g.ntiles - 1 - t : t;
+    const long long u = reverse ? g.ntiles - 1 - t : t;
+    if (g.xcd == 2 && (g.ntiles % NXCD) == 0 && (gridDim.x % NXCD) == 0) {
+        // step u runs on XCD u % 8 (as ntiles and the grid are multiples of
+        // 8, for both walking directions); its k-th step takes tile k % xg of
+        // chunk (k / xg) * 8 + u % 8.  Tiles past the last whole round of
+        // 8 chunks keep their place.  A bijection of [0, ntiles).
+        const long long span = (long long)g.xg * NXCD;
+        const long long full = g.ntiles / span * span;
+        if (u < full) {
+            const long long k = u / NXCD;
+            return ((k / g.xg) * NXCD + (u % NXCD)) * g.xg + (k % g.xg);
+        }
+    }
+    return u;
 }
 
 // ---- 32-bit cell offsets ---------------------------------------------------

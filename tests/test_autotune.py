@@ -13,13 +13,17 @@ autotune = pytest.importorskip("sedi_amd.autotune")
 
 # launch times (us) per (chunk, schedule), grid side and kernel
 TIMES = {
-    (8, 0): {"64": {"batch_upd_kernel": 70.0, "cg_update_kernel": 9.0}},
-    (8, 1): {"64": {"batch_upd_kernel": 69.0, "cg_update_kernel": 12.0}},
-    (16, 0): {"64": {"batch_upd_kernel": 60.0, "cg_update_kernel": 10.0}},
-    (16, 1): {"64": {"batch_upd_kernel": 61.0, "cg_update_kernel": 11.0}},
-    (32, 0): {"64": {"batch_upd_kernel": 65.0, "cg_update_kernel": 13.0}},
-    (32, 1): {"64": {"batch_upd_kernel": 58.5, "cg_update_kernel": 14.0}},
+    (8, 0): {"64": {"batch_upd_kernel": 70.0, "cg_update_kernel": 9.0, "batch_dir_kernel": 30.0}},
+    (8, 1): {"64": {"batch_upd_kernel": 69.0, "cg_update_kernel": 12.0, "batch_dir_kernel": 30.0}},
+    (16, 0): {"64": {"batch_upd_kernel": 60.0, "cg_update_kernel": 10.0, "batch_dir_kernel": 30.0}},
+    (16, 1): {"64": {"batch_upd_kernel": 61.0, "cg_update_kernel": 11.0, "batch_dir_kernel": 30.0}},
+    (32, 0): {"64": {"batch_upd_kernel": 65.0, "cg_update_kernel": 13.0, "batch_dir_kernel": 30.0}},
+    (32, 1): {"64": {"batch_upd_kernel": 58.5, "cg_update_kernel": 14.0, "batch_dir_kernel": 30.0}},
 }
+for zc in (8, 16, 32):
+    for sched in (3, 4):
+        TIMES[(zc, sched)] = {"64": {"batch_upd_kernel": 80.0, "cg_update_kernel": 20.0,
+                                     "batch_dir_kernel": 29.0 if (zc, sched) == (16, 3) else 31.0}}
 BATCHES = [dict(side=64, kind=2, precision=1, k=32, rates={"8": 1.0, "32": 3.0, "64": 2.0}),
            dict(side=64, kind=2, precision=0, k=1, rates={"1": 5.0, "8": 4.0})]
 
@@ -62,14 +66,16 @@ def test_table_takes_fastest_plan_and_best_batch(tmp_path, monkeypatch):
     assert e[("zc:batch_upd_kernel", 64)] == 32 and e[("sched:batch_upd_kernel", 64)] == 1
     assert e[("zc:cg_update_kernel", 64)] == 8 and e[("sched:cg_update_kernel", 64)] == 0
     assert e[("k:2:1", 64)] == 32 and e[("k:2:0", 64)] == 1
+    # the chunked XCD order is a schedule choice like the others
+    assert e[("zc:batch_dir_kernel", 64)] == 16 and e[("sched:batch_dir_kernel", 64)] == 3
     # the stencil's own plan: chunk, schedule (incl. one workgroup per tile)
     # and prefetch, from its cold timings
     assert e[("zc:stencil_apply_kernel", 64)] == 32 and e[("sched:stencil_apply_kernel", 64)] == 2
     assert e[("pf:stencil_apply_kernel", 64)] == 4
-    # six plans measured, then the batch sizes under the chosen plans, then
+    # twelve plans measured, then the batch sizes under the chosen plans, then
     # the 27 stencil plans
-    assert [c[0] for c in calls] == ["chunks"] * 6 + ["batches"] + ["stencil"] * 27
-    assert "zc:batch_upd_kernel 64 64 64 32" in [l.strip() for l in calls[6][1]]
+    assert [c[0] for c in calls] == ["chunks"] * 12 + ["batches"] + ["stencil"] * 27
+    assert "zc:batch_upd_kernel 64 64 64 32" in [l.strip() for l in calls[12][1]]
 
 
 def test_k_only_keeps_plans(tmp_path, monkeypatch):

@@ -2,7 +2,9 @@
 
 The tuning table (sedi_amd/tuning.txt) sets per kernel and grid the planes per
 tile (8, 16, 32) and the schedule (0: resident workgroups walk XCD-contiguous
-runs of tiles; 1: tiles in launch order, more workgroups), and the rows per
+runs of tiles; 1: tiles in launch order, more workgroups; 2: one workgroup
+per tile where a kernel has no partial sums, else as 1; 3/4: launch order with
+the tiles dealt to the XCDs in chunks of 4/8 tile rows), and the rows per
 tile of the FP32 batched CG update pass (8, or 16 by default from 256 rows up).
 The test grids have no entries of their own, so here a child process forces
 one plan on every kernel (SEDI_TUNING_FILE, read once per process; one plan
@@ -52,7 +54,10 @@ for kind in (1, 2, 3):
 print(json.dumps(out))
 """
 
-PLANS = [(8, 1, 0), (32, 0, 16), (16, 1, 0)]  # z chunk, schedule, update-pass rows (0: default)
+# z chunk, schedule, update-pass rows (0: default); schedules 3 and 4 deal
+# the tiles to the XCDs in chunks of 4 and 8 tile rows (2 is one workgroup per
+# tile, which only the stencil takes; elsewhere it acts as 1)
+PLANS = [(8, 1, 0), (32, 0, 16), (16, 1, 0), (16, 3, 0), (8, 4, 16), (32, 2, 0)]
 
 
 @pytest.fixture(scope="module")
@@ -66,7 +71,9 @@ def batched_default():
 @pytest.mark.parametrize("zc,sched,ty", PLANS)
 def test_forced_launch_plan_keeps_parity(tmp_path, ref32, ref_spread, batched_default, zc, sched, ty):
     table = tmp_path / "tuning.txt"
-    table.write_text("zc:* 0 0 0 %d\nsched:* 0 0 0 %d\n" % (zc, sched) +
+    # the stencil's loads in flight vary with the plan too (2, 4 or 8 planes)
+    pf = {2: 8, 4: 4}.get(sched, 2)
+    table.write_text("zc:* 0 0 0 %d\nsched:* 0 0 0 %d\npf:stencil_apply_kernel 0 0 0 %d\n" % (zc, sched, pf) +
                      ("ty:batch_upd_kernel 0 0 0 %d\n" % ty if ty else ""))
     env = dict(os.environ, SEDI_TUNING_FILE=str(table))
     code = CHILD % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), tmp=str(tmp_path))
