@@ -13,9 +13,13 @@ inputs are resident in HBM when the timed region starts.
 
 Beside it, on the same grid, stream and tolerance: the plain FP64
 one-probe-at-a-time CG (the drop-in) on the first 16 probes, the speed-up of
-the batched form over it, and the max / RMS relative error of the mixed
-form's diagonal against the FP64 one on those 16 probes; and the p50 / p99
-job latency of a 30-job sample of the serving stream (--serve-jobs).
+the batched form over it, and the error of the mixed form's diagonal against
+the FP64 one on those 16 probes (per cell relative to the exact diag(A^-1),
+tests/golden/exact_diag_256cube.npz, and normwise); the p50 / p99 job
+latency of the serving stream, on its first 100 jobs at 5 jobs/s (saturated,
+--serve-jobs) and its first 40 at 0.5 jobs/s (below saturation,
+--serve-sub-jobs/--serve-sub-rate); and the standalone stencil on device
+arrays, 256^3 warm and cold and 512^3 cold (--no-stencil-line to skip).
 
 Prints one JSON line with the throughput, the roofline of the dominant kernel
 (its algorithmic bytes per launch over its average duration, measured with HIP
@@ -104,6 +108,11 @@ def parse():
     p.add_argument("--stencil-side", type=int, default=0, help="--stencil: grid side (default 256)")
     p.add_argument("--stencil", action="store_true",
                    help="time the standalone 7-point stencil kernel (y = A x, FP64) at 256^3 and exit")
+    p.add_argument("--copy", action="store_true",
+                   help="time the HIP runtime's device-to-device copy of one vector (the streaming ceiling "
+                        "the stencil is compared with; --stencil-side, --cold as for --stencil) and exit")
+    p.add_argument("--no-stencil-line", dest="stencil_line", action="store_false",
+                   help="headline: skip the standalone stencil measurements reported beside it")
     p.add_argument("--cold", action="store_true",
                    help="--stencil: rotate through (x, y) pairs spanning 1 GiB so x comes from HBM")
     p.add_argument("--pmc-workload", action="store_true",
@@ -183,6 +192,53 @@ def stencil_only(reps=20, side=None, cold=False):
             "achieved_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def copy_only(reps=12, side=None, cold=False):
+    """The practical streaming ceiling beside the stencil: the HIP runtime's
+    own device-to-device copy (hipMemcpyAsync) of one side^3 FP64 vector, 2N*8
+    bytes moved, timed with HIP events; warm on one pair, cold rotating
+    through pairs spanning 1 GiB as stencil_only does."""
+    import ctypes
+    import sedi_amd
+    H = sedi_amd._hiprt()
+    H.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    H.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    H.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    H.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    H.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    grid = (side,) * 3 if side else GRID
+    n = grid[0] * grid[1] * grid[2]
+    npairs = max(2, -(-COLD_BYTES // (16 * n))) if cold else 1
+    pairs = [(sedi_amd.DeviceArray(n=n), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
+    for a, b in pairs:
+        a.fill_bytes(1)
+        b.fill_bytes(2)
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    H.hipEventCreate(ctypes.byref(e0))
+    H.hipEventCreate(ctypes.byref(e1))
+    times = []
+    for r in range(reps + npairs):
+        a, b = pairs[r % npairs]
+        H.hipEventRecord(e0, None)
+        H.hipMemcpyAsync(b.ptr, a.ptr, 8 * n, 3, None)  # device to device
+        H.hipEventRecord(e1, None)
+        H.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        H.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        if r >= npairs:
+            times.append(ms.value)
+    H.hipEventDestroy(e0)
+    H.hipEventDestroy(e1)
+    for a, b in pairs:
+        a.free()
+        b.free()
+    avg = sum(times) / len(times) * 1e-3
+    gbs = 2 * n * 8 / avg / 1e9
+    return {"kernel": "hipMemcpyAsync device to device", "grid": list(grid), "cache": "cold" if cold else "warm",
+            "pairs": npairs, "launches": len(times), "avg_launch_us": round(avg * 1e6, 2),
+            "achieved_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def traffic_ratio_from_profiles(kernel):
     """Measured HBM bytes over algorithmic bytes of a launch, from the committed
     rocprofv3 --pmc summary of the kernel (calibrated FETCH_SIZE + WRITE_SIZE
@@ -243,8 +299,23 @@ def cpu_baseline(kind, seed, tol, gpu_iters_mean, iters_cap):
     secs, iters = float(m.group(1)), int(m.group(2))
     per_iter = secs / max(iters, 1)
     probe_secs = per_iter * gpu_iters_mean
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            m = re.search(r"^model name\s*:\s*(.+)$", f.read(), re.M)
+            cpu = m.group(1).strip() if m else cpu
+    except OSError:
+        pass
+    build = ""
+    try:
+        with open(os.path.join(ROOT, "oracle", "_ref", "BUILD_LINE.txt")) as f:
+            build = f.read().strip()
+    except OSError:
+        pass
     return {"value": round(1.0 / probe_secs, 5), "unit": "probe-solves/s", "cores": p,
-            "kind": "reference",
+            "kind": "reference", "cpu": cpu,
+            "build": build + " (g++ 11 without -march: 2-lane SSE2 dot products, where the reference's "
+                             "Makefile builds with icpc -xHost; a bounded, scaled sample, not a tuned CPU run)",
             "sample": ("1 probe of the same workload (256^3, seed %d, probe 0), plain FP64 %s, "
                        "capped at %d iterations (-m %d -i) under mpirun -n %d -p %s: %.2f s = "
                        "%.2f ms/iteration, scaled to the GPU run's mean of %.1f iterations per "
@@ -367,6 +438,11 @@ def run_headline(args):
                           "unconverged": int(np.count_nonzero(codes))}
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
     line["roofline"] = roofline(BATCH_KERNEL, launches, kms, rhs_values * n * 4)
+    if args.stencil_line:
+        # the standalone 7-point stencil on device-resident FP64 vectors: warm
+        # (back-to-back on one pair) and cold (x from HBM) at 256^3, cold at 512^3
+        line["stencil"] = [stencil_only(side=256), stencil_only(side=256, cold=True),
+                           stencil_only(side=512, cold=True)]
     if not args.no_compare:
         # plain FP64 drop-in and the mixed form on the first 16 probes of the
         # same stream: the speed-up and the diagonal error of the mixed form
@@ -590,6 +666,9 @@ def main():
         return
     if args.stencil:
         print(json.dumps(stencil_only(side=args.stencil_side or None, cold=args.cold)), flush=True)
+        return
+    if args.copy:
+        print(json.dumps(copy_only(side=args.stencil_side or None, cold=args.cold)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "

@@ -30,7 +30,9 @@ prints and writes as .npz files next to this script:
                     with D's summary and 4096 strided cells; and the first 4
                     probes of each solver at -p 1x1x1 (SSE2, and AVX-512 for
                     BiCGStab): iterations and final errors, beside the -p 2x2x2
-                    run of ref_256cube.npz.
+                    run of ref_256cube.npz.  (--big-fixed) adds BiCGStab
+                    for -m 100 -i at -p 1x1x8 and -m 20 -i at both
+                    decompositions.
 
   ref_breakdown.json (--breakdown) BiCGStab on small systems built so that
                     its breakdowns happen in exact arithmetic (oracle/_ref/
@@ -210,6 +212,26 @@ def spread256():
     np.savez_compressed(os.path.join(HERE, "ref_spread_256cube.npz"), **d)
 
 
+def big_fixed():
+    """BiCGStab at 256^3 in fixed-iteration mode under a second summation
+    order (-p 1x1x8 beside the -p 2x2x2 run of big_extra), for 100 iterations,
+    and for 20 iterations under both: how far the reference's own trajectory
+    moves with the summation order over a fixed number of iterations."""
+    path = os.path.join(HERE, "ref_256cube_extra.npz")
+    with np.load(path, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    idx = d["sample_idx"]
+    for tag, procs, m in [("fixed100_p118", (1, 1, 8), 100), ("fixed20", (2, 2, 2), 20),
+                          ("fixed20_p118", (1, 1, 8), 20)]:
+        r = run_ref((256, 256, 256), procs, 16, 1e-9, m, 1, 7, ignore=True, nranks=8)
+        D = r["D"]
+        d.update({f"s1_{tag}_iters": r["iters"], f"s1_{tag}_codes": r["codes"], f"s1_{tag}_errs": r["errs"],
+                  f"s1_{tag}_Dsum": np.array(D.sum()), f"s1_{tag}_Dsq": np.array((D * D).sum()),
+                  f"s1_{tag}_Dsample": D[idx]})
+        print(tag, r["iters"].tolist(), flush=True)
+    np.savez_compressed(path, **d)
+
+
 def big_extra():
     d = {}
     n = 256 ** 3
@@ -282,5 +304,7 @@ if __name__ == "__main__":
         breakdown()
     elif "--spread256" in sys.argv:
         spread256()
+    elif "--big-fixed" in sys.argv:
+        big_fixed()
     else:
         small()

@@ -312,3 +312,25 @@ def test_exact_diag_fixture_256cube():
         assert np.allclose(again, d[pick], rtol=0, atol=1e-15)
     n = 256 ** 3
     assert np.array_equal(ex["idx_strided"], np.arange(0, n, n // 4096)[:4096])
+
+
+def test_reference_full_size_order_stability():
+    """At 256^3 (seed 7) the reference stops CG and pipelined CG at the same
+    iteration on one rank (-p 1x1x1) as on eight (-p 2x2x2), while BiCGStab
+    moves by tens of iterations between -p 1x1x1, -p 2x2x2 and an 8-lane
+    SIMD build (tests/golden/ref_256cube_extra.npz beside ref_256cube.npz)."""
+    from conftest import load_npz
+    big, extra = load_npz("ref_256cube.npz"), load_npz("ref_256cube_extra.npz")
+    for kind in (2, 3):
+        assert np.array_equal(extra[f"s{kind}_p111_iters"], big[f"s{kind}_iters"][:4])
+    dev = np.concatenate([np.abs(extra["s1_p111_iters"] - big["s1_iters"][:4]),
+                          np.abs(extra["s1_p111_avx512_iters"] - extra["s1_p111_iters"])])
+    assert dev.max() >= 20
+    assert np.all(extra["s1_fixed100_iters"] == 100) and np.all(extra["s1_fixed100_codes"] == 1)
+    # in fixed-iteration mode the reference's BiCGStab trajectory depends on
+    # the summation order only after many iterations: 20 agree to rounding,
+    # 100 do not
+    e20 = np.abs(extra["s1_fixed20_p118_errs"] / extra["s1_fixed20_errs"] - 1).max()
+    e100 = np.abs(extra["s1_fixed100_p118_errs"] / extra["s1_fixed100_errs"] - 1).max()
+    assert e20 < 1e-10 and e100 > 0.1
+    assert normwise(extra["s1_fixed20_p118_Dsample"], extra["s1_fixed20_Dsample"]) < 1e-11
