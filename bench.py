@@ -100,6 +100,13 @@ def parse():
     p.add_argument("--batch", type=int, default=64, help="right-hand sides per block (headline)")
     p.add_argument("--solver", default="cg", choices=sorted(NAMES), help="--mode plain only")
     p.add_argument("--c512-only", choices=["pcg", "bicgstab"], help="--mode c512: one solver only")
+    p.add_argument("--c512-compaction", action="store_true",
+                   help="--mode c512: run each mixed estimator with lane compaction off and on (no FP64 run) "
+                        "and report both rates and whether D is bitwise the same")
+    p.add_argument("--c512-mixed-only", action="store_true", help="--mode c512: the mixed estimator only")
+    p.add_argument("--no-compact", action="store_true", help="turn lane compaction of the batched solvers off")
+    p.add_argument("--no-graphs", action="store_true",
+                   help="launch every kernel directly instead of replaying captured HIP graphs")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-compare", action="store_true",
                    help="headline: skip the plain FP64 comparison run")
@@ -569,9 +576,32 @@ def run_c512(args):
     for kind, name in ((3, "PipelinedCG"), (1, "BiCGStab")):
         if args.c512_only and NAMES[args.c512_only] != kind:
             continue
+        if args.c512_compaction:
+            ab = {}
+            for on in (False, True):
+                sedi_amd.batch_compact_enable(on)
+                try:
+                    ab[on] = estimator_run(grid, progress=True, maxiter=4000, kind=kind, samples=32, tol=1e-8,
+                                           seed=23, precision=sedi_amd.FP32_MIXED, batch_k=16, rep_every=50)
+                finally:
+                    sedi_amd.batch_compact_enable(not args.no_compact)
+                print(json.dumps(dict(ab[on]["summary"], solver=name, precision="mixed", compaction=on)),
+                      flush=True)
+            out["runs"].append({"solver": name, "mixed_no_compaction": ab[False]["summary"],
+                                "mixed_compaction": ab[True]["summary"],
+                                "compaction_speedup": round(ab[True]["summary"]["probe_solves_per_s"] /
+                                                            ab[False]["summary"]["probe_solves_per_s"], 4),
+                                "D_bitwise_equal": bool((ab[True]["D"] == ab[False]["D"]).all()),
+                                "iters_equal": bool((ab[True]["iters"] == ab[False]["iters"]).all())})
+            del ab
+            continue
         m = estimator_run(grid, progress=True, maxiter=4000, kind=kind, samples=32, tol=1e-8, seed=23,
                           precision=sedi_amd.FP32_MIXED, batch_k=16, rep_every=50)
-        print(json.dumps(dict(m["summary"], solver=name, precision="mixed")), flush=True)
+        print(json.dumps(dict(m["summary"], solver=name, precision="mixed",
+                              iters=[int(v) for v in m["iters"]])), flush=True)
+        if args.c512_mixed_only:
+            out["runs"].append({"solver": name, "mixed": m["summary"]})
+            continue
         p = estimator_run(grid, progress=True, maxiter=4000, kind=kind, samples=32, tol=1e-8, seed=23)
         print(json.dumps(dict(p["summary"], solver=name, precision="fp64 plain")), flush=True)
         out["runs"].append({"solver": name, "mixed": m["summary"], "fp64_plain": p["summary"],
@@ -670,6 +700,12 @@ def main():
     if args.copy:
         print(json.dumps(copy_only(side=args.stencil_side or None, cold=args.cold)), flush=True)
         return
+    if args.no_compact:
+        import sedi_amd
+        sedi_amd.batch_compact_enable(False)
+    if args.no_graphs:
+        import sedi_amd
+        sedi_amd.graphs_enable(False)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)

@@ -239,6 +239,16 @@ int sedi_batch_solve(sedi_batch *batch, const double *b, int nrhs, double *x, in
  * error code. */
 int sedi_batch_err_history(sedi_batch *batch, int k, double *out, int cap);
 void sedi_batch_destroy(sedi_batch *batch);
+/* Lane compaction: a block of 4 right-hand sides runs until its slowest one
+ * converges, so between iterations the still-running right-hand sides of the
+ * emptiest block trade lanes with finished ones of other blocks (all at the
+ * same iteration) and the emptied block stops; every right-hand side's
+ * operations, and so its results, stay the same.  On by default for batch
+ * solves and the batched estimator (the server's blocks start at different
+ * times and keep their lanes).  sedi_batch_lane_moves gives the lanes moved
+ * in the batch's last solve.  No reference counterpart. */
+int sedi_batch_compact_enable(int on);
+int sedi_batch_lane_moves(sedi_batch *batch, int64_t *moves);
 
 /* Probes k0 .. k0+count-1 of the seeded stream (the estimator's probes), as
  * [count][ncells] doubles (host or device), each generated from its own

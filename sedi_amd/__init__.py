@@ -27,7 +27,7 @@ __all__ = ["SOLUTION_FOUND", "REACHED_MAXITER", "BREAKDOWN_RHO", "BREAKDOWN_OMEG
            "BICGSTAB", "CG", "PCG", "FP64", "FP32_MIXED", "SediError", "lib", "lib_path",
            "Stencil", "Cg", "PipelinedCg", "BiCgStab", "Solver", "rademacher_fill",
            "Estimator", "estimate", "Batch", "probe_batch", "Server", "h5_available", "h5_read", "h5_write",
-           "HOOK_X", "HOOK_V", "HOOK_D", "DeviceArray", "graphs_enable", "profile_enable", "profile_get",
+           "HOOK_X", "HOOK_V", "HOOK_D", "DeviceArray", "graphs_enable", "batch_compact_enable", "profile_enable", "profile_get",
            "profile_reset"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -116,6 +116,8 @@ def lib():
         L.sedi_batch_err_history.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int]
         L.sedi_batch_destroy.argtypes = [vp]
         L.sedi_batch_destroy.restype = None
+        L.sedi_batch_lane_moves.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.sedi_batch_compact_enable.argtypes = [ctypes.c_int]
         L.sedi_probe_batch.argtypes = [ctypes.c_uint32, ctypes.c_int64, ctypes.c_int, ctypes.c_int, vp]
         L.sedi_estimator_set_rhs.argtypes = [vp, vp]
         L.sedi_estimator_set_sample_hook.argtypes = [vp, SAMPLE_FN, ctypes.c_int, vp]
@@ -351,6 +353,13 @@ class Batch:
                                       codes.ctypes.data_as(ip)))
         return x, iters, errs, codes
 
+    @property
+    def lane_moves(self):
+        """Lanes that lane compaction moved in the last solve."""
+        m = ctypes.c_int64()
+        _check(lib().sedi_batch_lane_moves(self._h, ctypes.byref(m)))
+        return int(m.value)
+
     def err_history(self, k):
         n = _check(lib().sedi_batch_err_history(self._h, int(k), None, 0))
         out = np.zeros(n)
@@ -539,6 +548,11 @@ def h5_write(path, grid, **datasets):
 def graphs_enable(on=True):
     """Capture chunks of solver iterations into HIP graphs (default) or not."""
     _check(lib().sedi_graphs_enable(1 if on else 0))
+
+
+def batch_compact_enable(on=True):
+    """Lane compaction of batch solves (default) or not; results are the same."""
+    _check(lib().sedi_batch_compact_enable(1 if on else 0))
 
 
 def profile_enable(kernel_name):

@@ -280,6 +280,19 @@ int sedi_batch_err_history(sedi_batch* h, int k, double* out, int cap) {
 
 void sedi_batch_destroy(sedi_batch* h) { delete h; }
 
+int sedi_batch_lane_moves(sedi_batch* h, int64_t* moves) {
+    return guarded([&]() -> int {
+        SEDI_REQUIRE(h && moves, SEDI_ERR_ARG, "batch and moves must not be NULL");
+        *moves = h->impl.lane_moves();
+        return SEDI_OK;
+    });
+}
+
+int sedi_batch_compact_enable(int on) {
+    sedi::set_batch_compact_enabled(on != 0);
+    return SEDI_OK;
+}
+
 int sedi_probe_batch(uint32_t seed, int64_t ncells, int k0, int count, double* v) {
     return guarded([&]() -> int {
         SEDI_REQUIRE(ncells >= 1 && k0 >= 0 && count >= 1 && v, SEDI_ERR_ARG, "bad probe batch arguments");

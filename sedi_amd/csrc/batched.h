@@ -75,6 +75,8 @@ class BatchSolver {
     const int* snap_codes(const void* snap) const;
     const double* snap_errs(const void* snap) const;
     int nblocks() const { return nrb_; }
+    // Lanes moved by compaction in the last solve() (see solve_t).
+    long long lane_moves() const { return moves_last_; }
 
   private:
     template <class T> int bind_t(int hist_need);
@@ -84,6 +86,9 @@ class BatchSolver {
     template <class T> int accumulate_t(double* t, double* q, int k0, int k1);
     template <class T> int pack_t(const double* b, int kact, long long row_stride);
     template <class T> int unpack_t(double* x, int k0, int k1, bool rhs);
+    template <class T> int compact_t();
+    int store_rows(double* x, int k0, int k1, bool rhs);
+    unsigned read_moves();
     Shape shape_{};
     double a_ = 6.0, b_c_ = 1.0, c_ = 1.0;
     int kind_ = 0, K_ = 0, nrb_ = 0, nvec_ = 0;
@@ -98,9 +103,17 @@ class BatchSolver {
     std::vector<double> hist_host_;
     BRun* run_ = nullptr;
     // the captured chunk of iterations (count, executable graph)
-    int graph_count_ = 0;
+    int graph_count_ = 0, graph_compact_ = 0;
     hipGraphExec_t graph_ = nullptr;
     bool geometry_ready_ = false;
+    // lane_of_[k]: the lane that holds right-hand side k (compaction moves
+    // right-hand sides between lanes during a solve; identity otherwise)
+    std::vector<int> lane_of_;
+    long long moves_last_ = 0;
 };
+
+// Lane compaction of batch solves (on by default); no effect on results.
+bool batch_compact_enabled();
+void set_batch_compact_enabled(bool on);
 
 }  // namespace sedi

@@ -50,10 +50,24 @@
 // for a given grid.
 #include "batched_common.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
 namespace sedi {
+
+static std::atomic<bool> g_compact{true};
+bool batch_compact_enabled() { return g_compact.load(); }
+void set_batch_compact_enabled(bool on) { g_compact.store(on); }
+
+// control block layout: per-RHS doubles [CD_NDOUBLE][K], then per-RHS ints
+// [CI_NINT][K] (c.bb and c.fiter are the two bases)
+enum {
+    CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
+    CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW, CD_TOL,
+    CD_APEND, CD_NDOUBLE
+};
+enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_MAXIT, CI_PLANE, CI_NINT };
 
 // ---- set-up: r = b (x0 = 0), x = 0, <b,b>, <r, r.*iM> -----------------------
 // Cg.hpp:111-147, PipelinedCg.hpp:113-136 and BiCgStab.hpp:117-163 for x = 0:
@@ -110,6 +124,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         c.fold[k] = 0;
         c.sexit[k] = 0;
         c.done[k] = 0;
+        c.plane[k] = k;
         if (c.kind == SEDI_BICGSTAB) {
             c.rho[k] = acc[q];  // <r, r^> with r^ = r (:160)
             c.rho_prev[k] = 0.0;
@@ -492,16 +507,187 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     fp[0] = fp[1] = fp[2] = fp[3] = 0;
 }
 
-// ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
-// x = x_hi + x_lo (+ CG's pending apend * p: pend non-null)
+// ---- lane compaction ---------------------------------------------------------
+// A block runs while any of its 4 lanes does, so a right-hand side that
+// finished early still costs its share of every pass until the slowest lane
+// of its block converges.  Between iterations batch_compact_kernel plans
+// moves among the running blocks that share an iteration count (in a solve,
+// all of them): while the live lanes of the emptiest block fit into finished
+// lanes of the others, each trades places with a finished lane there, and the
+// emptied block retires.  batch_swap_kernel then exchanges the two lanes'
+// values in every vector, the FP64 rows of x and the error histories; the
+// planning kernel has already exchanged their per-RHS scalars.  A lane's
+// values go through the same operations in any lane and block (each lane's
+// sums have a fixed order per grid), and the two blocks are at the same
+// iteration, so a moved right-hand side continues exactly as if it had
+// stayed; plane[] records which right-hand side each lane holds.
+constexpr int COMPACT_MAX_RB = 256;
+
+__global__ __launch_bounds__(256) void batch_compact_kernel(BCtl c, int nrb) {
+    __shared__ int live[COMPACT_MAX_RB];                // live lanes per block (0: not running)
+    __shared__ int itr[COMPACT_MAX_RB];                 // its iteration
+    __shared__ unsigned char recv[COMPACT_MAX_RB];      // received a lane in this plan
+    __shared__ unsigned char occ[4 * COMPACT_MAX_RB];   // lane holds a live RHS after the plan
+    __shared__ int pairs[2 * 4 * COMPACT_MAX_RB];
+    __shared__ int npairs;
+    const int t = threadIdx.x;
+    const int K = c.K;
+    for (int rb = t; rb < nrb; rb += blockDim.x) {
+        int l = 0;
+        for (int q = 0; q < 4; q++) {
+            const int on = !c.done[4 * rb + q];
+            occ[4 * rb + q] = (unsigned char)on;
+            l += on;
+        }
+        live[rb] = c.rb_done[rb] ? 0 : l;
+        itr[rb] = c.iter[rb];
+        recv[rb] = 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int np = 0;
+        for (;;) {
+            // donor: the running block with the fewest live lanes that has
+            // received none (if its lanes do not fit elsewhere, no block's do)
+            int d = -1;
+            for (int rb = 0; rb < nrb; rb++)
+                if (live[rb] > 0 && !recv[rb] && (d < 0 || live[rb] < live[d])) d = rb;
+            if (d < 0) break;
+            int room = 0;
+            for (int rb = 0; rb < nrb; rb++)
+                if (rb != d && live[rb] > 0 && itr[rb] == itr[d]) room += 4 - live[rb];
+            if (room < live[d]) break;
+            for (int qd = 0; qd < 4; qd++) {
+                const int kd = 4 * d + qd;
+                if (!occ[kd]) continue;
+                int r = -1;  // receiver: the fullest block with room
+                for (int rb = 0; rb < nrb; rb++)
+                    if (rb != d && live[rb] > 0 && live[rb] < 4 && itr[rb] == itr[d] && (r < 0 || live[rb] > live[r]))
+                        r = rb;
+                int ks = 4 * r;
+                while (occ[ks]) ks++;
+                pairs[2 * np] = kd;
+                pairs[2 * np + 1] = ks;
+                np++;
+                occ[ks] = 1;
+                occ[kd] = 0;
+                live[r]++;
+                recv[r] = 1;
+            }
+            live[d] = -1;  // retires
+        }
+        npairs = np;
+        c.swaps[0] = np;
+        for (int j = 0; j < 2 * np; j++) c.swaps[1 + j] = pairs[j];
+    }
+    __syncthreads();
+    const int np = npairs;
+    if (np == 0) return;
+    double* dbase = c.bb;  // CD_BB = 0
+    int* ibase = c.fiter;  // CI_FITER = 0
+    for (int p = t; p < np; p += blockDim.x) {
+        const int a = pairs[2 * p], b = pairs[2 * p + 1];
+        for (int f = 0; f < CD_NDOUBLE; f++) {
+            const double v = dbase[f * K + a];
+            dbase[f * K + a] = dbase[f * K + b];
+            dbase[f * K + b] = v;
+        }
+        for (int f = 0; f < CI_NINT; f++) {
+            const int v = ibase[f * K + a];
+            ibase[f * K + a] = ibase[f * K + b];
+            ibase[f * K + b] = v;
+        }
+    }
+    if (t == 0) {
+        for (int rb = 0; rb < nrb; rb++)
+            if (live[rb] < 0) c.rb_done[rb] = 1;  // every lane of it now holds a finished RHS
+        *c.moves += (unsigned)np;
+    }
+}
+
+struct VSlots {
+    void* p[12];
+};
+
+// Grid: x over cells, y over the vector slots to exchange plus one for the
+// FP64 rows of x and the histories.  The two lanes of a pair are distinct
+// from every other pair's (each lane moves at most once per plan), so all
+// loads of a group of pairs can be in flight before their stores.
 template <class T>
-__global__ void batch_accum_kernel(long long n, int k0, int k1, const V4<T>* xlo, const double* xhi,
+__global__ __launch_bounds__(NTHREADS) void batch_swap_kernel(BCtl c, long long n, VSlots v, int nslot,
+                                                              double* xhi) {
+    __shared__ int pairs[2 * 4 * COMPACT_MAX_RB];
+    const int np = c.swaps[0];
+    if (np == 0) return;
+    for (int j = threadIdx.x; j < 2 * np; j += blockDim.x) pairs[j] = c.swaps[1 + j];
+    __syncthreads();
+    const long long g0 = (long long)blockIdx.x * NTHREADS + threadIdx.x;
+    const long long stride = (long long)gridDim.x * NTHREADS;
+    constexpr int G = 4;
+    if ((int)blockIdx.y < nslot) {
+        T* base = static_cast<T*>(v.p[blockIdx.y]);
+        for (long long i = g0; i < n; i += stride) {
+            for (int p0 = 0; p0 < np; p0 += G) {
+                long long ia[G], ib[G];
+                T va[G], vb[G];
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    if (p0 + j < np) {
+                        const int a = pairs[2 * (p0 + j)], b = pairs[2 * (p0 + j) + 1];
+                        ia[j] = ((long long)(a >> 2) * n + i) * 4 + (a & 3);
+                        ib[j] = ((long long)(b >> 2) * n + i) * 4 + (b & 3);
+                        va[j] = base[ia[j]];
+                        vb[j] = base[ib[j]];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    if (p0 + j < np) {
+                        base[ia[j]] = vb[j];
+                        base[ib[j]] = va[j];
+                    }
+                }
+            }
+        }
+        return;
+    }
+    if (xhi) {
+        for (long long i = g0; i < n; i += stride) {
+            for (int p = 0; p < np; p++) {
+                const long long a = pairs[2 * p], b = pairs[2 * p + 1];
+                const double xa = xhi[a * n + i];
+                xhi[a * n + i] = xhi[b * n + i];
+                xhi[b * n + i] = xa;
+            }
+        }
+    }
+    for (long long j = g0; j < c.hstride; j += stride) {
+        for (int p = 0; p < np; p++) {
+            const long long a = pairs[2 * p], b = pairs[2 * p + 1];
+            const double h = c.hist[a * c.hstride + j];
+            c.hist[a * c.hstride + j] = c.hist[b * c.hstride + j];
+            c.hist[b * c.hstride + j] = h;
+        }
+    }
+}
+
+// ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
+// x = x_hi + x_lo (+ CG's pending apend * p: pend non-null); the probes are
+// the lanes of `lanes`, in its order
+struct LaneList {
+    int n;
+    int lane[256];
+};
+
+template <class T>
+__global__ void batch_accum_kernel(long long n, LaneList lanes, const V4<T>* xlo, const double* xhi,
                                    const V4<T>* b, const V4<T>* pend, const double* apend, double* t,
                                    double* q) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         double tv = t[i], qv = q[i];
-        for (int k = k0; k < k1; k++) {
+        for (int j = 0; j < lanes.n; j++) {
+            const int k = lanes.lane[j];
             const long long cell = (long long)(k >> 2) * n + i;
             double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
             if (pend && apend[k] != 0.0) x = x + apend[k] * (double)pend[cell][k & 3];
@@ -653,13 +839,6 @@ static int cg_iteration(const BRun& r) {
     return SEDI_OK;
 }
 
-// control block layout (doubles, then ints)
-enum {
-    CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
-    CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW, CD_TOL,
-    CD_APEND, CD_NDOUBLE
-};
-enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_MAXIT, CI_NINT };
 
 static int nvec_of(int kind) {
     return kind == SEDI_CG ? CG_NVEC : (kind == SEDI_PCG ? PCG_NVEC : BI_NVEC);
@@ -706,12 +885,15 @@ int BatchSolver::create(const Shape& s, double a, double b, double c, int kind, 
     xhi_ = fp32 ? reinterpret_cast<double*>(base + nvec_ * vec) : nullptr;
     SEDI_HIP(hipMemset(vec_.p, 0, vec * nvec_ + hib));
     // control: doubles [CD_NDOUBLE][K], ints [CI_NINT][K], iter[nrb], rb_done[nrb],
-    // counters[nrb], partials[nrb][12][8*CUs]
+    // counters[nrb], swaps[1 + 2K], moves, partials[nrb][12][8*CUs]
     const size_t pst = 12 * (size_t)(8 * batch_cus());
     const size_t bytes = sizeof(double) * CD_NDOUBLE * K + sizeof(int) * (CI_NINT * K + 2 * nrb_) +
-                         sizeof(unsigned) * nrb_ + 256 + sizeof(double) * pst * nrb_;
+                         sizeof(unsigned) * nrb_ + sizeof(int) * (1 + 2 * (size_t)K) + sizeof(unsigned) + 256 +
+                         sizeof(double) * pst * nrb_;
     SEDI_HIP(ctl_.reserve(bytes));
     SEDI_HIP(hipMemset(ctl_.p, 0, bytes));
+    lane_of_.resize(K);
+    for (int k = 0; k < K; k++) lane_of_[k] = k;
     return SEDI_OK;
 }
 
@@ -745,10 +927,13 @@ int BatchSolver::bind_t(int hist_need) {
     int* ip = reinterpret_cast<int*>(d + CD_NDOUBLE * K);
     c.fiter = ip + CI_FITER * K; c.code = ip + CI_CODE * K; c.done = ip + CI_DONE * K; c.rep = ip + CI_REP * K;
     c.fold = ip + CI_FOLD * K; c.sexit = ip + CI_SEXIT * K; c.maxv = ip + CI_MAXIT * K;
+    c.plane = ip + CI_PLANE * K;
     c.iter = ip + CI_NINT * K;
     c.rb_done = c.iter + nrb_;
     c.counters = reinterpret_cast<unsigned*>(c.rb_done + nrb_);
-    char* pp = reinterpret_cast<char*>(c.counters + nrb_);
+    c.swaps = reinterpret_cast<int*>(c.counters + nrb_);
+    c.moves = reinterpret_cast<unsigned*>(c.swaps + 1 + 2 * K);
+    char* pp = reinterpret_cast<char*>(c.moves + 1);
     pp += (256 - (reinterpret_cast<uintptr_t>(pp) & 255)) & 255;
     c.partials = reinterpret_cast<double*>(pp);
     c.pstride = 12LL * 8 * batch_cus();
@@ -786,6 +971,7 @@ int BatchSolver::bind_t(int hist_need) {
     for (int k = 0; k < 12; k++) run.v[k] = v_[k];
     run.xhi = xhi_;
     run.mixed = fp32_;
+    run.nvec = nvec_;
     return SEDI_OK;
 }
 
@@ -817,6 +1003,7 @@ int BatchSolver::iterate_t(int count) {
     auto direct = [&]() -> int {
         int rc = SEDI_OK;
         for (int k = 0; k < count && rc == SEDI_OK; k++) {
+            if (run.compact > 0 && k % run.compact == 0 && (rc = compact_t<T>())) break;
             if (kind_ == SEDI_CG) rc = cg_iteration<T>(run);
             else if (kind_ == SEDI_PCG) rc = pcg_iteration<T>(run);
             else rc = bicg_iteration<T>(run);
@@ -828,7 +1015,7 @@ int BatchSolver::iterate_t(int count) {
         geometry_ready_ = true;
         return rc;
     }
-    if (!graph_ || graph_count_ != count) {
+    if (!graph_ || graph_count_ != count || graph_compact_ != run.compact) {
         if (graph_) {
             (void)hipGraphExecDestroy(graph_);
             graph_ = nullptr;
@@ -846,9 +1033,41 @@ int BatchSolver::iterate_t(int count) {
         (void)hipGraphDestroy(g);
         SEDI_HIP(ei);
         graph_count_ = count;
+        graph_compact_ = run.compact;
     }
     SEDI_HIP(hipGraphLaunch(graph_, stream_));
     return SEDI_OK;
+}
+
+// The planning kernel and the lane exchange (which returns at once when
+// nothing moves).
+template <class T>
+int BatchSolver::compact_t() {
+    // the slots whose lanes carry state: all but the FP32 part of x in mixed
+    // BiCGStab, whose x is wholly in the FP64 rows (x_lo stays zero)
+    VSlots vs;
+    int ns = 0;
+    for (int k = 0; k < nvec_; k++)
+        if (!(k == SLOT_XLO && fp32_ && kind_ == SEDI_BICGSTAB)) vs.p[ns++] = v_[k];
+    SEDI_HIP(launch("batch_compact_kernel", batch_compact_kernel, dim3(1), dim3(256), stream_, run_->c, nrb_));
+    SEDI_HIP(launch("batch_swap_kernel", batch_swap_kernel<T>, dim3(ew_blocks(shape_.n()), ns + 1), dim3(NTHREADS),
+                    stream_, run_->c, shape_.n(), vs, ns, xhi_));
+    return SEDI_OK;
+}
+
+unsigned BatchSolver::read_moves() {
+    unsigned m = 0;
+    if (run_ && run_->c.moves && hipMemcpy(&m, run_->c.moves, sizeof(m), hipMemcpyDeviceToHost) != hipSuccess)
+        m = 0;
+    return m;
+}
+
+// Compaction period: every iteration on grids of 2^20 cells and more, where
+// an iteration of a block takes far longer than the two small launches;
+// once per chunk of 16 on smaller grids.
+static int compact_period(const Shape& s, int nrb) {
+    if (!batch_compact_enabled() || nrb < 2 || nrb > COMPACT_MAX_RB) return 0;
+    return s.n() >= (1LL << 20) ? 1 : 16;
 }
 
 template <class T>
@@ -856,7 +1075,12 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
     SEDI_REQUIRE(maxiter >= 0, SEDI_ERR_ARG, "maxiter must be >= 0");
     const int K = K_;
     int rc = bind_t<T>(maxiter);
-    if (rc || (rc = start_t<T>(0, nrb_, tol, maxiter))) return rc;
+    if (rc) return rc;
+    SEDI_HIP(hipStreamSynchronize(stream_));
+    const unsigned moves0 = read_moves();
+    run_->compact = compact_period(shape_, nrb_);
+    for (int k = 0; k < K; k++) lane_of_[k] = k;
+    if ((rc = start_t<T>(0, nrb_, tol, maxiter))) return rc;
     const BCtl& c = run_->c;
     const int chunk = 16;
     int launched = 0, slot = 0;
@@ -898,17 +1122,31 @@ int BatchSolver::solve_t(int maxiter, double tol, BatchResult* res) {
         set_error("batched solve failed: %s", hipGetErrorString(se));
         return SEDI_ERR_HIP;
     }
+    // which lane holds which right-hand side after the moves
+    std::vector<int> plane(K);
+    SEDI_HIP(hipMemcpy(plane.data(), c.plane, sizeof(int) * K, hipMemcpyDeviceToHost));
+    for (int l = 0; l < K; l++) {
+        SEDI_REQUIRE(plane[l] >= 0 && plane[l] < K, SEDI_ERR_STATE, "lane %d holds no right-hand side", l);
+        lane_of_[plane[l]] = l;
+    }
+    moves_last_ = (long long)(read_moves() - moves0);
     if (res) {
+        std::vector<int> it(K), code(K), done(K);
+        std::vector<double> err(K);
+        SEDI_HIP(hipMemcpy(it.data(), c.fiter, sizeof(int) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(code.data(), c.code, sizeof(int) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(err.data(), c.ferr, sizeof(double) * K, hipMemcpyDeviceToHost));
+        SEDI_HIP(hipMemcpy(done.data(), c.done, sizeof(int) * K, hipMemcpyDeviceToHost));
         res->iters.assign(K, 0);
         res->codes.assign(K, 0);
         res->errs.assign(K, 0.0);
-        std::vector<int> done(K);
-        SEDI_HIP(hipMemcpy(res->iters.data(), c.fiter, sizeof(int) * K, hipMemcpyDeviceToHost));
-        SEDI_HIP(hipMemcpy(res->codes.data(), c.code, sizeof(int) * K, hipMemcpyDeviceToHost));
-        SEDI_HIP(hipMemcpy(res->errs.data(), c.ferr, sizeof(double) * K, hipMemcpyDeviceToHost));
-        SEDI_HIP(hipMemcpy(done.data(), c.done, sizeof(int) * K, hipMemcpyDeviceToHost));
-        for (int k = 0; k < K; k++)
-            SEDI_REQUIRE(done[k], SEDI_ERR_STATE, "batched solve stopped without a result for RHS %d", k);
+        for (int k = 0; k < K; k++) {
+            const int l = lane_of_[k];
+            SEDI_REQUIRE(done[l], SEDI_ERR_STATE, "batched solve stopped without a result for RHS %d", k);
+            res->iters[k] = it[l];
+            res->codes[k] = code[l];
+            res->errs[k] = err[l];
+        }
     }
     hist_host_.resize((size_t)K * hist_cap_);
     SEDI_HIP(hipMemcpy(hist_host_.data(), hist_.p, sizeof(double) * hist_host_.size(), hipMemcpyDeviceToHost));
@@ -924,6 +1162,9 @@ __global__ void batch_free_all_kernel(BCtl c, int nrb) {
 int BatchSolver::slots_open(int hist_cap) {
     int rc = fp32_ ? bind_t<float>(hist_cap) : bind_t<double>(hist_cap);
     if (rc) return rc;
+    // blocks start at different times here: each keeps its lanes
+    run_->compact = 0;
+    for (int k = 0; k < K_; k++) lane_of_[k] = k;
     SEDI_HIP(launch("batch_free_all_kernel", batch_free_all_kernel, dim3(1), dim3(256), stream_, run_->c, nrb_));
     SEDI_HIP(hipStreamSynchronize(stream_));
     return SEDI_OK;
@@ -981,9 +1222,14 @@ int BatchSolver::accumulate_t(double* t, double* q, int k0, int k1) {
     const long long n = shape_.n();
     long long g = (n + 255) / 256;
     if (g > 8192) g = 8192;
-    SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n,
-                    k0, k1, (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B],
-                    pending_dir<T>(*run_), (const double*)run_->c.apend, t, q));
+    for (int k = k0; k < k1; k += 256) {
+        LaneList L;
+        L.n = std::min(256, k1 - k);
+        for (int j = 0; j < L.n; j++) L.lane[j] = lane_of_[k + j];
+        SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n, L,
+                        (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B],
+                        pending_dir<T>(*run_), (const double*)run_->c.apend, t, q));
+    }
     return SEDI_OK;
 }
 
@@ -1022,21 +1268,37 @@ int BatchSolver::load_rhs(const double* b, int kact, bool replicate) {
     return fp32_ ? pack_t<float>(b, kact, stride) : pack_t<double>(b, kact, stride);
 }
 
+// Rows k0 .. k1-1 in right-hand-side order: one unpacking when they sit in
+// their own lanes, else one per row.
+int BatchSolver::store_rows(double* x, int k0, int k1, bool rhs) {
+    bool identity = true;
+    for (int k = k0; k < k1; k++) identity = identity && lane_of_[k] == k;
+    if (identity) return fp32_ ? unpack_t<float>(x, k0, k1, rhs) : unpack_t<double>(x, k0, k1, rhs);
+    for (int k = k0; k < k1; k++) {
+        const int l = lane_of_[k];
+        double* row = x + (long long)(k - k0) * shape_.n();
+        const int rc = fp32_ ? unpack_t<float>(row, l, l + 1, rhs) : unpack_t<double>(row, l, l + 1, rhs);
+        if (rc) return rc;
+    }
+    return SEDI_OK;
+}
+
 int BatchSolver::store_x(double* x, int k0, int k1) {
     SEDI_REQUIRE(0 <= k0 && k0 < k1 && k1 <= K_, SEDI_ERR_ARG, "solutions %d..%d of a batch of %d", k0, k1, K_);
-    return fp32_ ? unpack_t<float>(x, k0, k1, false) : unpack_t<double>(x, k0, k1, false);
+    return store_rows(x, k0, k1, false);
 }
 
 int BatchSolver::store_rhs(double* v, int k0, int k1) {
     SEDI_REQUIRE(0 <= k0 && k0 < k1 && k1 <= K_, SEDI_ERR_ARG, "right-hand sides %d..%d of a batch of %d", k0,
                  k1, K_);
-    return fp32_ ? unpack_t<float>(v, k0, k1, true) : unpack_t<double>(v, k0, k1, true);
+    return store_rows(v, k0, k1, true);
 }
 
 std::vector<double> BatchSolver::history(int k) const {
     std::vector<double> h;
     if (k < 0 || k >= K_ || hist_host_.empty()) return h;
-    h.assign(hist_host_.begin() + (size_t)k * hist_cap_, hist_host_.begin() + (size_t)(k + 1) * hist_cap_);
+    const size_t l = (size_t)lane_of_[k];
+    h.assign(hist_host_.begin() + l * hist_cap_, hist_host_.begin() + (l + 1) * hist_cap_);
     return h;
 }
 

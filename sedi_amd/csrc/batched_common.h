@@ -127,12 +127,15 @@ struct BCtl {
     double *rho, *rho_prev, *omega, *ss, *cbw;               // BiCGStab
     double* apend;  // CG: coefficient of the pending p (the last odd iteration's) not yet in x
     int *fiter, *code, *done, *rep, *fold, *sexit;
+    int* plane;  // per lane: the right-hand side (index at load) it holds
     // per RHS block
     int *iter, *rb_done;
     double* tolv;  // per RHS: tolerance
     int* maxv;     // per RHS: iteration cap
     double* partials;
     unsigned* counters;
+    int* swaps;       // lane compaction: [count, (lane a, lane b) x count]
+    unsigned* moves;  // lane compaction: lanes moved since creation
     double drop;     // replacement when the residual dropped by this factor
     double restart;  // restart when true > restart * recursive residual
     int rep_every, mixed, K, kind;
@@ -261,6 +264,8 @@ struct BRun {
     void* v[12];
     double* xhi;  // FP64 part of x ([K][N]; mixed form only)
     bool mixed;
+    int nvec;     // vector slots in use (v[0 .. nvec-1])
+    int compact;  // lane compaction before every compact-th iteration (0: off)
 };
 
 // Per-solver kernel sets (batched_pcg.hip, batched_bicg.hip); each launches
