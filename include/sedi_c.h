@@ -345,8 +345,8 @@ int sedi_graphs_enable(int on);
  * grid, or `fallback`.  Keys "zc:<kernel>" (planes per tile of a solver
  * kernel), "sched:<kernel>" (1 = tiles in launch order, 0 = XCD-contiguous
  * runs), "k:<solver>:<precision>" (probes per batch; nearest grid by cell
- * count) and "ty:batch_upd_kernel" (rows per tile, 8 or 16, of the FP32
- * batched CG update pass).  No reference counterpart (its tiling is Platform.hpp's
+ * count), "ty:batch_upd_kernel" and "ty:batch_dir_kernel" (rows per tile,
+ * 8 or 16, of the FP32 batched CG update and direction passes).  No reference counterpart (its tiling is Platform.hpp's
  * compile-time constants). */
 int sedi_tuning_lookup(const char *key, const sedi_grid *grid, int fallback);
 

@@ -176,13 +176,12 @@ struct PassBDir {
     }
 };
 
-template <class T, int ZC>
-__global__ __launch_bounds__(TX * CgDirTile<T>::ty, CgDirTile<T>::min_waves) void batch_dir_kernel(
+template <class T, int ZC, int TYK>
+__global__ __launch_bounds__(TX * TYK, CgDirTile<T>::min_waves) void batch_dir_kernel(
     TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r0, const V4<T>* r1, V4<T>* p0, V4<T>* p1,
     V4<T>* x, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
-    constexpr int TYK = CgDirTile<T>::ty;
     __shared__ V4<T> lds[2][TYK + 2][TX + 2];
     __shared__ double red[4 * TYK];
     __shared__ int lastflag;
@@ -805,6 +804,23 @@ static hipError_t launch_upd(const BRun& r, T a, T b, T cc, T im, const V4<T>* P
                   im);
 }
 
+// The direction pass with TYK-row tiles (8: the default in FP32 storage; 16
+// halves the y-halo re-reads of r and p, chosen by the tuning table's
+// "ty:batch_dir_kernel"; FP64 blocks keep 4 rows).
+template <class T, int TYK>
+static hipError_t launch_dir(const BRun& r, T a, T b, T cc, T im, const V4<T>* R0, const V4<T>* R1, V4<T>* P0,
+                             V4<T>* P1, V4<T>* XL) {
+    const KGeom g = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16, TYK>, r.shape, SEDI_PREFETCH_BATCH, TYK);
+    const dim3 grid(g.nblocks, r.nrb), block(TX * TYK);
+    auto kern = g.tg.zc == 8 ? batch_dir_kernel<T, 8, TYK>
+                             : (g.tg.zc == 32 ? batch_dir_kernel<T, 32, TYK> : batch_dir_kernel<T, 16, TYK>);
+    return launch("batch_dir_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, R0, R1, P0, P1, XL, im);
+}
+
+static int dir_rows(const Shape& s) {
+    return tuned_value("ty:batch_dir_kernel", s, 0) == 2 * TY_TALL ? 2 * TY_TALL : TY_TALL;
+}
+
 // Rows per tile of the update pass in FP32 storage: 16 (one 16-wave
 // workgroup per CU) halve the y-halo re-reads of p against 8 and measured
 // 1-3.5% faster on 256^3 and 512^3 grids, but 6-10% slower on 64^3 and 128^3,
@@ -824,9 +840,14 @@ static int cg_iteration(const BRun& r) {
     V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
     V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
     V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
-    const KGeom g1 = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH, CgDirTile<T>::ty);
-    SEDI_HIP(SEDI_BZC_LAUNCH("batch_dir_kernel", batch_dir_kernel, T, g1, dim3(g1.nblocks, r.nrb), r.stream,
-                             g1.tg, r.c, a, b, cc, r.n, (const V4<T>*)R0, (const V4<T>*)R1, P0, P1, XL, im));
+    if constexpr (sizeof(T) == 4) {
+        if (dir_rows(r.shape) == 2 * TY_TALL)
+            SEDI_HIP((launch_dir<T, 2 * TY_TALL>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+        else
+            SEDI_HIP((launch_dir<T, TY_TALL>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+    } else {
+        SEDI_HIP((launch_dir<T, TY>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+    }
     if constexpr (sizeof(T) == 4) {
         if (upd_rows(r.shape) == 2 * TY_TALL)
             SEDI_HIP((launch_upd<T, 2 * TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));

@@ -15,7 +15,8 @@ namespace sedi {
 //                           XCD-contiguous runs of tiles;
 //       "k:<solver>:<precision>" -- probes per batch of the estimator;
 //       "ty:batch_upd_kernel" -- rows per tile of the FP32 batched CG update
-//                                pass (8 or 16; default 16 from 256 rows up).
+//                                pass (8 or 16; default 16 from 256 rows up);
+//       "ty:batch_dir_kernel" -- the same for its direction pass (default 8).
 // An entry matches the exact grid, or any grid when written for 0 0 0; the
 // keys "zc:*" and "sched:*" stand for every kernel.  With `nearest`, an entry of the same
 // key for the grid of nearest cell count also matches.
