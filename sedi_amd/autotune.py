@@ -24,12 +24,15 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
    solves, which real solves hide but short timing runs would not (at 256^3 a
    probe takes ~48 ms, 240 plain iterations).
 
-3. the standalone stencil's plan: chunk (8, 16, 32) x schedule (0, 1, and 2:
-   one workgroup per tile, which only a kernel without partial sums can use)
-   x planes of loads in flight (2, 4, 8), each forced in a child process and
-   timed cold -- launches rotate through (x, y) pairs spanning 1 GiB of device
-   memory, so x comes from HBM as in a sweep over vectors larger than the
-   Infinity Cache -- replacing step 1's stencil entries.
+3. the standalone stencil's plan: the one-cell march with chunk (8, 16, 32) x
+   schedule (0, 1, and 2: one workgroup per tile, which only a kernel without
+   partial sums can use) x planes of loads in flight (2, 4, 8), and the pair
+   march (two cells per lane, 16-byte accesses, one 16-wave workgroup per CU)
+   with chunk (8, 16, 32) x schedule (0, 3, 4) x planes in flight (2, 4), each
+   forced in a child process and timed cold -- launches rotate through (x, y)
+   pairs spanning 1 GiB of device memory, so x comes from HBM as in a sweep
+   over vectors larger than the Infinity Cache -- replacing step 1's stencil
+   entries.
 
 ``--k-only`` re-measures the batch sizes alone, with the chunk and schedule
 entries of the existing table (--out), which it keeps; ``--stencil-only``
@@ -135,7 +138,9 @@ def measure_chunks(sides, iters):
     return out
 
 
-STENCIL_PLANS = [(zc, sched, pf) for zc in (8, 16, 32) for sched in (0, 1, 2) for pf in (2, 4, 8)]
+# (pairs, chunk, schedule, planes in flight): the one-cell march, then the pair march
+STENCIL_PLANS = ([(0, zc, sched, pf) for zc in (8, 16, 32) for sched in (0, 1, 2) for pf in (2, 4, 8)] +
+                 [(1, zc, sched, pf) for zc in (8, 16, 32) for sched in (0, 3, 4) for pf in (2, 4)])
 # schedules of the solver kernels (march.h tuned_geometry): 0 XCD-contiguous
 # runs, 1 launch order, 3 and 4 launch order with tiles dealt to the XCDs in
 # chunks of 4 or 8 rows of tiles
@@ -175,21 +180,26 @@ def measure_stencil(sides, reps):
 
 
 def stencil_lines(args):
-    """Step 3: the stencil's best (chunk, schedule, prefetch) per grid."""
+    """Step 3: the stencil's best (march, chunk, schedule, prefetch) per grid."""
     t = {}
-    for zc, sched, pf in STENCIL_PLANS:
-        print("stencil: z chunk %d, schedule %d, prefetch %d" % (zc, sched, pf), file=sys.stderr, flush=True)
-        t[(zc, sched, pf)] = child(args, ["zc:stencil_apply_kernel 0 0 0 %d" % zc,
-                                          "sched:stencil_apply_kernel 0 0 0 %d" % sched,
-                                          "pf:stencil_apply_kernel 0 0 0 %d" % pf], "stencil")
+    for x2, zc, sched, pf in STENCIL_PLANS:
+        print("stencil: %s march, z chunk %d, schedule %d, prefetch %d" % ("pair" if x2 else "one-cell", zc, sched,
+                                                                         pf), file=sys.stderr, flush=True)
+        t[(x2, zc, sched, pf)] = child(args, ["x2:stencil_apply_kernel 0 0 0 %d" % x2,
+                                              "zc:stencil_apply_kernel 0 0 0 %d" % zc,
+                                              "sched:stencil_apply_kernel 0 0 0 %d" % sched,
+                                              "pf:stencil_apply_kernel 0 0 0 %d" % pf], "stencil")
     lines = []
     for side in args.sides:
         tt = {c: t[c][str(side)] for c in t}
-        zc, sched, pf = min(tt, key=tt.get)
+        x2, zc, sched, pf = min(tt, key=tt.get)
         best = sorted(tt, key=tt.get)[:4]
-        note = " ".join("%d/%d/%d:%.1f" % (c[0], c[1], c[2], tt[c]) for c in best)
-        lines.append("zc:stencil_apply_kernel %d %d %d %d   # cold us per launch (chunk/schedule/prefetch), "
-                     "best four: %s" % (side, side, side, zc, note))
+        note = " ".join("%s%d/%d/%d:%.1f" % ("p" if c[0] else "", c[1], c[2], c[3], tt[c]) for c in best)
+        one = min((c for c in tt if not c[0]), key=tt.get)
+        lines.append("x2:stencil_apply_kernel %d %d %d %d   # cold us per launch (p: pair march; chunk/schedule/"
+                     "prefetch), best four: %s; best one-cell plan %d/%d/%d:%.1f" % (
+                         side, side, side, x2, note, one[1], one[2], one[3], tt[one]))
+        lines.append("zc:stencil_apply_kernel %d %d %d %d" % (side, side, side, zc))
         lines.append("sched:stencil_apply_kernel %d %d %d %d" % (side, side, side, sched))
         lines.append("pf:stencil_apply_kernel %d %d %d %d" % (side, side, side, pf))
     return lines
@@ -327,7 +337,8 @@ def main():
             "# <key> <n1> <n2> <n3> <value>; zc:<kernel> = planes per tile, sched:<kernel> = 1 for tiles in",
             "# launch order (0: XCD-contiguous runs; 2: one workgroup per tile, stencil only; 3/4: launch order",
             "# with tiles dealt to the XCDs in chunks of 4/8 tile rows), pf:<kernel> =",
-            "# planes of loads in flight (stencil), k:<solver>:<precision> = probes per batch",
+            "# planes of loads in flight (stencil), x2:<kernel> = 1 for the pair march (stencil),",
+            "# wpc:<kernel> = workgroups per CU (0: occupancy), k:<solver>:<precision> = probes per batch",
             "# (solver 1 BiCGStab, 2 CG, 3 PipelinedCG; precision 0 FP64, 1 FP32 storage)"]
     with open(args.out, "w") as f:
         f.write("\n".join(head + lines) + "\n")

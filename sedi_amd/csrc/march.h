@@ -58,6 +58,7 @@ constexpr int NXCD = 8;
 struct TileGrid {
     int n1, n2, n3;
     int nbx, nby, nbz, zc, ty;
+    int tx;   // cells per tile row: TX (one per lane) or TX2 (a pair per lane)
     int xcd;  // 1: each XCD takes a contiguous run of tiles; 2: chunks of xg tiles
               // dealt round-robin to the XCDs; 0: tiles in launch order
     int xg;   // xcd == 2: tiles per chunk (whole rows of tiles along x)
@@ -65,10 +66,11 @@ struct TileGrid {
     long long ntiles;
 };
 
-inline TileGrid make_tile_grid(const Shape& s, int zc, int ty = TY) {
+inline TileGrid make_tile_grid(const Shape& s, int zc, int ty = TY, int tx = TX) {
     TileGrid t;
     t.n1 = s.n1; t.n2 = s.n2; t.n3 = s.n3;
-    t.nbx = (s.n1 + TX - 1) / TX;
+    t.tx = tx;
+    t.nbx = (s.n1 + tx - 1) / tx;
     t.nby = (s.n2 + ty - 1) / ty;
     t.zc = zc;
     t.ty = ty;
@@ -109,7 +111,7 @@ inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sc
                             bool one_per_tile = false) {
     KGeom k = planned;
     if (zc == 8 || zc == 16 || zc == 32) {
-        k.tg = make_tile_grid(s, zc, planned.tg.ty);
+        k.tg = make_tile_grid(s, zc, planned.tg.ty, planned.tg.tx);
         k.nblocks = (int)std::min<long long>(k.tg.ntiles, planned.nblocks);
     }
     if (sched >= 1 && sched <= 4) {
@@ -127,8 +129,8 @@ inline KGeom tuned_geometry(const KGeom& planned, const Shape& s, int zc, int sc
     return k;
 }
 
-inline KGeom plan_geometry(const Shape& s, int resident, int D, int ty = TY) {
-    const long long nxy = (long long)((s.n1 + TX - 1) / TX) * ((s.n2 + ty - 1) / ty);
+inline KGeom plan_geometry(const Shape& s, int resident, int D, int ty = TY, int tx = TX) {
+    const long long nxy = (long long)((s.n1 + tx - 1) / tx) * ((s.n2 + ty - 1) / ty);
     long long best_cost = -1;
     int best_zc = 16;
     for (int zc : ZC_CHOICES) {
@@ -141,7 +143,7 @@ inline KGeom plan_geometry(const Shape& s, int resident, int D, int ty = TY) {
         }
     }
     KGeom k;
-    k.tg = make_tile_grid(s, best_zc, ty);
+    k.tg = make_tile_grid(s, best_zc, ty, tx);
     k.nblocks = (int)std::min<long long>(k.tg.ntiles, resident);
     return k;
 }
@@ -438,6 +440,144 @@ __device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
     for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
         march_tile<P, ZC, TYK>(p, g, tile_of(g, t, reverse), lds, acc);
+}
+
+// ---- the march with two cells per lane (FP64 pairs) -------------------------
+// Lane l of a wave owns the adjacent cells i0+2l and i0+2l+1, so a tile row is
+// TX2 = 128 cells and every access moves 16 bytes (one FP64 pair).  This needs
+// n1 even (a pair never straddles a row, and the pair index stays even) and
+// 16-byte aligned vectors; the host checks both.  A plane's second load is
+// made by three waves only: waves 0 and 1 take the y-halo rows below and above
+// the tile (one pair per lane), and lanes 2r and 2r+1 of wave 2 take the pairs
+// left and right of tile row r, of which the cells i0-1 and i0+128 are used.
+// The other waves issue one load per plane, where the one-cell march issues
+// two.  The LDS row holds the tile in columns 2..129 (16-byte aligned pairs),
+// cell i0-1 in column 1 and cell i0+128 in column 130.
+//
+// Pass interface (a pair of adjacent cells; offsets are the first cell's):
+//   static constexpr int ND, D;
+//   struct Raw;  Raw load(cidx_t) const;   double2 f(const Raw&) const;
+//   struct Aux;  Aux load_aux(cidx_t) const;
+//   void finish(cidx_t, const Raw&, const Aux&, double2 f, double2 Af, double (&acc)[ND]) const;
+// The stencil is the reference's expression, left to right, for each cell.
+constexpr int TX2 = 128;
+constexpr int LX2 = TX2 + 4;
+
+__device__ __forceinline__ double2 ldc2(const double* base, cidx_t i) {
+    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (unsigned)(i * 8u));
+}
+// Non-temporal store of a pair (the two stores merge into one 16-byte store).
+__device__ __forceinline__ void stc2_nt(double* base, cidx_t i, double2 v) {
+    double* q = reinterpret_cast<double*>(reinterpret_cast<char*>(base) + (unsigned)(i * 8u));
+    __builtin_nontemporal_store(v.x, q);
+    __builtin_nontemporal_store(v.y, q + 1);
+}
+
+template <class P, int ZC, int TYK>
+__device__ __forceinline__ void march_tile_x2(const P& p, const TileGrid& g, long long tile,
+                                              double (&lds)[2][TYK + 2][LX2], double (&acc)[P::ND]) {
+    static_assert(TYK >= 3 && 2 * TYK <= 64, "three waves load the halo; wave 2 covers 2*TYK x-halo pairs");
+    constexpr int D = P::D;
+    constexpr int R = D + 1;
+    const double2 zero = {0.0, 0.0};
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long t_xy = (long long)g.nbx * g.nby;
+    const int bz = (int)(tile / t_xy);
+    const long long rem = tile - (long long)bz * t_xy;
+    const int by = (int)(rem / g.nbx);
+    const int bx = (int)(rem - (long long)by * g.nbx);
+    const int i0 = bx * TX2, j0 = by * TYK, zs = bz * ZC;
+    const int n1 = g.n1, n2 = g.n2, n3 = g.n3;
+    const cidx_t s3 = (cidx_t)g.s3;
+    const int i = i0 + 2 * lane, j = j0 + w;
+    const bool c_in = (i < n1) && (j < n2);
+    // clamped, in-grid, even pair index (n1 is even)
+    const cidx_t cxy = (cidx_t)(min(i, n1 - 2) + min(j, n2 - 1) * n1);
+
+    int ei = i, ej = j, xr = 0, xs = 0;
+    bool ex_in = false;
+    const bool has_ex = w < 3;
+    if (w == 0) {
+        ej = j0 - 1;
+        ex_in = i < n1 && ej >= 0;
+    } else if (w == 1) {
+        ej = j0 + TYK;
+        ex_in = i < n1 && ej < n2;
+    } else if (w == 2) {
+        xr = lane >> 1;
+        xs = lane & 1;
+        ej = j0 + xr;
+        ei = xs ? i0 + TX2 : i0 - 2;
+        ex_in = lane < 2 * TYK && ej < n2 && ei >= 0 && ei < n1;
+    }
+    const cidx_t exy = (cidx_t)(min(max(ei, 0), n1 - 2) + min(max(ej, 0), n2 - 1) * n1);
+
+    typename P::Raw rc[R], rx[R];
+    typename P::Aux ra[R];
+    auto zoff = [&](int zz) { return (cidx_t)min(max(zz, 0), n3 - 1) * s3; };
+    auto ld_f = [&](int zz, int slot) {
+        rc[slot] = p.load(cxy + zoff(zz));
+        if (has_ex) rx[slot] = p.load(exy + zoff(zz));
+    };
+    auto ld_a = [&](int zz, int slot) { ra[slot] = p.load_aux(cxy + zoff(zz)); };
+    auto put = [&](int buf, const double2& fcv, const double2& fe) {
+        *reinterpret_cast<double2*>(&lds[buf][w + 1][2 + 2 * lane]) = fcv;
+        if (w == 0) {
+            *reinterpret_cast<double2*>(&lds[buf][0][2 + 2 * lane]) = fe;
+        } else if (w == 1) {
+            *reinterpret_cast<double2*>(&lds[buf][TYK + 1][2 + 2 * lane]) = fe;
+        } else if (w == 2 && lane < 2 * TYK) {
+            if (xs) lds[buf][xr + 1][2 + TX2] = fe.x;
+            else lds[buf][xr + 1][1] = fe.y;
+        }
+    };
+
+    ld_f(zs - 1, 0);
+    ld_f(zs, 1 % R);
+#pragma unroll
+    for (int k = 1; k < D && k <= ZC; k++) ld_f(zs + k, (k + 1) % R);
+#pragma unroll
+    for (int k = 0; k < D && k < ZC; k++) ld_a(zs + k, k);
+    double2 fm = (c_in && zs >= 1) ? p.f(rc[0]) : zero;
+    double2 fc = (c_in && zs < n3) ? p.f(rc[1 % R]) : zero;
+    typename P::Raw cc = rc[1 % R];
+    put(0, fc, (ex_in && zs < n3) ? p.f(rx[1 % R]) : zero);
+    if (D <= ZC) ld_f(zs + D, (D + 1) % R);
+    lds_barrier();
+
+#pragma unroll
+    for (int u = 0; u < ZC; u++) {
+        const int z = zs + u;
+        const int b = u & 1;
+        if (u + 1 + D <= ZC) ld_f(z + 1 + D, (u + 1) % R);
+        if (u + D < ZC) ld_a(z + D, (u + D) % R);
+        const bool nxt = z + 1 < n3;
+        const typename P::Raw cn = rc[(u + 2) % R];
+        const double2 fn = (c_in && nxt) ? p.f(cn) : zero;
+        if (u + 1 < ZC) put(b ^ 1, fn, (ex_in && nxt) ? p.f(rx[(u + 2) % R]) : zero);
+        const double2 xl = *reinterpret_cast<const double2*>(&lds[b][w + 1][2 * lane]);      // cells 2l-2, 2l-1
+        const double2 xh = *reinterpret_cast<const double2*>(&lds[b][w + 1][2 * lane + 4]);  // cells 2l+2, 2l+3
+        const double2 ym = *reinterpret_cast<const double2*>(&lds[b][w][2 + 2 * lane]);
+        const double2 yp = *reinterpret_cast<const double2*>(&lds[b][w + 2][2 + 2 * lane]);
+        double2 Af;
+        Af.x = p.a * fc.x - p.c * xl.y - p.c * ym.x - p.c * fm.x - p.b * fc.y - p.b * yp.x - p.b * fn.x;
+        Af.y = p.a * fc.y - p.c * fc.x - p.c * ym.y - p.c * fm.y - p.b * xh.x - p.b * yp.y - p.b * fn.y;
+        if (c_in && z < n3) p.finish(cxy + (cidx_t)z * s3, cc, ra[u % R], fc, Af, acc);
+        lds_barrier();
+        fm = fc;
+        fc = fn;
+        cc = cn;
+    }
+}
+
+template <class P, int ZC, int TYK>
+__device__ __forceinline__ void run_tiles_x2(const P& p, const TileGrid& g, double (&lds)[2][TYK + 2][LX2],
+                                             double (&acc)[P::ND], bool reverse = false) {
+#pragma unroll
+    for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
+    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+        march_tile_x2<P, ZC, TYK>(p, g, tile_of(g, t, reverse), lds, acc);
 }
 
 }  // namespace sedi

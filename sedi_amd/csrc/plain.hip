@@ -399,6 +399,44 @@ __global__ __launch_bounds__(TX * STY) void stencil_apply_kernel_pf8(TileGrid g,
     run_tiles<decltype(P), ZC, STY>(P, g, lds, acc);
 }
 
+// y = A x with two cells per lane (march_tile_x2): one 16-byte load of x and
+// one 16-byte store of y per pair and plane, and one load per plane for all
+// but three waves.  Measured cold at 512^3: 5-8% faster than the one-cell
+// stencil; needs n1 even and 16-byte aligned x and y.
+template <int DD>
+struct PassApplyX2 {
+    static constexpr int ND = 1, D = DD;
+    double a, b, c;
+    const double* x;
+    double* y;
+    using Raw = double2;
+    struct Aux {};
+    __device__ Raw load(cidx_t i) const { return ldc2(x, i); }
+    __device__ Aux load_aux(cidx_t) const { return {}; }
+    __device__ double2 f(const Raw& r) const { return r; }
+    __device__ void finish(cidx_t i, const Raw&, const Aux&, const double2&, const double2& Af,
+                           double (&)[ND]) const {
+        stc2_nt(y, i, Af);
+    }
+};
+
+template <int ZC>
+__global__ __launch_bounds__(TX * STY) void stencil_apply_kernel_x2(TileGrid g, double a, double b, double c,
+                                                                    const double* x, double* y) {
+    __shared__ double lds[2][STY + 2][LX2];
+    PassApplyX2<2> P{a, b, c, x, y};
+    double acc[1];
+    run_tiles_x2<decltype(P), ZC, STY>(P, g, lds, acc);
+}
+template <int ZC>
+__global__ __launch_bounds__(TX * STY) void stencil_apply_kernel_x2_pf4(TileGrid g, double a, double b, double c,
+                                                                        const double* x, double* y) {
+    __shared__ double lds[2][STY + 2][LX2];
+    PassApplyX2<4> P{a, b, c, x, y};
+    double acc[1];
+    run_tiles_x2<decltype(P), ZC, STY>(P, g, lds, acc);
+}
+
 template <class PC, int ZC>
 __global__ __launch_bounds__(PTHREADS, 4) void residual_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, const double* x,
@@ -645,7 +683,7 @@ constexpr int MAX_BLOCKS_PER_CU = 8;
 // (the ZC variants differ only in unrolling); cached per kernel and shape.
 template <typename K>
 static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, int ty = PTY,
-                             bool one_per_tile = false) {
+                             bool one_per_tile = false, int tx = TX) {
     static std::mutex mu;
     static std::map<std::tuple<const void*, int, int, int>, KGeom> cache;
     const auto key = std::make_tuple(reinterpret_cast<const void*>(kern16), s.n1, s.n2, s.n3);
@@ -658,13 +696,20 @@ static const KGeom& geom_for(const char* name, K kern16, const Shape& s, int D, 
         occ = 4;
     }
     occ = std::min(occ, MAX_BLOCKS_PER_CU);
+    // "wpc:<kernel>" > 0 launches at most that many workgroups per CU, which
+    // loop over the tiles (default 1 for the pair march: one 16-wave workgroup
+    // per CU measured faster than two); 0, the default of the other kernels,
+    // queues up to MAX_BLOCKS_PER_CU per CU in the launch-order schedules.
+    // One workgroup per tile (schedule 2) ignores it.
+    const int wpc = tuned_value(std::string("wpc:") + name, s, tx == TX2 ? 1 : 0);
+    if (wpc > 0) occ = std::min(occ, wpc);
     // the occupancy plan, with the autotuner's z chunk and schedule for this
     // kernel and grid (the partial sums hold MAX_BLOCKS_PER_CU per CU)
-    const KGeom planned = plan_geometry(s, occ * num_cus(), D, ty);
+    const KGeom planned = plan_geometry(s, occ * num_cus(), D, ty, tx);
     const int zc = tuned_value(std::string("zc:") + name, s, 0);
     const int sched = tuned_value(std::string("sched:") + name, s, 0);
-    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, MAX_BLOCKS_PER_CU * num_cus(), one_per_tile))
-        .first->second;
+    const int max_blocks = (wpc > 0 ? occ : MAX_BLOCKS_PER_CU) * num_cus();
+    return cache.emplace(key, tuned_geometry(planned, s, zc, sched, max_blocks, one_per_tile)).first->second;
 }
 
 // kernel<..., ZC> for the ZC the geometry chose
@@ -681,20 +726,36 @@ int stencil_apply_dev(const Shape& s, double a, double b, double c, const double
                       hipStream_t stream) {
     SEDI_REQUIRE(offsets_fit_32(s.n(), sizeof(double)), SEDI_ERR_UNSUPPORTED,
                  "grid of %lld cells too large: an FP64 vector must be < 4 GiB (32-bit offsets)", s.n());
-    // no partial sums: one workgroup per tile is a schedule choice too
-    const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY, true);
-    // planes of loads in flight: the autotuner's "pf:stencil_apply_kernel" (2, 4 or 8)
+    // planes of loads in flight: the autotuner's "pf:stencil_apply_kernel" (2, 4 or 8;
+    // the pair march has 2 and 4); "x2:stencil_apply_kernel" 0 selects the
+    // one-cell march where the pair march could run (default: the pair march)
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int>, int> pf_cache;
-    int pf;
+    static std::map<std::tuple<int, int, int>, std::pair<int, int>> plan_cache;
+    int pf, x2;
     {
         std::lock_guard<std::mutex> lk(mu);
-        auto it = pf_cache.find(std::make_tuple(s.n1, s.n2, s.n3));
-        if (it == pf_cache.end())
-            it = pf_cache.emplace(std::make_tuple(s.n1, s.n2, s.n3),
-                                  tuned_value("pf:stencil_apply_kernel", s, SEDI_PREFETCH_APPLY)).first;
-        pf = it->second;
+        auto it = plan_cache.find(std::make_tuple(s.n1, s.n2, s.n3));
+        if (it == plan_cache.end())
+            it = plan_cache.emplace(std::make_tuple(s.n1, s.n2, s.n3),
+                                    std::make_pair(tuned_value("pf:stencil_apply_kernel", s, SEDI_PREFETCH_APPLY),
+                                                   tuned_value("x2:stencil_apply_kernel", s, 1))).first;
+        pf = it->second.first;
+        x2 = it->second.second;
     }
+    const bool pairs_ok = s.n1 % 2 == 0 && (reinterpret_cast<uintptr_t>(x) % 16) == 0 &&
+                          (reinterpret_cast<uintptr_t>(y) % 16) == 0;
+    if (x2 && pairs_ok) {
+        const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel_x2<16>, s, 2, STY, true, TX2);
+        if (pf >= 4) {
+            SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel_x2_pf4, k, stream, k.tg, a, b, c,
+                                    x, y));
+        } else {
+            SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel_x2, k, stream, k.tg, a, b, c, x, y));
+        }
+        return SEDI_OK;
+    }
+    // no partial sums: one workgroup per tile is a schedule choice too
+    const KGeom& k = geom_for("stencil_apply_kernel", stencil_apply_kernel<16>, s, PassApply::D, STY, true);
     if (pf == 8) {
         SEDI_HIP(SEDI_ZC_LAUNCH("stencil_apply_kernel", stencil_apply_kernel_pf8, k, stream, k.tg, a, b, c, x, y));
     } else if (pf == 4) {

@@ -74,9 +74,10 @@ const std::string& tuning_path() {
 int tuned_value(const std::string& key, const Shape& s, int fallback, bool nearest) {
     std::call_once(g_once, load);
     if (g_table.empty()) return fallback;
-    // per-kernel keys ("zc:<kernel>", "sched:<kernel>") fall back to "<kind>:*"
+    // per-kernel keys ("zc:<kernel>", "sched:<kernel>", "wpc:<kernel>") fall back to "<kind>:*"
     const size_t colon = key.find(':');
-    const bool per_kernel = key.compare(0, 3, "zc:") == 0 || key.compare(0, 6, "sched:") == 0;
+    const bool per_kernel = key.compare(0, 3, "zc:") == 0 || key.compare(0, 6, "sched:") == 0 ||
+                            key.compare(0, 4, "wpc:") == 0;
     const std::string wild = per_kernel ? key.substr(0, colon + 1) + "*" : key;
     for (const std::string& k : {key, wild}) {
         if (const Entry* e = find(k, s, false)) return e->value;
