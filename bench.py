@@ -149,8 +149,29 @@ def pmc_workload():
 
 
 # footprint of a cold measurement: the (x, y) pairs it rotates through span
-# at least four times the 256 MB Infinity Cache
+# at least four times the 256 MB Infinity Cache, and there are at least four
+# of them: at 512^3 a launch's time depends on where its pair's buffers lie
+# in physical memory (one plan ran 363 us on two of four pairs and 416 us on
+# the other two), so the mean is taken over several placements
 COLD_BYTES = 1 << 30
+COLD_MIN_PAIRS = 4
+# y of pair k < 4 is allocated after a spacer of k * COLD_SPACER bytes, so
+# the pairs place y at different offsets from x
+COLD_SPACER = 34 << 20
+
+
+def cold_pairs(n, npairs, x0=None):
+    """npairs device-resident (x, y) FP64 pairs of n cells for a cold
+    measurement, y of pair k < 4 allocated after a k * COLD_SPACER spacer
+    (the spacers are returned to be freed with the pairs)."""
+    import sedi_amd
+    pairs, spacers = [], []
+    for k in range(npairs):
+        x = sedi_amd.DeviceArray(x0) if x0 is not None else sedi_amd.DeviceArray(n=n)
+        if 0 < k < 4:
+            spacers.append(sedi_amd.DeviceArray(n=k * COLD_SPACER // 8))
+        pairs.append((x, sedi_amd.DeviceArray(n=n)))
+    return pairs, spacers
 
 
 def stencil_only(reps=20, side=None, cold=False):
@@ -159,18 +180,19 @@ def stencil_only(reps=20, side=None, cold=False):
     around each launch; algorithmic bytes 2*N*8 per launch (read x, write y).
     Warm: back-to-back launches on one (x, y), so at 256^3 part of x is still
     in the 256 MB Infinity Cache from the previous launch.  Cold: launches
-    rotate through enough (x, y) pairs to span 1 GiB (4 at 256^3, 2 at
-    512^3), so each launch reads an x untouched since three launches back
-    (evicted) while the cache holds the previous launch's dirty y, as in a
-    solver's sweep over vectors larger than the cache."""
+    rotate through four (x, y) pairs (spanning 1 GiB at 256^3, 8 GiB at
+    512^3; y at four different offsets from x, see cold_pairs), so each
+    launch reads an x untouched since three launches back (evicted) while the
+    cache holds the previous launch's dirty y, as in a solver's sweep over
+    vectors larger than the cache."""
     import ctypes
     import numpy as np
     import sedi_amd
     grid = (side,) * 3 if side else GRID
     n = grid[0] * grid[1] * grid[2]
-    npairs = max(2, -(-COLD_BYTES // (16 * n))) if cold else 1
+    npairs = max(COLD_MIN_PAIRS, -(-COLD_BYTES // (16 * n))) if cold else 1
     x0 = np.random.default_rng(0).standard_normal(n)
-    pairs = [(sedi_amd.DeviceArray(x0), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
+    pairs, spacers = cold_pairs(n, npairs, x0)
     g, _ = sedi_amd._grid(grid)
     st = sedi_amd.Stencil()._c()
     L = sedi_amd.lib()
@@ -191,9 +213,8 @@ def stencil_only(reps=20, side=None, cold=False):
     launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
     avg = ms / 1e3 / launches
     gbs = 2 * n * 8 / avg / 1e9
-    for x, y in pairs:
-        x.free()
-        y.free()
+    for a in [v for p in pairs for v in p] + spacers:
+        a.free()
     return {"kernel": "stencil_apply_kernel", "grid": list(grid), "cache": "cold" if cold else "warm",
             "pairs": npairs, "launches": launches, "avg_launch_us": round(avg * 1e6, 2),
             "achieved_GBs": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)}
@@ -215,8 +236,8 @@ def copy_only(reps=12, side=None, cold=False):
     H.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
     grid = (side,) * 3 if side else GRID
     n = grid[0] * grid[1] * grid[2]
-    npairs = max(2, -(-COLD_BYTES // (16 * n))) if cold else 1
-    pairs = [(sedi_amd.DeviceArray(n=n), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
+    npairs = max(COLD_MIN_PAIRS, -(-COLD_BYTES // (16 * n))) if cold else 1
+    pairs, spacers = cold_pairs(n, npairs)
     for a, b in pairs:
         a.fill_bytes(1)
         b.fill_bytes(2)
@@ -236,9 +257,8 @@ def copy_only(reps=12, side=None, cold=False):
             times.append(ms.value)
     H.hipEventDestroy(e0)
     H.hipEventDestroy(e1)
-    for a, b in pairs:
+    for a in [v for p in pairs for v in p] + spacers:
         a.free()
-        b.free()
     avg = sum(times) / len(times) * 1e-3
     gbs = 2 * n * 8 / avg / 1e9
     return {"kernel": "hipMemcpyAsync device to device", "grid": list(grid), "cache": "cold" if cold else "warm",

@@ -32,7 +32,8 @@ at run time (``sedi_amd/tuning.txt``; see csrc/tuning.h).
    forced in a child process and timed cold -- launches rotate through (x, y)
    pairs spanning 1 GiB of device memory, so x comes from HBM as in a sweep
    over vectors larger than the Infinity Cache -- replacing step 1's stencil
-   entries.
+   entries.  At 512^3 a launch's time depends on where x and y lie in
+   physical memory, so the pairs place y at different offsets from x.
 
 ``--k-only`` re-measures the batch sizes alone, with the chunk and schedule
 entries of the existing table (--out), which it keeps; ``--stencil-only``
@@ -149,22 +150,40 @@ SCHEDS = (0, 1, 3, 4)
 
 def measure_stencil(sides, reps):
     """Child: per grid, the mean launch time of the standalone stencil, cold
-    (launches rotate through (x, y) pairs spanning 1 GiB)."""
+    (back-to-back launches rotate through at least four (x, y) pairs spanning
+    1 GiB, y at different offsets from x)."""
     import ctypes
     import numpy as np
     import sedi_amd
     out = {}
     L = sedi_amd.lib()
     st = sedi_amd.Stencil()._c()
-    for side in sides:
+    # largest grid first: its buffers are then laid out as in a fresh process
+    # (after the many small pairs of a small grid were allocated and freed,
+    # 512^3 plans measured up to 10% faster here than they run in one)
+    for side in sorted(sides, reverse=True):
         n = side ** 3
         g, _ = sedi_amd._grid((side,) * 3)
-        npairs = max(2, -(-(1 << 30) // (16 * n)))
+        # at least four pairs: at 512^3 a launch's time depends on where its
+        # pair's buffers lie in physical memory (the offset of y from x), so
+        # plans are compared on the mean over several placements
+        npairs = max(4, -(-(1 << 30) // (16 * n)))
         x0 = np.random.default_rng(0).standard_normal(n)
-        pairs = [(sedi_amd.DeviceArray(x0), sedi_amd.DeviceArray(n=n)) for _ in range(npairs)]
+        # y of pair k < 4 after a k * 34 MiB spacer: the pairs place y at
+        # different offsets from x, and a plan is judged over all of them
+        pairs, spacers = [], []
+        for k in range(npairs):
+            x = sedi_amd.DeviceArray(x0)
+            if 0 < k < 4:
+                spacers.append(sedi_amd.DeviceArray(n=k * (34 << 20) // 8))
+            pairs.append((x, sedi_amd.DeviceArray(n=n)))
         for x, y in pairs:
             if L.sedi_stencil_apply(ctypes.byref(g), ctypes.byref(st), x.ptr, y.ptr) < 0:
                 raise RuntimeError(L.sedi_last_error().decode())
+        # back-to-back launches, as in a solver's sweep: a launch also pays for
+        # writing back what the previous one left dirty in the Infinity Cache
+        # (timing launches one at a time, with the device idle in between,
+        # ranked the plans differently and 7-12% faster than they sustain)
         sedi_amd.profile_reset()
         sedi_amd.profile_enable("stencil_apply_kernel")
         for r in range(reps):
@@ -173,9 +192,8 @@ def measure_stencil(sides, reps):
         sedi_amd.profile_enable(None)
         launches, ms = sedi_amd.profile_get("stencil_apply_kernel")
         out[side] = 1e3 * ms / launches
-        for x, y in pairs:
-            x.free()
-            y.free()
+        for a in [v for p in pairs for v in p] + spacers:
+            a.free()
     return out
 
 
@@ -267,7 +285,7 @@ def main():
         print(json.dumps(measure_chunks(args.sides, args.iters)))
         return
     if args.child == "stencil":
-        print(json.dumps(measure_stencil(args.sides, 12)))
+        print(json.dumps(measure_stencil(args.sides, 16)))
         return
     if args.child == "batches":
         print(json.dumps(measure_batches(args.sides, args.iters)))
