@@ -36,6 +36,11 @@ iteration counts and the mixed diagonal's error against the FP64 one.
 batches of 16, tol 1e-8, maxiter 4000; the replacement policy of each
 solver's mixed form) against the plain FP64 GPU solvers on the same probes:
 throughput and max / RMS diagonal error.
+--mode arith: the arithmetic type of that mixed pipelined CG and BiCGStab:
+the default library (FP32 storage, stencil and updates in FP64 registers)
+beside its f32compute build (the same kernels in FP32 registers) and the
+plain FP64 estimator, on the first --arith-samples probes of the 512^3
+configuration in batches of 8.
 --mode serve: the serving stream -- Poisson arrivals at --rate jobs/s, grid
 side uniform over {64, 96, 128, 192, 256}, samples uniform in [8, 64], tol
 uniform over {1e-6, 1e-8, 1e-9}, solver uniform over CG / pipelined CG /
@@ -89,7 +94,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512", "serve"])
+    p.add_argument("--mode", default="headline", choices=["headline", "plain", "ksweep", "c512", "serve", "arith"])
+    p.add_argument("--arith-child", type=int, default=0, help=argparse.SUPPRESS)
+    p.add_argument("--arith-out", default="", help=argparse.SUPPRESS)
+    p.add_argument("--arith-samples", type=int, default=8, help="--mode arith: probes per solver")
     p.add_argument("--jobs", type=int, default=200, help="serve: jobs in the stream")
     p.add_argument("--rate", default="5", help="serve: Poisson arrival rate(s) in jobs/s, comma-separated")
     p.add_argument("--serve-sub-jobs", type=int, default=40,
@@ -431,6 +439,12 @@ def run_headline(args):
     import sedi_amd
     n = GRID[0] * GRID[1] * GRID[2]
     K = max(4, (args.batch + 3) // 4 * 4)
+    stencil = None
+    if args.stencil_line:
+        # the standalone 7-point stencil on device-resident FP64 vectors: warm
+        # (back-to-back on one pair) and cold (x from HBM) at 256^3, cold at
+        # 512^3; measured first, on buffers laid out as in a fresh process
+        stencil = [stencil_only(side=256), stencil_only(side=256, cold=True), stencil_only(side=512, cold=True)]
     est = sedi_amd.Estimator(GRID, kind=2, samples=BATCH["samples"], maxiter=MAXITER,
                              tol=BATCH["tol"], seed=BATCH["seed"], ignore_errors=True,
                              precision=sedi_amd.FP32_MIXED, batch_k=K,
@@ -465,11 +479,8 @@ def run_headline(args):
                           "unconverged": int(np.count_nonzero(codes))}
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
     line["roofline"] = roofline(BATCH_KERNEL, launches, kms, rhs_values * n * 4)
-    if args.stencil_line:
-        # the standalone 7-point stencil on device-resident FP64 vectors: warm
-        # (back-to-back on one pair) and cold (x from HBM) at 256^3, cold at 512^3
-        line["stencil"] = [stencil_only(side=256), stencil_only(side=256, cold=True),
-                           stencil_only(side=512, cold=True)]
+    if stencil is not None:
+        line["stencil"] = stencil
     if not args.no_compare:
         # plain FP64 drop-in and the mixed form on the first 16 probes of the
         # same stream: the speed-up and the diagonal error of the mixed form
@@ -585,6 +596,64 @@ def run_ksweep(args):
             print(json.dumps(line), flush=True)
             out["runs"].append({kk: v for kk, v in line.items() if kk != "iters"})
     return out
+
+
+ARITH_LIB = os.path.join(ROOT, "sedi_amd", "variants", "libsedi_amd_f32compute.so")
+
+
+def run_arith(args):
+    """The arithmetic type of the mixed pipelined CG and BiCGStab on the 512^3
+    configuration (tol 1e-8, seed 23, k = 8, first --arith-samples probes):
+    the default library (FP32 storage, stencil and updates computed in FP64
+    registers) beside the f32compute build (the same kernels computing in
+    FP32 registers; a child process loads it through SEDI_AMD_LIB) and the
+    plain FP64 estimator on the same probes.  Reported per form: iterations,
+    unconverged probes, probe-solves/s and D against FP64."""
+    import numpy as np
+    import sedi_amd
+    import tempfile
+    grid = (512, 512, 512)
+    ns = args.arith_samples
+    common = dict(samples=ns, tol=1e-8, seed=23)
+    out = {"mode": "arith", "grid": list(grid), "samples": ns, "seed": 23, "tol": 1e-8, "batch_k": 8,
+           "maxiter": 4000, "runs": []}
+    if not os.path.exists(ARITH_LIB):
+        raise RuntimeError("%s is missing: build it with sedi_amd.build.build_variant('f32compute')" % ARITH_LIB)
+    for kind in (3, 1):
+        name = SOLVER_LABEL[kind]
+        with tempfile.TemporaryDirectory() as tmp:
+            dpath = os.path.join(tmp, "D.npy")
+            env = dict(os.environ, SEDI_AMD_LIB=ARITH_LIB)
+            res = subprocess.run([sys.executable, os.path.abspath(__file__), "--mode", "arith", "--arith-child",
+                                  str(kind), "--arith-out", dpath, "--arith-samples", str(ns)],
+                                 env=env, stdout=subprocess.PIPE, check=True)
+            f32 = json.loads(res.stdout.decode().strip().splitlines()[-1])
+            D32 = np.load(dpath)
+        m = estimator_run(grid, progress=True, maxiter=4000, kind=kind, precision=sedi_amd.FP32_MIXED, batch_k=8,
+                          rep_every=50, **common)
+        print(json.dumps(dict(m["summary"], solver=name, arithmetic="fp64 registers")), flush=True)
+        p = estimator_run(grid, progress=True, maxiter=4000, kind=kind, **common)
+        print(json.dumps(dict(p["summary"], solver=name, precision="fp64 plain")), flush=True)
+        out["runs"].append({"solver": name, "fp64_registers": m["summary"], "fp32_registers": f32,
+                            "fp64_plain": p["summary"], "D_vs_fp64": {"fp64_registers": d_errors(m["D"], p["D"]),
+                                                                      "fp32_registers": d_errors(D32, p["D"])}})
+        print(json.dumps(out["runs"][-1]), flush=True)
+        del m, p
+    return out
+
+
+def arith_child(args):
+    """run_arith's child: the mixed estimator of one solver with the library
+    SEDI_AMD_LIB names; prints its summary, saves D to --arith-out."""
+    import numpy as np
+    import sedi_amd
+    if os.path.realpath(sedi_amd.lib_path) != os.path.realpath(ARITH_LIB):
+        raise RuntimeError("the f32compute child loaded %s" % sedi_amd.lib_path)
+    kind = args.arith_child
+    r = estimator_run((512, 512, 512), progress=True, maxiter=4000, kind=kind, samples=args.arith_samples,
+                      tol=1e-8, seed=23, precision=sedi_amd.FP32_MIXED, batch_k=8, rep_every=50)
+    np.save(args.arith_out, r["D"])
+    print(json.dumps(dict(r["summary"], solver=SOLVER_LABEL[kind], arithmetic="fp32 registers")), flush=True)
 
 
 def run_c512(args):
@@ -730,6 +799,9 @@ def main():
         print("bench.py measures one GPU: this build runs the whole grid on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "arith" and args.arith_child:
+        arith_child(args)
+        return
     if args.mode == "serve":
         runs = []
         for rate in [float(v) for v in args.rate.split(",")]:
@@ -743,6 +815,8 @@ def main():
         line = run_ksweep(args)
     elif args.mode == "c512":
         line = run_c512(args)
+    elif args.mode == "arith":
+        line = run_arith(args)
     elif args.mode == "plain":
         line = run_plain(args)
     else:

@@ -85,6 +85,10 @@ VARIANTS = {
     # with an acquire fence (the C++ memory model's own guarantee) instead of
     # agent-scope atomic stores and a wait on their completion
     "fence": ["-DSEDI_GRID_SUM_FENCE"],
+    # the mixed-precision pipelined CG and BiCGStab computing their stencil
+    # and vector updates in FP32 registers (the default computes them in FP64
+    # on the FP32-stored vectors); bench.py --mode arith measures both
+    "f32compute": ["-DSEDI_MIXED_COMPUTE_F64=0"],
 }
 
 
