@@ -12,10 +12,10 @@
 // workgroups per CU; 16 for the standalone stencil and, on large grids, the
 // batched CG update pass), and marches over a chunk of ZC planes along z.  For
 // each plane it evaluates f at its cells plus the 2*64 + 2*TYK in-plane halo
-// cells (lanes 0..31 of the first four waves take half a y-halo row, lanes
-// 32/33 of every wave the two x-halo cells of its row), stores them in a
-// double-buffered LDS plane and needs one workgroup barrier per plane.  The z
-// neighbours stay in registers.  The y halo costs 2/TYK extra rows per row.
+// cells (waves 0 and 1 take the y-halo rows, wave 2 the x-halo cells of every
+// row; the other waves load no halo), stores them in a double-buffered LDS
+// plane and needs one workgroup barrier per plane.  The z neighbours stay in
+// registers.  The y halo costs 2/TYK extra rows per row.
 //
 // Latency: all loads are issued D planes before they are consumed, into a
 // ring of registers whose slots are compile-time constants (the z loop is
@@ -328,11 +328,11 @@ struct value_of<P, std::void_t<typename P::V>> { using type = typename P::V; };
 template <class P>
 using value_t = typename value_of<P>::type;
 
-template <class P, int ZC, int TYK = TY>
+template <class P, int ZC, int TYK, bool EX>
 __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long long tile,
                                            value_t<P> (&lds)[2][TYK + 2][TX + 2],
                                            double (&acc)[P::ND]) {
-    static_assert(TYK >= 4, "the halo rows are loaded by the first four waves");
+    static_assert(TYK >= 3 && 2 * TYK <= 64, "three waves load the halo; wave 2 covers 2*TYK x-halo cells");
     using V = value_t<P>;
     const V zero{};
     constexpr int D = P::D;
@@ -355,23 +355,32 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     // code, so the compiler's wait counts on the prefetch ring stay exact
     const cidx_t cxy = (cidx_t)(min(i, g.n1 - 1) + min(j, g.n2 - 1) * g.n1);
 
-    // the one extra (halo) cell this lane evaluates per plane
-    int ex_i = i, ex_j = j, ex_r = 0, ex_c = 0;
+    // the halo: waves 0 and 1 evaluate the y-halo rows below and above the
+    // tile (one cell per lane), lanes 2r and 2r+1 of wave 2 the x-halo cells
+    // left and right of tile row r; the other waves load no halo, so a plane
+    // costs them one load per field.  EX (true for waves 0-2) is a compile-time
+    // constant: each kind of wave runs straight-line code whose wait counts
+    // keep exactly D planes of its own loads in flight (a run-time branch
+    // around the halo loads made the compiler wait for the next plane's loads
+    // as well in the halo waves, which every wave then waited for at the
+    // plane's barrier)
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    int ex_i = i, ex_j = j, ex_r = w + 1, ex_c = lane + 1;
     bool ex_act = false;
-    if (lane < 32 && w < 4) {
-        ex_j = (w < 2) ? j0 - 1 : j0 + TYK;
-        ex_i = i0 + (w & 1) * 32 + lane;
-        ex_r = (w < 2) ? 0 : TYK + 1;
-        ex_c = 1 + (w & 1) * 32 + lane;
+    if (!EX) {
+    } else if (wu == 0) {
+        ex_j = j0 - 1; ex_r = 0; ex_act = true;
+    } else if (wu == 1) {
+        ex_j = j0 + TYK; ex_r = TYK + 1; ex_act = true;
+    } else if (wu == 2 && lane < 2 * TYK) {
+        ex_j = j0 + (lane >> 1);
+        ex_r = (lane >> 1) + 1;
+        ex_i = (lane & 1) ? i0 + TX : i0 - 1;
+        ex_c = (lane & 1) ? TX + 1 : 0;
         ex_act = true;
-    } else if (lane == 32) {
-        ex_i = i0 - 1; ex_r = w + 1; ex_c = 0; ex_act = true;
-    } else if (lane == 33) {
-        ex_i = i0 + TX; ex_r = w + 1; ex_c = TX + 1; ex_act = true;
     }
     const bool ex_in = ex_act && ex_i >= 0 && ex_i < g.n1 && ex_j >= 0 && ex_j < g.n2;
     const cidx_t exy = (cidx_t)(min(max(ex_i, 0), g.n1 - 1) + min(max(ex_j, 0), g.n2 - 1) * g.n1);
-    if (!ex_act) { ex_r = w + 1; ex_c = lane + 1; }  // harmless duplicate of the centre slot
 
     // rings: f-inputs of plane zs+q-1 live in slot q % R, update inputs of
     // plane zs+q in slot q % R
@@ -380,7 +389,7 @@ __device__ __forceinline__ void march_tile(const P& p, const TileGrid& g, long l
     auto zoff = [&](int zz) { return (cidx_t)min(max(zz, 0), n3 - 1) * s3; };
     auto ld_f = [&](int zz, int slot) {
         rc[slot] = p.load(cxy + zoff(zz));
-        rx[slot] = p.load(exy + zoff(zz));
+        if (EX) rx[slot] = p.load(exy + zoff(zz));
     };
     auto ld_a = [&](int zz, int slot) { ra[slot] = p.load_aux(cxy + zoff(zz)); };
 
@@ -438,8 +447,15 @@ __device__ __forceinline__ void run_tiles(const P& p, const TileGrid& g,
                                           double (&acc)[P::ND], bool reverse = false) {
 #pragma unroll
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
-    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
-        march_tile<P, ZC, TYK>(p, g, tile_of(g, t, reverse), lds, acc);
+    // waves 0-2 load the halo (march_tile's EX); both bodies cross the same
+    // barriers
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) < 3) {
+        for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+            march_tile<P, ZC, TYK, true>(p, g, tile_of(g, t, reverse), lds, acc);
+    } else {
+        for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+            march_tile<P, ZC, TYK, false>(p, g, tile_of(g, t, reverse), lds, acc);
+    }
 }
 
 // ---- the march with two cells per lane (FP64 pairs) -------------------------
@@ -473,7 +489,7 @@ __device__ __forceinline__ void stc2_nt(double* base, cidx_t i, double2 v) {
     __builtin_nontemporal_store(v.y, q + 1);
 }
 
-template <class P, int ZC, int TYK>
+template <class P, int ZC, int TYK, bool EX>
 __device__ __forceinline__ void march_tile_x2(const P& p, const TileGrid& g, long long tile,
                                               double (&lds)[2][TYK + 2][LX2], double (&acc)[P::ND]) {
     static_assert(TYK >= 3 && 2 * TYK <= 64, "three waves load the halo; wave 2 covers 2*TYK x-halo pairs");
@@ -497,8 +513,8 @@ __device__ __forceinline__ void march_tile_x2(const P& p, const TileGrid& g, lon
 
     int ei = i, ej = j, xr = 0, xs = 0;
     bool ex_in = false;
-    const bool has_ex = w < 3;
-    if (w == 0) {
+    if (!EX) {
+    } else if (w == 0) {
         ej = j0 - 1;
         ex_in = i < n1 && ej >= 0;
     } else if (w == 1) {
@@ -518,12 +534,13 @@ __device__ __forceinline__ void march_tile_x2(const P& p, const TileGrid& g, lon
     auto zoff = [&](int zz) { return (cidx_t)min(max(zz, 0), n3 - 1) * s3; };
     auto ld_f = [&](int zz, int slot) {
         rc[slot] = p.load(cxy + zoff(zz));
-        if (has_ex) rx[slot] = p.load(exy + zoff(zz));
+        if (EX) rx[slot] = p.load(exy + zoff(zz));
     };
     auto ld_a = [&](int zz, int slot) { ra[slot] = p.load_aux(cxy + zoff(zz)); };
     auto put = [&](int buf, const double2& fcv, const double2& fe) {
         *reinterpret_cast<double2*>(&lds[buf][w + 1][2 + 2 * lane]) = fcv;
-        if (w == 0) {
+        if (!EX) {
+        } else if (w == 0) {
             *reinterpret_cast<double2*>(&lds[buf][0][2 + 2 * lane]) = fe;
         } else if (w == 1) {
             *reinterpret_cast<double2*>(&lds[buf][TYK + 1][2 + 2 * lane]) = fe;
@@ -576,8 +593,13 @@ __device__ __forceinline__ void run_tiles_x2(const P& p, const TileGrid& g, doub
                                              double (&acc)[P::ND], bool reverse = false) {
 #pragma unroll
     for (int d = 0; d < P::ND; d++) acc[d] = 0.0;
-    for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
-        march_tile_x2<P, ZC, TYK>(p, g, tile_of(g, t, reverse), lds, acc);
+    if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) < 3) {
+        for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+            march_tile_x2<P, ZC, TYK, true>(p, g, tile_of(g, t, reverse), lds, acc);
+    } else {
+        for (long long t = blockIdx.x; t < g.ntiles; t += gridDim.x)
+            march_tile_x2<P, ZC, TYK, false>(p, g, tile_of(g, t, reverse), lds, acc);
+    }
 }
 
 }  // namespace sedi
