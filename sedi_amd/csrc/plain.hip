@@ -39,6 +39,22 @@
 
 namespace sedi {
 
+// Store of a pass's output vector: non-temporal by default (at 256^3, three
+// A/B runs: cg_update_kernel 112-115 -> 106-108 us, cg_direction_kernel
+// 85-86 -> 83 us, bicgstab_half_kernel, which follows a storing pass,
+// 73 -> 68 us; pipelined CG unchanged), although the next pass reads it;
+// SEDI_PLAIN_NT=0 selects ordinary stores.
+#ifndef SEDI_PLAIN_NT
+#define SEDI_PLAIN_NT 1
+#endif
+__device__ __forceinline__ void stp(double* base, cidx_t i, double v) {
+#if SEDI_PLAIN_NT
+    stc_nt(base, i, v);
+#else
+    stc(base, i, v);
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // device-side scalar updates (run by thread 0 of a pass's last workgroup)
 // ---------------------------------------------------------------------------
@@ -131,8 +147,8 @@ struct PassResidual {
     __device__ double f(const Raw& r) const { return r.x; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& q, double, double Af, double (&acc)[ND]) const {
         const double rv = 1.0 * q.b + (-1.0) * Af;  // combine2(1.0, b, -1.0, r, r)
-        stc(r, i, rv);
-        if (rhat) stc(rhat, i, rv);
+        stp(r, i, rv);
+        if (rhat) stp(rhat, i, rv);
         acc[0] += q.b * q.b;
         acc[1] += rv * rv;
         acc[2] += rv * (rv * q.im);
@@ -160,7 +176,7 @@ struct PassCgDirection {
         return first ? u : 1.0 * u + beta * q.p;
     }
     __device__ void finish(cidx_t i, const Raw&, const Aux&, double p, double s, double (&acc)[ND]) const {
-        stc(pnew, i, p);
+        stp(pnew, i, p);
         acc[0] += s * p;
     }
 };
@@ -204,10 +220,10 @@ struct PassCgUpdate {
     __device__ void finish(cidx_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
         if (xstep) {
             const double xm = 1.0 * q.x + apend * q.pp;
-            stc(x, i, 1.0 * xm + alpha * pv);
+            stp(x, i, 1.0 * xm + alpha * pv);
         }
         const double rv = 1.0 * q.r + (-alpha) * s;
-        stc(r, i, rv);
+        stp(r, i, rv);
         const double u = rv * q.im;
         acc[0] += rv * u;
         acc[1] += rv * rv;
@@ -231,9 +247,9 @@ struct PassPcgInit {
     __device__ Aux load_aux(cidx_t) const { return {}; }
     __device__ double f(const Raw& c) const { return c.r * c.im; }
     __device__ void finish(cidx_t i, const Raw& c, const Aux&, double uv, double wv, double (&acc)[ND]) const {
-        stc(u, i, uv);
-        stc(w, i, wv);
-        stc(z, i, 0.0); stc(q, i, 0.0); stc(s, i, 0.0); stc(p, i, 0.0);
+        stp(u, i, uv);
+        stp(w, i, wv);
+        stp(z, i, 0.0); stp(q, i, 0.0); stp(s, i, 0.0); stp(p, i, 0.0);
         acc[0] += c.r * uv;
         acc[1] += wv * uv;
         acc[2] += c.r * c.r;
@@ -265,8 +281,8 @@ struct PassPcgIter {
         const double xv = 1.0 * o.x + alpha * pv;
         const double uv = 1.0 * o.u + (-alpha) * qv;
         const double wv = 1.0 * c.w + (-alpha) * zv;
-        stc(q, i, qv); stc(s, i, sv); stc(p, i, pv); stc(z, i, zv);
-        stc(r, i, rv); stc(x, i, xv); stc(u, i, uv); stc(wnew, i, wv);
+        stp(q, i, qv); stp(s, i, sv); stp(p, i, pv); stp(z, i, zv);
+        stp(r, i, rv); stp(x, i, xv); stp(u, i, uv); stp(wnew, i, wv);
         acc[0] += rv * uv;
         acc[1] += wv * uv;
         acc[2] += rv * rv;
@@ -293,8 +309,8 @@ struct PassBicgDirection {
     }
     __device__ double f(const Raw& q) const { return pdir(q) * q.im; }
     __device__ void finish(cidx_t i, const Raw& q, const Aux& o, double, double v, double (&acc)[ND]) const {
-        stc(pnew, i, pdir(q));
-        stc(vnew, i, v);
+        stp(pnew, i, pdir(q));
+        stp(vnew, i, v);
         acc[0] += v * o.rhat;
     }
 };
@@ -344,8 +360,8 @@ struct PassBicgUpdate {
         const double ph = o.p * q.im;
         const double xv = 1.0 * o.x + alpha * ph + omega * sh;  // combine3 (:217)
         const double rv = 1.0 * sv + (-omega) * t;
-        stc(x, i, xv);
-        stc(rnew, i, rv);
+        stp(x, i, xv);
+        stp(rnew, i, rv);
         acc[0] += rv * rv;
         acc[1] += rv * o.rhat;
     }
