@@ -9,8 +9,10 @@ properties that hold at any size.
   and final errors, D's sum, sum of squares, extremes and 4096 strided cells).
   The GPU's dot products sum in a different order than the reference's
   blocked sums, so over ~950 iterations the residual histories part at
-  rounding level; the bars are those of the 32^3 parity tests, widened by the
-  iteration count where it matters.
+  rounding level.  CG and pipelined CG stop at the reference's iteration
+  (+-1; D within 1e-10); BiCGStab, whose trajectory is chaotic in the
+  rounding, within twice how far the reference itself moves under seven
+  other summation orders (tests/golden/ref_spread_256cube.npz).
 * The mixed-precision estimator on the same probes: every probe converged and
   D within 1e-6 normwise of the reference's FP64 D.
 * 512^3, seed 23 (mixed pipelined CG and BiCGStab, replacement every 50
@@ -45,6 +47,8 @@ C4 = (512, 512, 512)
 @pytest.fixture(scope="module")
 def ref256():
     return load_npz("ref_256cube.npz")
+
+
 
 
 @pytest.mark.parametrize("side", [256, 512])
@@ -87,9 +91,17 @@ def test_plain_estimator_256cube_vs_reference(ref256, kind):
         assert np.all(np.abs(r["errs"][same] - ref256[f"s{kind}_errs"][same])
                       <= 1e-3 * ref256[f"s{kind}_errs"][same])
     else:
-        # BiCGStab's counts move by +-20% under rounding alone (as at 32^3)
-        assert np.all(np.abs(r["iters"] - ref_it) <= np.maximum(3, ref_it // 5)), (r["iters"], ref_it)
-        bar = 1e-6
+        # BiCGStab's trajectory is chaotic in the rounding: the reference's
+        # own counts and D move when only its summation order changes
+        # (ref_spread_256cube.npz: seven other orders beside this fixture's
+        # -p 2x2x2 run).  The GPU sums in yet another order; its bars are
+        # twice the reference's largest deviations (as for D at 32^3)
+        spread256 = load_npz("ref_spread_256cube.npz")
+        dit = int(np.max(np.abs(spread256["s1_iters"] - ref_it[None, :])))
+        dD = max(normwise(v, ref256["s1_Dsample"]) for v in spread256["s1_Dsample"])
+        print("reference spread: %d iterations, D %.3g normwise" % (dit, dD), flush=True)
+        assert np.all(np.abs(r["iters"] - ref_it) <= 2 * dit), (r["iters"], ref_it, dit)
+        bar = 2 * dD
     check_D(r["D"], ref256, kind, bar)
 
 
