@@ -70,7 +70,10 @@ int sedi_init(int device);
  * (Stencil.hpp:21-36) together with DistrAsyncStencil's
  * initiate/compute_local/finalize (DistrAsyncStencil.hpp:52-86) and the halo
  * exchange of CommHalo (CommHalo.hpp:84-181): on one GPU the grid is one
- * domain and its boundary is predicated in the kernel. */
+ * domain and its boundary is predicated in the kernel.  x and y may be host
+ * or device arrays of n1*n2*n3 doubles (i1 fastest); any 8-byte alignment
+ * works (16-byte aligned device arrays with n1 even take the faster march
+ * with two cells per lane).  y is bitwise the reference's. */
 int sedi_stencil_apply(const sedi_grid *grid, const sedi_stencil *stencil,
                        const double *x, double *y);
 
@@ -344,7 +347,12 @@ int sedi_graphs_enable(int on);
  * names another table, empty for none): the value stored for `key` and the
  * grid, or `fallback`.  Keys "zc:<kernel>" (planes per tile of a solver
  * kernel), "sched:<kernel>" (1 = tiles in launch order, 0 = XCD-contiguous
- * runs), "k:<solver>:<precision>" (probes per batch; nearest grid by cell
+ * runs, 2 = one workgroup per tile, 3/4 = launch order dealt to the XCDs in
+ * chunks of 4/8 tile rows), "pf:stencil_apply_kernel" (planes of loads in
+ * flight), "x2:stencil_apply_kernel" (1 = the march with two cells per lane,
+ * the default where n1 is even and x, y are 16-byte aligned),
+ * "wpc:<kernel>" (workgroups per CU; 0 = as occupancy allows),
+ * "k:<solver>:<precision>" (probes per batch; nearest grid by cell
  * count), "ty:batch_upd_kernel" and "ty:batch_dir_kernel" (rows per tile,
  * 8 or 16, of the FP32 batched CG update and direction passes).  No reference counterpart (its tiling is Platform.hpp's
  * compile-time constants). */
