@@ -7,7 +7,8 @@ per tile where a kernel has no partial sums, else as 1; 3/4: launch order with
 the tiles dealt to the XCDs in chunks of 4/8 tile rows), and the rows per
 tile of the FP32 batched CG update pass (8, or 16 by default from 256 rows up).
 The test grids have no entries of their own, so here a child process forces
-one plan on every kernel (SEDI_TUNING_FILE, read once per process; one plan
+one plan on every kernel (SEDI_TUNING_FILE, read once per process; two plans
+also cap the plain kernels at one workgroup per CU, "wpc:* 1"; one plan
 takes 16-row update tiles on the batched grid of 20 rows, a full tile and a
 partial one) and the results are checked
 against the reference fixtures with the bars of the parity suite: stencil
@@ -57,7 +58,8 @@ print(json.dumps(out))
 # z chunk, schedule, update-pass rows (0: default); schedules 3 and 4 deal
 # the tiles to the XCDs in chunks of 4 and 8 tile rows (2 is one workgroup per
 # tile, which only the stencil takes; elsewhere it acts as 1)
-PLANS = [(8, 1, 0), (32, 0, 16), (16, 1, 0), (16, 3, 0), (8, 4, 16), (32, 2, 0)]
+PLANS = [(8, 1, 0, 0), (32, 0, 16, 0), (16, 1, 0, 0), (16, 3, 0, 0), (8, 4, 16, 0), (32, 2, 0, 0), (16, 3, 0, 1),
+         (8, 0, 0, 1)]
 
 
 @pytest.fixture(scope="module")
@@ -68,16 +70,18 @@ def batched_default():
                              precision=sedi.FP32_MIXED, batch_k=8)["D"] for k in (1, 2, 3)}
 
 
-@pytest.mark.parametrize("zc,sched,ty", PLANS)
-def test_forced_launch_plan_keeps_parity(tmp_path, ref32, ref_spread, batched_default, zc, sched, ty):
+@pytest.mark.parametrize("zc,sched,ty,wpc", PLANS)
+def test_forced_launch_plan_keeps_parity(tmp_path, ref32, ref_spread, batched_default, zc, sched, ty, wpc):
     table = tmp_path / "tuning.txt"
-    # the stencil's loads in flight vary with the plan too (2, 4 or 8 planes)
+    # the stencil's loads in flight vary with the plan too (2, 4 or 8 planes);
+    # wpc 1 launches one workgroup per CU for every plain kernel (they loop
+    # over the tiles)
     pf = {2: 8, 4: 4}.get(sched, 2)
     table.write_text("zc:* 0 0 0 %d\nsched:* 0 0 0 %d\npf:stencil_apply_kernel 0 0 0 %d\n" % (zc, sched, pf) +
-                     ("ty:batch_upd_kernel 0 0 0 %d\n" % ty if ty else ""))
+                     ("ty:batch_upd_kernel 0 0 0 %d\n" % ty if ty else "") + ("wpc:* 0 0 0 %d\n" % wpc if wpc else ""))
     env = dict(os.environ, SEDI_TUNING_FILE=str(table))
     code = CHILD % dict(root=ROOT, tests=os.path.join(ROOT, "tests"), tmp=str(tmp_path))
-    print("plan zc=%d sched=%d ty=%d" % (zc, sched, ty), flush=True)
+    print("plan zc=%d sched=%d ty=%d wpc=%d" % (zc, sched, ty, wpc), flush=True)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
