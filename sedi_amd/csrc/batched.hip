@@ -806,7 +806,7 @@ static hipError_t launch_upd(const BRun& r, T a, T b, T cc, T im, const V4<T>* P
 
 // The direction pass with TYK-row tiles (8: the default in FP32 storage; 16
 // halves the y-halo re-reads of r and p, chosen by the tuning table's
-// "ty:batch_dir_kernel"; FP64 blocks keep 4 rows).
+// "ty:batch_dir_kernel"; FP64 blocks take SEDI_FP64_BATCH_TY rows).
 template <class T, int TYK>
 static hipError_t launch_dir(const BRun& r, T a, T b, T cc, T im, const V4<T>* R0, const V4<T>* R1, V4<T>* P0,
                              V4<T>* P1, V4<T>* XL) {
@@ -816,6 +816,14 @@ static hipError_t launch_dir(const BRun& r, T a, T b, T cc, T im, const V4<T>* R
                              : (g.tg.zc == 32 ? batch_dir_kernel<T, 32, TYK> : batch_dir_kernel<T, 16, TYK>);
     return launch("batch_dir_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, R0, R1, P0, P1, XL, im);
 }
+
+// Rows per tile of the FP64 batched CG passes: 8.  Their registers (about 180
+// VGPRs) allow two waves per SIMD, i.e. one 8-wave workgroup per CU or two
+// 4-wave ones; the taller tile halves the y-halo re-reads (update pass 7.47 ->
+// 6.83 ms at 256^3, k = 64; the direction pass unchanged).
+#ifndef SEDI_FP64_BATCH_TY
+#define SEDI_FP64_BATCH_TY TY_TALL
+#endif
 
 static int dir_rows(const Shape& s) {
     return tuned_value("ty:batch_dir_kernel", s, 0) == 2 * TY_TALL ? 2 * TY_TALL : TY_TALL;
@@ -846,7 +854,7 @@ static int cg_iteration(const BRun& r) {
         else
             SEDI_HIP((launch_dir<T, TY_TALL>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
     } else {
-        SEDI_HIP((launch_dir<T, TY>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+        SEDI_HIP((launch_dir<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
     }
     if constexpr (sizeof(T) == 4) {
         if (upd_rows(r.shape) == 2 * TY_TALL)
@@ -854,7 +862,7 @@ static int cg_iteration(const BRun& r) {
         else
             SEDI_HIP((launch_upd<T, TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
     } else {
-        SEDI_HIP((launch_upd<T, TY>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
+        SEDI_HIP((launch_upd<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
     }
     if (r.mixed) return replace_and_fold<T>(r, R0, nullptr, true);
     return SEDI_OK;

@@ -229,7 +229,7 @@ inline unsigned ew_blocks(long long n) {
 // Tile heights of the batched CG passes: the direction pass takes 8 rows in
 // FP32 storage (16 measured no faster); the update pass 8 or 16 rows by grid
 // (batched.hip, upd_rows).  FP64 blocks, whose registers do not fit two
-// 8-wave workgroups per CU, keep 4 rows.
+// 8-wave workgroups per CU, take one (SEDI_FP64_BATCH_TY, batched.hip).
 template <class T>
 struct CgTile {
     static constexpr int min_waves = sizeof(T) == 4 ? 4 : 1;
