@@ -334,3 +334,23 @@ def test_reference_full_size_order_stability():
     e100 = np.abs(extra["s1_fixed100_p118_errs"] / extra["s1_fixed100_errs"] - 1).max()
     assert e20 < 1e-10 and e100 > 0.1
     assert normwise(extra["s1_fixed20_p118_Dsample"], extra["s1_fixed20_Dsample"]) < 1e-11
+
+
+def test_reference_bicgstab_spread_256cube():
+    """The evidence behind the GPU's BiCGStab bars at 256^3: the reference's
+    own BiCGStab (seed 7, 16 probes, tol 1e-9) under other summation orders
+    (tests/golden/ref_spread_256cube.npz: -p 1x1x8, 1x8x1, 8x1x1, 1x2x4,
+    4x2x1 with 2-lane SIMD and -p 2x2x2 with 4- and 8-lane SIMD) against its
+    -p 2x2x2 2-lane run (ref_256cube.npz): every variant converges, their
+    counts move by up to a hundred-odd iterations on some probe (but not by
+    half the count anywhere), and D moves at the level the tolerance leaves
+    (between 1e-9 and 1e-6 normwise on the 4096 strided cells)."""
+    from conftest import load_npz
+    big, sp = load_npz("ref_256cube.npz"), load_npz("ref_spread_256cube.npz")
+    assert np.array_equal(sp["sample_idx"], np.arange(0, 256 ** 3, 256 ** 3 // 4096)[:4096])
+    its = sp["s1_iters"]
+    assert its.shape == (len(sp["variants"]), 16) and len(sp["variants"]) >= 1
+    dev = np.abs(its - big["s1_iters"][None, :])
+    assert 50 <= dev.max() < big["s1_iters"].min() // 2
+    ddist = [normwise(v, big["s1_Dsample"]) for v in sp["s1_Dsample"]]
+    assert all(1e-9 < d < 1e-6 for d in ddist), ddist
