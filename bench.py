@@ -74,17 +74,22 @@ CMP_SAMPLES = 16
 # (CG: see upd_values)
 DOMINANT = {2: ("cg_update_kernel", None), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
 # batched mixed CG: batch_upd_kernel streams p and r in and r out every
-# iteration, and every second (even) iteration also the previous direction and
-# x_lo in and x_lo out (the two iterations' solution updates are applied
-# together): 3 or 6 FP32 values per right-hand side and cell
+# iteration, and at every fourth iteration also x_lo in and out and the three
+# previous directions in (the solution updates of four iterations are applied
+# together; the library's default -DSEDI_CG_XGROUP=4): 3 or 8 FP32 values per
+# right-hand side and cell.  The plain solver's cg_update_kernel applies them
+# in pairs: 3 or 6 FP64 values.
 BATCH_KERNEL = "batch_upd_kernel"
+BATCH_CG_GROUP = 4
+PLAIN_CG_GROUP = 2
 
 
-def upd_values(iters):
-    """Vectors the CG update pass (batch_upd_kernel, cg_update_kernel) streams
-    for a right-hand side that ran `iters` iterations: 3 per odd iteration, 6
-    per even one."""
-    return 3 * ((iters + 1) // 2) + 6 * (iters // 2)
+def upd_values(iters, group):
+    """Vectors the CG update pass streams for a right-hand side that ran
+    `iters` iterations with solution updates in groups of `group`: 3 per
+    iteration, plus x in and out and the group's other directions in at every
+    group's last iteration."""
+    return 3 * iters + (2 + group - 1) * (iters // group)
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sedi_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
 
@@ -420,10 +425,9 @@ def run_plain(args):
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
     # algorithmic bytes of the iterations the probes ran (launches queued after
     # a solve converged return at once and move nothing); plain CG pairs its
-    # solution updates like the batched one: 3 vectors at odd iterations, 6
-    # at even ones
+    # solution updates: 3 vectors at odd iterations, 6 at even ones
     if kind == 2:
-        vecs = sum(upd_values(int(m)) for m in iters)
+        vecs = sum(upd_values(int(m), PLAIN_CG_GROUP) for m in iters)
     else:
         vecs = int(iters.sum()) * nvec
     line["roofline"] = roofline(kname, launches, kms, vecs * n * 8)
@@ -465,7 +469,7 @@ def run_headline(args):
     # algorithmic bytes: each right-hand side is credited with its own
     # iterations only (a converged one that still rides in an active block of
     # 4 is not counted)
-    rhs_values = int(sum(upd_values(int(m)) for m in iters))
+    rhs_values = int(sum(upd_values(int(m), BATCH_CG_GROUP) for m in iters))
     config = {"workload": "batched mixed-precision CG estimator, 256^3 grid, 64 probes (seed 11) "
                           "as one block of k=%d, FP32 vectors / FP64 reductions, residual "
                           "replacement every 50, rel. residual 1e-8, maxiter 2000" % K,
@@ -599,6 +603,7 @@ def run_ksweep(args):
 
 
 ARITH_LIB = os.path.join(ROOT, "sedi_amd", "variants", "libsedi_amd_f32compute.so")
+TUNING_TABLE = os.path.join(ROOT, "sedi_amd", "tuning.txt")
 
 
 def run_arith(args):
@@ -623,7 +628,9 @@ def run_arith(args):
         name = SOLVER_LABEL[kind]
         with tempfile.TemporaryDirectory() as tmp:
             dpath = os.path.join(tmp, "D.npy")
-            env = dict(os.environ, SEDI_AMD_LIB=ARITH_LIB)
+            # the variant lives in its own folder: point it at the package's
+            # tuning table, or it would run the untuned default plans
+            env = dict(os.environ, SEDI_AMD_LIB=ARITH_LIB, SEDI_TUNING_FILE=TUNING_TABLE)
             res = subprocess.run([sys.executable, os.path.abspath(__file__), "--mode", "arith", "--arith-child",
                                   str(kind), "--arith-out", dpath, "--arith-samples", str(ns)],
                                  env=env, stdout=subprocess.PIPE, check=True)

@@ -32,17 +32,18 @@
 // only when its true residual meets the tolerance.  With FP64 storage the
 // recurrences are the plain solvers'.
 //
-// Solution updates of CG come in pairs: an odd iteration leaves x alone and
-// records its alpha as pending (apend); the next even iteration adds
-// apend * p_prev + alpha * p in one read-modify-write of x, reading p_prev
-// from the other direction buffer, which it still holds.  That saves one x
-// read and one x write every other iteration (7.5 instead of 8 vector streams
-// per iteration).  Whatever reads x while an update is pending adds it: the
-// FP64 replacement (before its stencil) and fold, the accumulation and the
-// unpacking of x, and -- for a right-hand side that finished at an odd
-// iteration while its block goes on -- the next direction pass, which reads
-// that p as its old direction.  The pending p is always in the second
-// direction buffer, which odd iterations write.
+// Solution updates of CG come in groups of CG_XG (4; 2 when built with
+// -DSEDI_CG_XGROUP=2): an iteration that is not a multiple of CG_XG leaves x
+// alone and records its alpha as pending (apend); the group's last iteration
+// adds the pending apend_j * p_j and its own alpha * p in one read-modify-write
+// of x, in the order of the iterations, reading the pending directions from
+// the other direction buffers, which still hold them.  Per iteration that is
+// 6 + (2 + CG_XG - 1) / CG_XG vector streams instead of 8: 7.25 with groups
+// of 4, 7.5 with pairs.  Whatever reads x while updates are
+// pending adds them: the FP64 replacement (before its stencil) and fold, the
+// accumulation and the unpacking of x, and -- for a right-hand side that
+// finished inside a group while its block goes on -- the next direction pass.
+// The pending directions are always in buffers p1 .. p_{CG_XG-1}.
 //
 // Per-RHS state makes each probe's trajectory independent of the others in
 // its batch: an RHS that is done keeps x and r (alpha = beta = 0),
@@ -65,7 +66,7 @@ void set_batch_compact_enabled(bool on) { g_compact.store(on); }
 enum {
     CD_BB, CD_RU, CD_RUOLD, CD_ALPHA, CD_BETA, CD_ERR, CD_FERR, CD_ELAST,
     CD_GAMMA, CD_DELTA, CD_ERRP, CD_GAMMAP, CD_ALPHAP, CD_RHO, CD_RHOP, CD_OMEGA, CD_SS, CD_CBW, CD_TOL,
-    CD_APEND, CD_NDOUBLE
+    CD_APEND, CD_APEND2, CD_APEND3, CD_NDOUBLE
 };
 enum { CI_FITER, CI_CODE, CI_DONE, CI_REP, CI_FOLD, CI_SEXIT, CI_MAXIT, CI_PLANE, CI_NINT };
 
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
         c.ru_old[k] = acc[4 + q];
         c.alpha[k] = 0.0;
         c.beta[k] = 0.0;
-        c.apend[k] = 0.0;
+        for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + k] = 0.0;
         c.rep[k] = 0;
         c.fold[k] = 0;
         c.sexit[k] = 0;
@@ -146,6 +147,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_init_kernel(long long n, BCtl 
 }
 
 // ---- CG direction: p = u + beta p, s = A p, <s, p> (Cg.hpp:152-166,175,188) --
+// A lane that finished while its block goes on still has its last updates
+// pending: this pass adds them to x (flush, the lanes' pending coefficients,
+// reading p1 .. p_{CG_NPEND} at the cell; only when some lane needs it).
 template <class T>
 struct PassBDir {
     using V = V4<T>;
@@ -157,7 +161,8 @@ struct PassBDir {
     V* x;
     T im;
     V beta;
-    V flush;  // pending coefficients of lanes that finished (else 0)
+    V flush[CG_NPEND];  // pending coefficients of lanes that finished (else 0)
+    const V* pend[CG_NPEND];
     bool first, any_flush;
     struct Raw { V r, p; };
     struct Aux {};
@@ -169,7 +174,12 @@ struct PassBDir {
     }
     __device__ void finish(cidx_t i, const Raw& q, const Aux&, const V& p, const V& s,
                            double (&acc)[ND]) const {
-        if (any_flush) stv(x, i, ldc((const V*)x, i) + flush * q.p);
+        if (any_flush) {
+            V xv = ldc((const V*)x, i);
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) xv = xv + flush[j] * ldc(pend[j], i);
+            stv(x, i, xv);
+        }
         stv(pnew, i, p);
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[q] += (double)s[q] * (double)p[q];
@@ -178,7 +188,7 @@ struct PassBDir {
 
 template <class T, int ZC, int TYK>
 __global__ __launch_bounds__(TX * TYK, CgDirTile<T>::min_waves) void batch_dir_kernel(
-    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r0, const V4<T>* r1, V4<T>* p0, V4<T>* p1,
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* r0, const V4<T>* r1, CgDirs<T> P4,
     V4<T>* x, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
@@ -187,19 +197,26 @@ __global__ __launch_bounds__(TX * TYK, CgDirTile<T>::min_waves) void batch_dir_k
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
-    V4<T> beta, flush;
+    V4<T> beta, flush[CG_NPEND];
     bool any_flush = false;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const bool d = c.done[4 * rb + q];
         beta[q] = d ? (T)0 : (T)c.beta[4 * rb + q];
-        // a lane that finished at an odd iteration: its last update is
-        // pending on p_old, which this pass reads
-        flush[q] = d ? (T)c.apend[4 * rb + q] : (T)0;
-        any_flush |= flush[q] != (T)0;
+        // a lane that finished with updates pending
+#pragma unroll
+        for (int j = 0; j < CG_NPEND; j++) {
+            flush[j][q] = d ? (T)c.apend[j * c.K + 4 * rb + q] : (T)0;
+            any_flush |= flush[j][q] != (T)0;
+        }
     }
-    PassBDir<T> P{a, b, cc, ((it & 1) ? r1 : r0) + off, ((it & 1) ? p0 : p1) + off, ((it & 1) ? p1 : p0) + off,
-                  x + off, im, beta, flush, it == 1, any_flush};
+    PassBDir<T> P{a, b, cc, ((it & 1) ? r1 : r0) + off, P4.at((it + CG_XG - 1) % CG_XG) + off,
+                  P4.at(it % CG_XG) + off, x + off, im, beta, {}, {}, it == 1, any_flush};
+#pragma unroll
+    for (int j = 0; j < CG_NPEND; j++) {
+        P.flush[j] = flush[j];
+        P.pend[j] = P4.p[j + 1] + off;
+    }
     double acc[4];
     run_tiles<PassBDir<T>, ZC, TYK>(P, g, lds, acc, false);
     if (!grid_sum<4, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
@@ -208,33 +225,37 @@ __global__ __launch_bounds__(TX * TYK, CgDirTile<T>::min_waves) void batch_dir_k
     for (int q = 0; q < 4; q++) {
         const int k = 4 * rb + q;
         c.alpha[k] = c.done[k] ? 0.0 : c.ru[k] / acc[q];  // Cg.hpp:166
-        if (c.done[k]) c.apend[k] = 0.0;
+        if (c.done[k])
+            for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + k] = 0.0;
     }
 }
 
 // ---- CG update: x += alpha p, r -= alpha s, <r,u>, <r,r> (Cg.hpp:168-193) ---
-template <class T>
+// XS: the group's last iteration, which updates x (a compile-time choice: the
+// other iterations keep no registers for x and the pending directions)
+template <class T, bool XS>
 struct PassBUpd {
     using V = V4<T>;
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
     T a, b, c;
     const V* p;
-    const V* pprev;  // the previous iteration's direction (even iterations)
+    const V* pend[CG_NPEND];  // the pending directions p1 .. (x steps)
     V* xlo;
     const V* r;
     V* rnext;  // the next iteration's residual (the other buffer: stores stream
                // to fresh lines instead of back into the lines being read)
     T im;
-    V alpha, apend;
-    bool xstep;  // even iteration: x += apend p_prev + alpha p
+    V alpha, apend[CG_NPEND];
+    static constexpr bool xstep = XS;  // it % CG_XG == 0: x += apend_1 p_1 + ... + alpha p
     struct Raw { V p; };
-    struct Aux { V x, pp, r; };
+    struct Aux { V x, pp[CG_NPEND], r; };
     __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
     __device__ Aux load_aux(cidx_t i) const {
         Aux o;
         if (xstep) {
             o.x = ldc((const V*)xlo, i);
-            o.pp = ldc(pprev, i);
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) o.pp[j] = ldc(pend[j], i);
         }
         o.r = ldc(r, i);
         return o;
@@ -242,7 +263,12 @@ struct PassBUpd {
     __device__ V f(const Raw& q) const { return q.p; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V& pv, const V& s,
                            double (&acc)[ND]) const {
-        if (xstep) stv(xlo, i, (o.x + apend * o.pp) + alpha * pv);
+        if (xstep) {
+            V xv = o.x;
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) xv = xv + apend[j] * o.pp[j];
+            stv(xlo, i, xv + alpha * pv);
+        }
         const V rn = o.r - alpha * s;
         stv(rnext, i, rn);
         const V u = im * rn;
@@ -254,10 +280,32 @@ struct PassBUpd {
     }
 };
 
+template <class T, int ZC, int TYK, bool XS>
+__device__ __forceinline__ void run_upd(const TileGrid& g, const BCtl& c, T a, T b, T cc, long long off, int it,
+                                        const CgDirs<T>& P4, V4<T>* xlo, V4<T>* r0, V4<T>* r1, T im,
+                                        V4<T> (&lds)[2][TYK + 2][TX + 2], double (&acc)[8]) {
+    const int rb = blockIdx.y;
+    PassBUpd<T, XS> P;
+    P.a = a; P.b = b; P.c = cc;
+    P.p = P4.at(it % CG_XG) + off;
+#pragma unroll
+    for (int q = 0; q < 4; q++) P.alpha[q] = uniform((T)c.alpha[4 * rb + q]);
+#pragma unroll
+    for (int j = 0; j < CG_NPEND; j++) {
+        P.pend[j] = P4.p[j + 1] + off;
+#pragma unroll
+        for (int q = 0; q < 4; q++) P.apend[j][q] = XS ? uniform((T)c.apend[j * c.K + 4 * rb + q]) : (T)0;
+    }
+    P.xlo = xlo + off;
+    P.r = ((it & 1) ? r1 : r0) + off;
+    P.rnext = ((it & 1) ? r0 : r1) + off;
+    P.im = im;
+    run_tiles<PassBUpd<T, XS>, ZC, TYK>(P, g, lds, acc, true);
+}
+
 template <class T, int ZC, int TYK>
 __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kernel(
-    TileGrid g, BCtl c, T a, T b, T cc, long long n, const V4<T>* p0, const V4<T>* p1, V4<T>* xlo,
-    V4<T>* r0, V4<T>* r1, T im) {
+    TileGrid g, BCtl c, T a, T b, T cc, long long n, CgDirs<T> P4, V4<T>* xlo, V4<T>* r0, V4<T>* r1, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     __shared__ V4<T> lds[2][TYK + 2][TX + 2];
@@ -265,25 +313,21 @@ __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kern
     __shared__ int lastflag;
     const int it = c.iter[rb];
     const long long off = (long long)rb * n;
-    V4<T> alpha, apend;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        alpha[q] = (T)c.alpha[4 * rb + q];
-        apend[q] = (T)c.apend[4 * rb + q];
-    }
-    const bool xstep = (it & 1) == 0;
-    PassBUpd<T> P{a, b, cc, ((it & 1) ? p1 : p0) + off, ((it & 1) ? p0 : p1) + off, xlo + off,
-                  ((it & 1) ? r1 : r0) + off, ((it & 1) ? r0 : r1) + off, im, alpha, apend, xstep};
+    const int jp = it % CG_XG;
     double acc[8];
-    run_tiles<PassBUpd<T>, ZC, TYK>(P, g, lds, acc, true);
+    if (jp == 0) run_upd<T, ZC, TYK, true>(g, c, a, b, cc, off, it, P4, xlo, r0, r1, im, lds, acc);
+    else run_upd<T, ZC, TYK, false>(g, c, a, b, cc, off, it, P4, xlo, r0, r1, im, lds, acc);
     if (!grid_sum<8, TYK>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
     for (int q = 0; q < 4; q++) {
         const int k = 4 * rb + q;
-        // odd iteration: this alpha (0 for a finished lane) waits for the next
-        // even one; even: both updates are in x
-        c.apend[k] = xstep ? 0.0 : c.alpha[k];
+        // this alpha (0 for a finished lane) waits for the group's last
+        // iteration; at that one every update of the group is in x
+        if (jp == 0)
+            for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + k] = 0.0;
+        else
+            c.apend[(jp - 1) * c.K + k] = c.alpha[k];
         if (c.done[k]) continue;
         const double ru_new = acc[q];
         c.ru_old[k] = c.ru[k];
@@ -312,7 +356,7 @@ __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kern
 
 // ---- FP64 replacement: r = b - A (x_hi + x_lo) -------------------------------
 // The new residual is computed in FP64 from the FP64 sum of the two parts of
-// x (plus CG's pending update, apend * p) and stored rounded to the vector
+// x (plus CG's pending updates, apend_j * p_j) and stored rounded to the vector
 // type.  x itself is read at the halo by
 // neighbouring tiles during this pass, so the fold x_hi += x_lo, x_lo = 0 is
 // left to batch_fold_kernel, which runs after it.  The second dot is <r, u>
@@ -327,19 +371,23 @@ struct PassBRep {
     V4<T>* r;
     const V4<T>* rhs;
     V4<T>* rhat;
-    const V4<T>* pend;  // CG: the direction with a pending update (null: none)
-    V apend;
+    const V4<T>* pend[CG_NPEND];  // CG: the directions with pending updates (null: none)
+    V apend[CG_NPEND];
+    bool pending;
     T im;
     int rep[4];
     bool reset;  // BiCGStab: r^ restarts from the replaced residual
-    struct Raw { V xh; V4<T> xl, pp; };
+    struct Raw { V xh; V4<T> xl, pp[CG_NPEND]; };
     struct Aux { V4<T> b, r, rh; };
     __device__ Raw load(cidx_t i) const {
         Raw q;
 #pragma unroll
         for (int k = 0; k < 4; k++) q.xh[k] = ldc(xh[k], i);
         q.xl = ldc(xlo, i);
-        if (pend) q.pp = ldc(pend, i);
+        if (pending) {
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) q.pp[j] = ldc(pend[j], i);
+        }
         return q;
     }
     __device__ Aux load_aux(cidx_t i) const {
@@ -353,9 +401,11 @@ struct PassBRep {
         V x;
 #pragma unroll
         for (int k = 0; k < 4; k++) x[k] = q.xh[k] + (double)q.xl[k];
-        if (pend) {
+        if (pending) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) x[k] = x[k] + apend[k] * (double)q.pp[k];
+            for (int j = 0; j < CG_NPEND; j++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) x[k] = x[k] + apend[j][k] * (double)q.pp[j][k];
         }
         return x;
     }
@@ -386,7 +436,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
                                                              double cc, long long n, const double* xhi,
                                                              const V4<T>* xlo, V4<T>* r0, V4<T>* r1,
                                                              const V4<T>* rhs, V4<T>* rhat,
-                                                             const V4<T>* pend, T im) {
+                                                             PendDirs<T> pend, T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     const int* rp = c.rep + 4 * rb;
@@ -411,11 +461,15 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     P.rhat = RHAT ? rhat + off : nullptr;
     bool pending = false;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        P.apend[k] = pend ? c.apend[4 * rb + k] : 0.0;
-        pending |= P.apend[k] != 0.0;
+    for (int j = 0; j < CG_NPEND; j++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            P.apend[j][k] = pend.p[j] ? c.apend[j * c.K + 4 * rb + k] : 0.0;
+            pending |= P.apend[j][k] != 0.0;
+        }
+        P.pend[j] = pend.p[j] ? pend.p[j] + off : nullptr;
     }
-    P.pend = pending ? pend + off : nullptr;
+    P.pending = pending;
     P.im = im;
     P.reset = c.bicg_reset != 0;
     double acc[8];
@@ -462,7 +516,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
 // sum, which the final x = x_hi + x_lo would make anyway.
 template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl c, double* xhi,
-                                                              V4<T>* xlo, const V4<T>* pend) {
+                                                              V4<T>* xlo, PendDirs<T> pend) {
     const int rb = blockIdx.y;
     int* fp = c.fold + 4 * rb;
     const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
@@ -470,27 +524,34 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     __shared__ double red[NWAVES];
     __shared__ int lastflag;
     const int fl[4] = {f0, f1, f2, f3};
-    double ap[4];
-    bool pending = false;
+    double ap[CG_NPEND][4];
+    const V4<T>* pq[CG_NPEND];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        ap[k] = (pend && fl[k]) ? c.apend[4 * rb + k] : 0.0;
-        pending |= ap[k] != 0.0;
+    for (int j = 0; j < CG_NPEND; j++) {
+        bool pending = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            ap[j][k] = (pend.p[j] && fl[k]) ? c.apend[j * c.K + 4 * rb + k] : 0.0;
+            pending |= ap[j][k] != 0.0;
+        }
+        pq[j] = pending ? pend.p[j] + (long long)rb * n : nullptr;
     }
     V4<T>* xq = xlo + (long long)rb * n;
-    const V4<T>* pq = pending ? pend + (long long)rb * n : nullptr;
     for (long long i = (long long)blockIdx.x * NTHREADS + threadIdx.x; i < n;
          i += (long long)gridDim.x * NTHREADS) {
         V4<T> xl = xq[i];
-        V4<T> pp{};
-        if (pq) pp = pq[i];
+        V4<T> pp[CG_NPEND];
+#pragma unroll
+        for (int j = 0; j < CG_NPEND; j++) pp[j] = pq[j] ? pq[j][i] : V4<T>{};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (fl[k]) {
                 double* h = xhi + (long long)(4 * rb + k) * n + i;
-                // x_hi + x_lo (+ CG's pending apend * p), as the replacement used
+                // x_hi + x_lo (+ CG's pending apend_j * p_j), as the replacement used
                 double v = *h + (double)xl[k];
-                if (pq) v = v + ap[k] * (double)pp[k];
+#pragma unroll
+                for (int j = 0; j < CG_NPEND; j++)
+                    if (pq[j]) v = v + ap[j][k] * (double)pp[j][k];
                 *h = v;
                 xl[k] = (T)0;
             }
@@ -502,7 +563,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
         threadIdx.x != 0)
         return;
     for (int k = 0; k < 4; k++)
-        if (fp[k] && pend) c.apend[4 * rb + k] = 0.0;
+        if (fp[k] && pend.p[0])
+            for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + 4 * rb + k] = 0.0;
     fp[0] = fp[1] = fp[2] = fp[3] = 0;
 }
 
@@ -680,7 +742,7 @@ struct LaneList {
 
 template <class T>
 __global__ void batch_accum_kernel(long long n, LaneList lanes, const V4<T>* xlo, const double* xhi,
-                                   const V4<T>* b, const V4<T>* pend, const double* apend, double* t,
+                                   const V4<T>* b, PendDirs<T> pend, const double* apend, int K, double* t,
                                    double* q) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -689,7 +751,10 @@ __global__ void batch_accum_kernel(long long n, LaneList lanes, const V4<T>* xlo
             const int k = lanes.lane[j];
             const long long cell = (long long)(k >> 2) * n + i;
             double x = (xhi ? xhi[(long long)k * n + i] : 0.0) + (double)xlo[cell][k & 3];
-            if (pend && apend[k] != 0.0) x = x + apend[k] * (double)pend[cell][k & 3];
+#pragma unroll
+            for (int jp = 0; jp < CG_NPEND; jp++)
+                if (pend.p[jp] && apend[jp * K + k] != 0.0)
+                    x = x + apend[jp * K + k] * (double)pend.p[jp][cell][k & 3];
             const double v = (double)b[cell][k & 3];
             tv = tv + x * v;  // stochastic-estimator.cc:195
             qv = qv + v * v;  // :198
@@ -719,11 +784,12 @@ __global__ void batch_pack_kernel(long long n, int kact, long long row_stride, c
 
 // Rows k0 .. k1-1 of the batch (x = x_hi + x_lo, or the right-hand sides when
 // xhi is null and `src` is b) into [k1-k0][n] FP64.  Grid y = RHS blocks.
-// (plus CG's pending apend * p for x: pend non-null, apend indexed like xhi)
+// (plus CG's pending apend_j * p_j for x: pend.p[j] non-null, offset like src;
+// apend indexed like xhi, with stride K between the pending directions)
 template <class T>
 __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __restrict__ src,
-                                    const double* __restrict__ xhi, const V4<T>* __restrict__ pend,
-                                    const double* __restrict__ apend, double* __restrict__ x) {
+                                    const double* __restrict__ xhi, PendDirs<T> pend,
+                                    const double* __restrict__ apend, int K, double* __restrict__ x) {
     const int rb = blockIdx.y;
     const V4<T>* s = src + (long long)rb * n;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -734,7 +800,10 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
             const long long k = 4 * rb + q;
             if (k >= k0 && k < k1) {
                 double xv = (xhi ? xhi[k * n + i] : 0.0) + (double)v[q];
-                if (pend && apend[k] != 0.0) xv = xv + apend[k] * (double)pend[(long long)rb * n + i][q];
+#pragma unroll
+                for (int jp = 0; jp < CG_NPEND; jp++)
+                    if (pend.p[jp] && apend[jp * K + k] != 0.0)
+                        xv = xv + apend[jp * K + k] * (double)pend.p[jp][(long long)rb * n + i][q];
                 x[(k - k0) * n + i] = xv;
             }
         }
@@ -744,11 +813,21 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// The direction buffer that holds CG's pending update (odd iterations write
-// the second one); null for the other solvers, whose x is always current.
+// The direction buffers that hold CG's pending updates (p1 .. p_{CG_NPEND});
+// null for the other solvers, whose x is always current.
 template <class T>
-static const V4<T>* pending_dir(const BRun& r) {
-    return r.c.kind == SEDI_CG ? static_cast<const V4<T>*>(r.v[CG_P1]) : nullptr;
+static PendDirs<T> pending_dirs(const BRun& r) {
+    PendDirs<T> d;
+    for (int j = 0; j < CG_NPEND; j++)
+        d.p[j] = r.c.kind == SEDI_CG ? static_cast<const V4<T>*>(r.v[cg_pslot(j + 1)]) : nullptr;
+    return d;
+}
+
+template <class T>
+static CgDirs<T> cg_dirs(const BRun& r) {
+    CgDirs<T> d;
+    for (int j = 0; j < 4; j++) d.p[j] = j < CG_XG ? static_cast<V4<T>*>(r.v[cg_pslot(j)]) : nullptr;
+    return d;
 }
 
 template <class T, bool RHAT>
@@ -759,7 +838,7 @@ static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
     const double* xhi = r.xhi;
     const V4<T>* xlo = static_cast<const V4<T>*>(r.v[SLOT_XLO]);
     const V4<T>* b = static_cast<const V4<T>*>(r.v[SLOT_B]);
-    const V4<T>* pend = pending_dir<T>(r);
+    const PendDirs<T> pend = pending_dirs<T>(r);
     if (g.tg.zc == 8)
         return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 8>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
                       r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
@@ -773,7 +852,7 @@ static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
 template <class T>
 int fold_only(const BRun& r) {
     SEDI_HIP(launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS),
-                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dir<T>(r)));
+                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dirs<T>(r)));
     return SEDI_OK;
 }
 
@@ -794,27 +873,25 @@ int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
 // The update pass with TYK-row tiles: its geometry and a launch of the
 // kernel for the geometry's z chunk.
 template <class T, int TYK>
-static hipError_t launch_upd(const BRun& r, T a, T b, T cc, T im, const V4<T>* P0, const V4<T>* P1, V4<T>* XL,
-                             V4<T>* R0, V4<T>* R1) {
+static hipError_t launch_upd(const BRun& r, T a, T b, T cc, T im, CgDirs<T> P4, V4<T>* XL, V4<T>* R0, V4<T>* R1) {
     const KGeom g = bgeom("batch_upd_kernel", batch_upd_kernel<T, 16, TYK>, r.shape, SEDI_PREFETCH_BATCH, TYK);
     const dim3 grid(g.nblocks, r.nrb), block(TX * TYK);
     auto kern = g.tg.zc == 8 ? batch_upd_kernel<T, 8, TYK>
                              : (g.tg.zc == 32 ? batch_upd_kernel<T, 32, TYK> : batch_upd_kernel<T, 16, TYK>);
-    return launch("batch_upd_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, P0, P1, XL, R0, R1,
-                  im);
+    return launch("batch_upd_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, P4, XL, R0, R1, im);
 }
 
 // The direction pass with TYK-row tiles (8: the default in FP32 storage; 16
 // halves the y-halo re-reads of r and p, chosen by the tuning table's
 // "ty:batch_dir_kernel"; FP64 blocks take SEDI_FP64_BATCH_TY rows).
 template <class T, int TYK>
-static hipError_t launch_dir(const BRun& r, T a, T b, T cc, T im, const V4<T>* R0, const V4<T>* R1, V4<T>* P0,
-                             V4<T>* P1, V4<T>* XL) {
+static hipError_t launch_dir(const BRun& r, T a, T b, T cc, T im, const V4<T>* R0, const V4<T>* R1, CgDirs<T> P4,
+                             V4<T>* XL) {
     const KGeom g = bgeom("batch_dir_kernel", batch_dir_kernel<T, 16, TYK>, r.shape, SEDI_PREFETCH_BATCH, TYK);
     const dim3 grid(g.nblocks, r.nrb), block(TX * TYK);
     auto kern = g.tg.zc == 8 ? batch_dir_kernel<T, 8, TYK>
                              : (g.tg.zc == 32 ? batch_dir_kernel<T, 32, TYK> : batch_dir_kernel<T, 16, TYK>);
-    return launch("batch_dir_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, R0, R1, P0, P1, XL, im);
+    return launch("batch_dir_kernel", kern, grid, block, r.stream, g.tg, r.c, a, b, cc, r.n, R0, R1, P4, XL, im);
 }
 
 // Rows per tile of the FP64 batched CG passes: 8.  Their registers (about 180
@@ -845,24 +922,23 @@ static int cg_iteration(const BRun& r) {
     const T a = (T)r.a, b = (T)r.b, cc = (T)r.cc, im = (T)r.im;
     V4<T>* R0 = static_cast<V4<T>*>(r.v[CG_R]);
     V4<T>* R1 = static_cast<V4<T>*>(r.v[CG_R1]);
-    V4<T>* P0 = static_cast<V4<T>*>(r.v[CG_P0]);
-    V4<T>* P1 = static_cast<V4<T>*>(r.v[CG_P1]);
+    const CgDirs<T> P4 = cg_dirs<T>(r);
     V4<T>* XL = static_cast<V4<T>*>(r.v[SLOT_XLO]);
     if constexpr (sizeof(T) == 4) {
         if (dir_rows(r.shape) == 2 * TY_TALL)
-            SEDI_HIP((launch_dir<T, 2 * TY_TALL>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+            SEDI_HIP((launch_dir<T, 2 * TY_TALL>(r, a, b, cc, im, R0, R1, P4, XL)));
         else
-            SEDI_HIP((launch_dir<T, TY_TALL>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+            SEDI_HIP((launch_dir<T, TY_TALL>(r, a, b, cc, im, R0, R1, P4, XL)));
     } else {
-        SEDI_HIP((launch_dir<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, R0, R1, P0, P1, XL)));
+        SEDI_HIP((launch_dir<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, R0, R1, P4, XL)));
     }
     if constexpr (sizeof(T) == 4) {
         if (upd_rows(r.shape) == 2 * TY_TALL)
-            SEDI_HIP((launch_upd<T, 2 * TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
+            SEDI_HIP((launch_upd<T, 2 * TY_TALL>(r, a, b, cc, im, P4, XL, R0, R1)));
         else
-            SEDI_HIP((launch_upd<T, TY_TALL>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
+            SEDI_HIP((launch_upd<T, TY_TALL>(r, a, b, cc, im, P4, XL, R0, R1)));
     } else {
-        SEDI_HIP((launch_upd<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, P0, P1, XL, R0, R1)));
+        SEDI_HIP((launch_upd<T, SEDI_FP64_BATCH_TY>(r, a, b, cc, im, P4, XL, R0, R1)));
     }
     if (r.mixed) return replace_and_fold<T>(r, R0, nullptr, true);
     return SEDI_OK;
@@ -1257,7 +1333,7 @@ int BatchSolver::accumulate_t(double* t, double* q, int k0, int k1) {
         for (int j = 0; j < L.n; j++) L.lane[j] = lane_of_[k + j];
         SEDI_HIP(launch("batch_accum_kernel", batch_accum_kernel<T>, dim3((unsigned)g), dim3(256), stream_, n, L,
                         (const V4<T>*)v_[SLOT_XLO], (const double*)xhi_, (const V4<T>*)v_[SLOT_B],
-                        pending_dir<T>(*run_), (const double*)run_->c.apend, t, q));
+                        pending_dirs<T>(*run_), (const double*)run_->c.apend, K_, t, q));
     }
     return SEDI_OK;
 }
@@ -1282,12 +1358,12 @@ int BatchSolver::unpack_t(double* x, int k0, int k1, bool rhs) {
     const int rb0 = k0 / 4, rb1 = (k1 + 3) / 4;
     const V4<T>* src = static_cast<const V4<T>*>(rhs ? v_[SLOT_B] : v_[SLOT_XLO]) + (long long)rb0 * n;
     const double* xhi = rhs ? nullptr : xhi_;
-    const V4<T>* pend = rhs ? nullptr : pending_dir<T>(*run_);
+    PendDirs<T> pend = pending_dirs<T>(*run_);
+    for (int j = 0; j < CG_NPEND; j++) pend.p[j] = (rhs || !pend.p[j]) ? nullptr : pend.p[j] + (long long)rb0 * n;
     // the kernel indexes RHS blocks from rb0: shift the row range to match
     SEDI_HIP(launch("batch_unpack_kernel", batch_unpack_kernel<T>, dim3(ew_blocks(n), rb1 - rb0),
                     dim3(NTHREADS), stream_, n, k0 - 4 * rb0, k1 - 4 * rb0, src,
-                    xhi ? xhi + (long long)4 * rb0 * n : nullptr, pend ? pend + (long long)rb0 * n : nullptr,
-                    run_->c.apend + 4 * rb0, x));
+                    xhi ? xhi + (long long)4 * rb0 * n : nullptr, pend, run_->c.apend + 4 * rb0, K_, x));
     return SEDI_OK;
 }
 

@@ -110,6 +110,18 @@ __device__ __forceinline__ void stv(V4<T>* base, cidx_t i, const V4<T>& v) {
 #endif
 }
 
+// A wave-uniform value moved to scalar registers (the per-block coefficients
+// a pass multiplies by: kept in vector registers they would take 4 per value).
+__device__ __forceinline__ float uniform(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Per-lane select: a where m[k], else b.
 template <class T>
 __device__ __forceinline__ V4<T> sel(const int (&m)[4], const V4<T>& a, const V4<T>& b) {
@@ -125,7 +137,7 @@ struct BCtl {
     double *bb, *ru, *ru_old, *alpha, *beta, *err, *ferr, *elast, *hist;
     double *gamma, *delta, *errp, *gamma_prev, *alpha_prev;  // pipelined CG
     double *rho, *rho_prev, *omega, *ss, *cbw;               // BiCGStab
-    double* apend;  // CG: coefficient of the pending p (the last odd iteration's) not yet in x
+    double* apend;  // CG: coefficients of the pending p1 .. p_{CG_NPEND} not yet in x, [CG_NPEND][K]
     int *fiter, *code, *done, *rep, *fold, *sexit;
     int* plane;  // per lane: the right-hand side (index at load) it holds
     // per RHS block
@@ -247,7 +259,41 @@ struct CgDirTile {
 enum { SLOT_B = 0, SLOT_XLO = 1, SLOT_K0 = 2 };
 // CG: r0, p0, p1, r1 (the residual of iteration it is in r1 when it is odd,
 // in r0 when even: the update pass writes its successor to the other buffer)
-enum { CG_R = SLOT_K0, CG_P0, CG_P1, CG_R1, CG_NVEC };
+// and, with solution updates in groups of 4, p2, p3.  The direction of
+// iteration it is in p[it % CG_XG].
+//
+// CG's solution updates are applied in groups of CG_XG (2 or 4) iterations:
+// x += alpha p of iterations it-CG_XG+1 .. it in one read-modify-write at the
+// iterations it that are multiples of CG_XG, in the order of successive
+// updates (so an FP64 x is bitwise what one update per iteration gives).  In
+// between, an iteration's alpha waits in c.apend[(j-1)*K + k] for the
+// direction in buffer p_j, j = it % CG_XG: the pending directions of a
+// right-hand side are always in buffers p1 .. p_{CG_XG-1}.
+#ifndef SEDI_CG_XGROUP
+#define SEDI_CG_XGROUP 4
+#endif
+constexpr int CG_XG = SEDI_CG_XGROUP;
+constexpr int CG_NPEND = CG_XG - 1;
+static_assert(CG_XG == 2 || CG_XG == 4, "CG solution updates come in groups of 2 or 4");
+enum { CG_R = SLOT_K0, CG_P0, CG_P1, CG_R1, CG_P2, CG_P3 };
+constexpr int CG_NVEC = CG_R1 + 1 + (CG_XG - 2);
+__host__ __device__ constexpr int cg_pslot(int j) { return j == 0 ? CG_P0 : j == 1 ? CG_P1 : j == 2 ? CG_P2 : CG_P3; }
+
+// CG's direction buffers p0 .. p_{CG_XG-1} (kernel argument)
+template <class T>
+struct CgDirs {
+    V4<T>* p[4];
+    // buffer j for a run-time j (selects: indexing the kernel argument would
+    // copy it to scratch memory)
+    __device__ V4<T>* at(int j) const { return j == 0 ? p[0] : j == 1 ? p[1] : j == 2 ? p[2] : p[3]; }
+};
+
+// CG's pending directions, buffers p1 .. p_{CG_NPEND} of the batch (all null:
+// not CG)
+template <class T>
+struct PendDirs {
+    const V4<T>* p[CG_NPEND];
+};
 // pipelined CG: r, u, w0, w1, z, q, s, p
 enum { PCG_R = SLOT_K0, PCG_U, PCG_W0, PCG_W1, PCG_Z, PCG_Q, PCG_S, PCG_P, PCG_NVEC };
 // BiCGStab: r0, r1, r^, p0, p1, v0, v1

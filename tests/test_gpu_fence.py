@@ -48,7 +48,9 @@ print(json.dumps(out))
 
 
 def run_child(lib, tmp):
-    env = dict(os.environ)
+    # both children read the package's table (a variant's folder holds none:
+    # it would run the untuned default plans, another summation order)
+    env = dict(os.environ, SEDI_TUNING_FILE=os.path.join(ROOT, "sedi_amd", "tuning.txt"))
     if lib:
         env["SEDI_AMD_LIB"] = lib
     os.makedirs(tmp, exist_ok=True)
