@@ -236,7 +236,7 @@ __global__ __launch_bounds__(TX * TYK, CgDirTile<T>::min_waves) void batch_dir_k
 template <class T, bool XS>
 struct PassBUpd {
     using V = V4<T>;
-    static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
+    static constexpr int ND = 8, D = XS ? SEDI_PREFETCH_BATCH : SEDI_PREFETCH_CG_UPD;
     T a, b, c;
     const V* p;
     const V* pend[CG_NPEND];  // the pending directions p1 .. (x steps)

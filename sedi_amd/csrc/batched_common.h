@@ -17,6 +17,12 @@
 #ifndef SEDI_PREFETCH_BATCH
 #define SEDI_PREFETCH_BATCH 1
 #endif
+// The CG update pass's iterations that leave x alone: two planes of loads in
+// flight (at 256^3, k = 64: headline 12.555 -> 12.60 probe-solves/s; with two
+// for every batched pass, x steps included, the update pass ran 6% slower)
+#ifndef SEDI_PREFETCH_CG_UPD
+#define SEDI_PREFETCH_CG_UPD 2
+#endif
 #ifndef SEDI_BICG_RESET
 #define SEDI_BICG_RESET 1
 #endif
