@@ -15,8 +15,8 @@
 // run in FP32 with the FP64 scalars rounded to FP32, while the dot products
 // accumulate the exact FP64 products of the FP32 values, and the scalars
 // (alpha, beta, the norms) stay FP64.  An RHS gets an FP64 residual
-// replacement r = b - A (x_hi + x_lo) (computed in FP64, stored as FP32;
-// then x_hi += x_lo, x_lo = 0) every rep_every iterations, when its recursive
+// replacement r = b - A (x_hi + x_lo) (x_hi += x_lo, x_lo = 0 first; computed
+// in FP64, stored as FP32) every rep_every iterations, when its recursive
 // residual has dropped tenfold since the last replacement, when it reaches the
 // tolerance, and at maxiter (CG; the pipelined CG and BiCGStab policies are
 // in bind_t).  The tenfold rule is the reliable-update
@@ -40,7 +40,7 @@
 // the other direction buffers, which still hold them.  Per iteration that is
 // 6 + (2 + CG_XG - 1) / CG_XG vector streams instead of 8: 7.25 with groups
 // of 4, 7.5 with pairs.  Whatever reads x while updates are
-// pending adds them: the FP64 replacement (before its stencil) and fold, the
+// pending adds them: the fold ahead of the FP64 replacement, the
 // accumulation and the unpacking of x, and -- for a right-hand side that
 // finished inside a group while its block goes on -- the next direction pass.
 // The pending directions are always in buffers p1 .. p_{CG_XG-1}.
@@ -354,40 +354,32 @@ __global__ __launch_bounds__(TX * TYK, CgTile<T>::min_waves) void batch_upd_kern
     brb_update(c, rb);
 }
 
-// ---- FP64 replacement: r = b - A (x_hi + x_lo) -------------------------------
-// The new residual is computed in FP64 from the FP64 sum of the two parts of
-// x (plus CG's pending updates, apend_j * p_j) and stored rounded to the vector
-// type.  x itself is read at the halo by
-// neighbouring tiles during this pass, so the fold x_hi += x_lo, x_lo = 0 is
-// left to batch_fold_kernel, which runs after it.  The second dot is <r, u>
-// (CG) or <r, r^> (BiCGStab, RHAT).
+// ---- FP64 replacement: r = b - A x_hi ----------------------------------------
+// The new residual is computed in FP64 from x_hi, which holds all of x for
+// the right-hand sides being replaced: CG folds x_lo and its pending updates
+// into x_hi first (batch_fold_kernel, a pointwise pass, so no tile of this
+// pass can see a half-folded neighbour), and BiCGStab keeps x in FP64 (x_lo
+// stays zero).  The other lanes of a block take part in the stencil but
+// their results are discarded.  r is stored rounded to the vector type.  The
+// second dot is <r, u> (CG) or <r, r^> (BiCGStab, RHAT).
 template <class T, bool RHAT>
 struct PassBRep {
     using V = V4<double>;
     static constexpr int ND = 8, D = SEDI_PREFETCH_BATCH;
     double a, b, c;
     const double* xh[4];  // per-RHS FP64 parts of x of this block ([K][N] layout)
-    const V4<T>* xlo;
     V4<T>* r;
     const V4<T>* rhs;
     V4<T>* rhat;
-    const V4<T>* pend[CG_NPEND];  // CG: the directions with pending updates (null: none)
-    V apend[CG_NPEND];
-    bool pending;
     T im;
     int rep[4];
     bool reset;  // BiCGStab: r^ restarts from the replaced residual
-    struct Raw { V xh; V4<T> xl, pp[CG_NPEND]; };
+    struct Raw { V xh; };
     struct Aux { V4<T> b, r, rh; };
     __device__ Raw load(cidx_t i) const {
         Raw q;
 #pragma unroll
         for (int k = 0; k < 4; k++) q.xh[k] = ldc(xh[k], i);
-        q.xl = ldc(xlo, i);
-        if (pending) {
-#pragma unroll
-            for (int j = 0; j < CG_NPEND; j++) q.pp[j] = ldc(pend[j], i);
-        }
         return q;
     }
     __device__ Aux load_aux(cidx_t i) const {
@@ -397,18 +389,7 @@ struct PassBRep {
         if (RHAT) o.rh = ldc((const V4<T>*)rhat, i);
         return o;
     }
-    __device__ V f(const Raw& q) const {
-        V x;
-#pragma unroll
-        for (int k = 0; k < 4; k++) x[k] = q.xh[k] + (double)q.xl[k];
-        if (pending) {
-#pragma unroll
-            for (int j = 0; j < CG_NPEND; j++)
-#pragma unroll
-                for (int k = 0; k < 4; k++) x[k] = x[k] + apend[j][k] * (double)q.pp[j][k];
-        }
-        return x;
-    }
+    __device__ V f(const Raw& q) const { return q.xh; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& o, const V&, const V& Ax,
                            double (&acc)[ND]) const {
         V4<T> rn = o.r;
@@ -434,9 +415,8 @@ struct PassBRep {
 template <class T, bool RHAT, int ZC>
 __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c, double a, double b,
                                                              double cc, long long n, const double* xhi,
-                                                             const V4<T>* xlo, V4<T>* r0, V4<T>* r1,
-                                                             const V4<T>* rhs, V4<T>* rhat,
-                                                             PendDirs<T> pend, T im) {
+                                                             V4<T>* r0, V4<T>* r1, const V4<T>* rhs, V4<T>* rhat,
+                                                             T im) {
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
     const int* rp = c.rep + 4 * rb;
@@ -455,21 +435,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
         P.xh[k] = xhi + (long long)(4 * rb + k) * n;
         P.rep[k] = rp[k];
     }
-    P.xlo = xlo + off;
     P.r = r + off;
     P.rhs = rhs + off;
     P.rhat = RHAT ? rhat + off : nullptr;
-    bool pending = false;
-#pragma unroll
-    for (int j = 0; j < CG_NPEND; j++) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            P.apend[j][k] = pend.p[j] ? c.apend[j * c.K + 4 * rb + k] : 0.0;
-            pending |= P.apend[j][k] != 0.0;
-        }
-        P.pend[j] = pend.p[j] ? pend.p[j] + off : nullptr;
-    }
-    P.pending = pending;
     P.im = im;
     P.reset = c.bicg_reset != 0;
     double acc[8];
@@ -482,7 +450,6 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
         const int k = 4 * rb + q;
         if (!c.rep[k]) continue;
         c.rep[k] = 0;
-        c.fold[k] = 1;
         const double e = sqrt(acc[4 + q] / c.bb[k]);
         const bool drifted = e > c.restart * c.err[k];
         c.err[k] = e;
@@ -510,15 +477,18 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     brb_update(c, rb);
 }
 
-// ---- x_hi += x_lo, x_lo = 0 for the RHS just replaced ------------------------
-// Runs for a block whether or not it is done (the last update may have
-// finished it).  x_hi + x_lo is unchanged up to the one FP64 rounding of the
-// sum, which the final x = x_hi + x_lo would make anyway.
+// ---- x_hi += x_lo, x_lo = 0 for the RHS being replaced ------------------------
+// CG: before its replacement, for the right-hand sides that asked for one
+// (c.rep; the replacement clears it), also adding the pending updates.
+// Pipelined CG: after its replacement, for those it replaced (c.fold, cleared
+// here).  Runs for a block whether or not it is done (the last update may
+// have finished it).  x_hi + x_lo is unchanged up to the one FP64 rounding of
+// the sum, which the final x = x_hi + x_lo would make anyway.
 template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl c, double* xhi,
-                                                              V4<T>* xlo, PendDirs<T> pend) {
+                                                              V4<T>* xlo, PendDirs<T> pend, bool before_rep) {
     const int rb = blockIdx.y;
-    int* fp = c.fold + 4 * rb;
+    int* fp = (before_rep ? c.rep : c.fold) + 4 * rb;
     const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
     if (!(f0 | f1 | f2 | f3)) return;
     __shared__ double red[NWAVES];
@@ -547,7 +517,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
         for (int k = 0; k < 4; k++) {
             if (fl[k]) {
                 double* h = xhi + (long long)(4 * rb + k) * n + i;
-                // x_hi + x_lo (+ CG's pending apend_j * p_j), as the replacement used
+                // x_hi + x_lo (+ CG's pending apend_j * p_j, in the order of the updates)
                 double v = *h + (double)xl[k];
 #pragma unroll
                 for (int j = 0; j < CG_NPEND; j++)
@@ -565,7 +535,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
     for (int k = 0; k < 4; k++)
         if (fp[k] && pend.p[0])
             for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + 4 * rb + k] = 0.0;
-    fp[0] = fp[1] = fp[2] = fp[3] = 0;
+    if (!before_rep) fp[0] = fp[1] = fp[2] = fp[3] = 0;
 }
 
 // ---- lane compaction ---------------------------------------------------------
@@ -836,30 +806,38 @@ static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
     const dim3 grid(g.nblocks, r.nrb);
     const T im = (T)r.im;
     const double* xhi = r.xhi;
-    const V4<T>* xlo = static_cast<const V4<T>*>(r.v[SLOT_XLO]);
     const V4<T>* b = static_cast<const V4<T>*>(r.v[SLOT_B]);
-    const PendDirs<T> pend = pending_dirs<T>(r);
     if (g.tg.zc == 8)
         return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 8>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
-                      r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
+                      r.a, r.b, r.cc, r.n, xhi, r0, r1, b, rhat, im);
     if (g.tg.zc == 32)
         return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 32>, grid, dim3(NTHREADS), r.stream, g.tg,
-                      r.c, r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
+                      r.c, r.a, r.b, r.cc, r.n, xhi, r0, r1, b, rhat, im);
     return launch("batch_rep_kernel", batch_rep_kernel<T, RHAT, 16>, grid, dim3(NTHREADS), r.stream, g.tg, r.c,
-                  r.a, r.b, r.cc, r.n, xhi, xlo, r0, r1, b, rhat, pend, im);
+                  r.a, r.b, r.cc, r.n, xhi, r0, r1, b, rhat, im);
+}
+
+// before_rep: CG's fold ahead of its replacement (mask c.rep), else the
+// fold after pipelined CG's (mask c.fold)
+template <class T>
+static hipError_t launch_fold(const BRun& r, bool before_rep) {
+    return launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS), r.stream,
+                  r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dirs<T>(r), before_rep);
 }
 
 template <class T>
 int fold_only(const BRun& r) {
-    SEDI_HIP(launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS),
-                    r.stream, r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dirs<T>(r)));
+    SEDI_HIP(launch_fold<T>(r, false));
     return SEDI_OK;
 }
 
-// CG: res is r0 and rhat is null.  BiCGStab: res is r0 and rhat r^.  The
-// kernel picks r0 / r1 by the parity of the next iteration.
+// CG: res is r0 and rhat is null; x_lo and the pending updates of the
+// right-hand sides to replace are folded into x_hi first (fold).  BiCGStab:
+// res is r0 and rhat r^; its x is in x_hi already.  The kernel picks r0 / r1
+// by the parity of the next iteration.
 template <class T>
 int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
+    if (fold) SEDI_HIP(launch_fold<T>(r, true));
     hipError_t e;
     if (rhat) e = launch_rep<T, true>(r, res, static_cast<V4<T>*>(r.v[BI_R1]), rhat);
     else e = launch_rep<T, false>(r, res, static_cast<V4<T>*>(r.v[CG_R1]), nullptr);
@@ -867,7 +845,7 @@ int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
         set_error("replacement launch failed: %s", hipGetErrorString(e));
         return SEDI_ERR_HIP;
     }
-    return fold ? fold_only<T>(r) : SEDI_OK;
+    return SEDI_OK;
 }
 
 // The update pass with TYK-row tiles: its geometry and a launch of the

@@ -326,8 +326,10 @@ struct BRun {
 template <class T> int pcg_setup(const BRun& r, int rb0, int count);
 template <class T> int pcg_iteration(const BRun& r);
 template <class T> int bicg_iteration(const BRun& r);
-// the FP64 replacement r = b - A x with <r,u> (CG) or <r,r^> (BiCGStab) and,
-// when `fold`, the fold x_hi += x_lo that follows it (batched.hip)
+// the FP64 replacement r = b - A x_hi with <r,u> (CG) or <r,r^> (BiCGStab),
+// preceded, when `fold` (CG), by the fold of x_lo and the pending updates
+// into x_hi (batched.hip); fold_only: pipelined CG's fold after its own
+// replacement
 template <class T> int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold);
 template <class T> int fold_only(const BRun& r);
 
