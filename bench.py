@@ -77,11 +77,11 @@ DOMINANT = {2: ("cg_update_kernel", None), 3: ("pcg_iter_kernel", 16), 1: ("bicg
 # iteration, and at every fourth iteration also x_lo in and out and the three
 # previous directions in (the solution updates of four iterations are applied
 # together; the library's default -DSEDI_CG_XGROUP=4): 3 or 8 FP32 values per
-# right-hand side and cell.  The plain solver's cg_update_kernel applies them
-# in pairs: 3 or 6 FP64 values.
+# right-hand side and cell.  The plain solver's cg_update_kernel groups them
+# the same way: 3 or 8 FP64 values.
 BATCH_KERNEL = "batch_upd_kernel"
 BATCH_CG_GROUP = 4
-PLAIN_CG_GROUP = 2
+PLAIN_CG_GROUP = 4
 
 
 def upd_values(iters, group):
@@ -424,8 +424,9 @@ def run_plain(args):
                           "mean": round(float(iters.mean()), 2)}
     line["estimator_loop_s"] = round(sum(r["seconds"] for r in runs), 4)
     # algorithmic bytes of the iterations the probes ran (launches queued after
-    # a solve converged return at once and move nothing); plain CG pairs its
-    # solution updates: 3 vectors at odd iterations, 6 at even ones
+    # a solve converged return at once and move nothing); plain CG groups its
+    # solution updates by four: 3 vectors at iterations 1-3 of a group, 8 at
+    # the 4th
     if kind == 2:
         vecs = sum(upd_values(int(m), PLAIN_CG_GROUP) for m in iters)
     else:

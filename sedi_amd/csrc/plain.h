@@ -17,11 +17,14 @@ namespace sedi {
 struct DevState {
     double bb, tol, err, final_err;
     double ru, alpha, beta;          // CG (alpha, beta shared with the others)
-    double apend;                    // CG: alpha of the odd iteration whose x update is pending
+    double apend[3];                 // CG: alphas of the group's iterations whose x updates are pending
     double gamma, delta, err_pending;  // pipelined CG
     double rho, rho_prev, omega, ss, cbw;  // BiCGStab
     int maxiter, iter, done, code, final_iter, nhist, sexit, kind;
 };
+
+// the control block holds the state in its first 256 bytes
+static_assert(sizeof(DevState) <= 256, "DevState outgrew its slot of the control block");
 
 struct Ctl {
     DevState* st;
@@ -48,9 +51,10 @@ struct PlainCore {
     double* v[8] = {};
 };
 
-// Vectors a solver of `kind` keeps: CG r, p x2; pipelined CG r, u, w x2, z,
-// q, s, p; BiCGStab r x2, r^, p x2, v x2.
-inline int plain_vectors(int kind) { return kind == SEDI_CG ? 3 : (kind == SEDI_PCG ? 8 : 7); }
+// Vectors a solver of `kind` keeps: CG r, p x4 (one per iteration of a group
+// of solution updates); pipelined CG r, u, w x2, z, q, s, p; BiCGStab r x2,
+// r^, p x2, v x2.
+inline int plain_vectors(int kind) { return kind == SEDI_CG ? 5 : (kind == SEDI_PCG ? 8 : 7); }
 
 // Bytes of the control block for `nblocks` workgroups (3 partial sums each).
 inline size_t plain_ctl_bytes(int nblocks) { return 512 + sizeof(double) * 3 * (size_t)nblocks; }

@@ -94,14 +94,26 @@ __global__ void init_state_kernel(DevState* st, int kind, int maxiter, double to
     }
 }
 
-// CG's last solution update when the solve ended at an odd iteration: x +=
-// apend p (p of an odd iteration is in the second direction buffer).
-__global__ void cg_flush_kernel(const DevState* st, long long n, double* x, const double* p1) {
-    const double ap = st->apend;
-    if (ap == 0.0) return;
+// CG's direction buffers p0 .. p3: the direction of iteration it is in
+// p[it % 4].
+struct CgDirs {
+    double* p[4];
+    __device__ double* at(int j) const { return j == 0 ? p[0] : j == 1 ? p[1] : j == 2 ? p[2] : p[3]; }
+};
+
+// CG's last solution updates when the solve ended inside a group: x +=
+// apend_j p_j for the pending ones, in the order of their iterations (the
+// direction of the group's j-th iteration is in p_j).
+__global__ void cg_flush_kernel(const DevState* st, long long n, double* x, CgDirs d) {
+    const double a1 = st->apend[0], a2 = st->apend[1], a3 = st->apend[2];
+    if (a1 == 0.0) return;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        x[i] = 1.0 * x[i] + ap * p1[i];
+         i += (long long)gridDim.x * blockDim.x) {
+        double v = 1.0 * x[i] + a1 * d.p[1][i];
+        if (a2 != 0.0) v = 1.0 * v + a2 * d.p[2][i];
+        if (a3 != 0.0) v = 1.0 * v + a3 * d.p[3][i];
+        x[i] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -184,33 +196,35 @@ struct PassCgDirection {
 // CG update: s = A p recomputed; x += alpha p; r -= alpha s; u = r.*iM;
 // <r,u>, <r,r> (Cg.hpp:168-182).
 //
-// x is write-only in CG's recurrences, so its updates go in pairs: an odd
-// iteration leaves x alone (its alpha becomes DevState::apend), the next even
-// one computes x + apend p_prev, then + alpha p -- the reference's two
-// combine2 updates (SliceOps.hpp:38-40) in their order, so x is bitwise the
-// same -- reading p_prev from the other direction buffer, which still holds
-// it.  A solve that ends at an odd iteration gets its last update from
-// cg_flush_kernel.  The pass streams 3 vectors at odd iterations and 6 at
-// even ones instead of 5 every iteration.
-template <class PC>
+// x is write-only in CG's recurrences, so its updates go in groups of four
+// iterations: iterations 1-3 of a group leave x alone (their alphas wait in
+// DevState::apend), the 4th computes x + alpha_1 p_1 + alpha_2 p_2 +
+// alpha_3 p_3 + alpha p as four successive combine2 updates (SliceOps.hpp:
+// 38-40) in the reference's order, so x is bitwise the same, reading the
+// earlier directions from their own buffers, which still hold them.  A solve
+// that ends inside a group gets its last updates from cg_flush_kernel.  The
+// pass streams 3 vectors at iterations 1-3 and 8 at the 4th (4.25 a
+// iteration) instead of 5 every iteration.  XS (the 4th) is a compile-time
+// choice, so the other iterations carry no loads or registers for x.
+template <class PC, bool XS>
 struct PassCgUpdate {
     static constexpr int ND = 2, D = SEDI_PREFETCH_CG;
     double a, b, c;
     const double* p;
-    const double* pprev;
+    const double* pend[3];
     double* x;
     double* r;
     PC iM;
-    double alpha, apend;
-    bool xstep;
+    double alpha, apend[3];
     struct Raw { double p; };
-    struct Aux { double x, pp, r, im; };
+    struct Aux { double x, pp[3], r, im; };
     __device__ Raw load(cidx_t i) const { return {ldc(p, i)}; }
     __device__ Aux load_aux(cidx_t i) const {
         Aux o;
-        if (xstep) {
+        if (XS) {
             o.x = ldc(x, i);
-            o.pp = ldc(pprev, i);
+#pragma unroll
+            for (int j = 0; j < 3; j++) o.pp[j] = ldc(pend[j], i);
         }
         o.r = ldc(r, i);
         o.im = iM.at(i);
@@ -218,8 +232,10 @@ struct PassCgUpdate {
     }
     __device__ double f(const Raw& q) const { return q.p; }
     __device__ void finish(cidx_t i, const Raw&, const Aux& q, double pv, double s, double (&acc)[ND]) const {
-        if (xstep) {
-            const double xm = 1.0 * q.x + apend * q.pp;
+        if (XS) {
+            double xm = q.x;
+#pragma unroll
+            for (int j = 0; j < 3; j++) xm = 1.0 * xm + apend[j] * q.pp[j];
             stp(x, i, 1.0 * xm + alpha * pv);
         }
         const double rv = 1.0 * q.r + (-alpha) * s;
@@ -494,12 +510,12 @@ __global__ __launch_bounds__(PTHREADS, 4) void residual_kernel(TileGrid g, Ctl c
 template <class PC, int ZC>
 __global__ __launch_bounds__(PTHREADS, 4) void cg_direction_kernel(TileGrid g, Ctl ctl, double a,
                                                                 double b, double c, const double* r,
-                                                                double* p0, double* p1, PC iM) {
+                                                                CgDirs d, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    PassCgDirection<PC> P{a, b, c, r, (it & 1) ? p0 : p1, (it & 1) ? p1 : p0, iM, cst->beta, it == 1};
+    PassCgDirection<PC> P{a, b, c, r, d.at((it + 3) & 3), d.at(it & 3), iM, cst->beta, it == 1};
     double acc[1];
     run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, false);
     if (!grid_sum<1, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
@@ -507,23 +523,32 @@ __global__ __launch_bounds__(PTHREADS, 4) void cg_direction_kernel(TileGrid g, C
     st->alpha = st->ru / acc[0];  // Cg.hpp:166
 }
 
+template <class PC, int ZC, bool XS>
+__device__ __forceinline__ void run_cg_update(const TileGrid& g, const DevState* cst, double a, double b, double c,
+                                              const CgDirs& d, int it, double* x, double* r, PC iM,
+                                              double (&lds)[2][PTY + 2][TX + 2], double (&acc)[2]) {
+    PassCgUpdate<PC, XS> P{a, b, c, d.at(it & 3), {d.p[1], d.p[2], d.p[3]}, x, r, iM, cst->alpha,
+                           {cst->apend[0], cst->apend[1], cst->apend[2]}};
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, true);
+}
+
 template <class PC, int ZC>
 __global__ __launch_bounds__(PTHREADS, 4) void cg_update_kernel(TileGrid g, Ctl ctl, double a, double b,
-                                                             double c, const double* p0,
-                                                             const double* p1, double* x, double* r,
+                                                             double c, CgDirs d, double* x, double* r,
                                                              PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    const bool xstep = (it & 1) == 0;
-    PassCgUpdate<PC> P{a, b, c, (it & 1) ? p1 : p0, (it & 1) ? p0 : p1, x, r, iM, cst->alpha, cst->apend,
-                       xstep};
+    const int jp = it & 3;
     double acc[2];
-    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, true);
+    if (jp == 0) run_cg_update<PC, ZC, true>(g, cst, a, b, c, d, it, x, r, iM, lds, acc);
+    else run_cg_update<PC, ZC, false>(g, cst, a, b, c, d, it, x, r, iM, lds, acc);
     if (!grid_sum<2, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
     DevState* st = ctl.st;
-    st->apend = xstep ? 0.0 : st->alpha;
+    // this alpha waits for the group's last iteration, which applies them all
+    if (jp == 0) st->apend[0] = st->apend[1] = st->apend[2] = 0.0;
+    else st->apend[jp - 1] = st->alpha;
     const double ru_new = acc[0];
     st->beta = ru_new / st->ru;  // Cg.hpp:185
     st->ru = ru_new;
@@ -810,10 +835,11 @@ static int core_iterations(const PlainCore& k, int count, PC pc, double* x) {
     for (int it = 0; it < count; it++) {
         if (k.kind == SEDI_CG) {
             const KGeom& g1 = geom_for("cg_direction_kernel", cg_direction_kernel<PC, 16>, k.shape, PassCgDirection<PC>::D);
-            const KGeom& g2 = geom_for("cg_update_kernel", cg_update_kernel<PC, 16>, k.shape, PassCgUpdate<PC>::D);
-            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, k.stream, g1.tg, k.ctl, k.a, k.b, k.c, (const double*)k.v[0], k.v[1], k.v[2], pc));
+            const KGeom& g2 = geom_for("cg_update_kernel", cg_update_kernel<PC, 16>, k.shape, PassCgUpdate<PC, false>::D);
+            const CgDirs d{{k.v[1], k.v[2], k.v[3], k.v[4]}};
+            SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_direction_kernel", cg_direction_kernel, PC, g1, k.stream, g1.tg, k.ctl, k.a, k.b, k.c, (const double*)k.v[0], d, pc));
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, k.stream, g2.tg,
-                            k.ctl, k.a, k.b, k.c, (const double*)k.v[1], (const double*)k.v[2], x, k.v[0], pc));
+                            k.ctl, k.a, k.b, k.c, d, x, k.v[0], pc));
         } else if (k.kind == SEDI_PCG) {
             const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, k.shape, PassPcgIter<PC>::D);
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, k.stream, g1.tg,
@@ -830,13 +856,13 @@ static int core_iterations(const PlainCore& k, int count, PC pc, double* x) {
     return SEDI_OK;
 }
 
-// CG's pending solution update of a solve that ended at an odd iteration.
+// CG's pending solution updates of a solve that ended inside a group.
 static int core_flush(const PlainCore& k, double* x) {
     if (k.kind != SEDI_CG) return SEDI_OK;
     const long long n = k.shape.n();
     const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4LL * num_cus());
     SEDI_HIP(launch("cg_flush_kernel", cg_flush_kernel, dim3(std::max(1u, blocks)), dim3(256), k.stream,
-                    (const DevState*)k.ctl.st, n, x, (const double*)k.v[2]));
+                    (const DevState*)k.ctl.st, n, x, CgDirs{{k.v[1], k.v[2], k.v[3], k.v[4]}}));
     return SEDI_OK;
 }
 

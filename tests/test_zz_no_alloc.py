@@ -187,7 +187,7 @@ def test_noalloc_rejects_host_arrays_and_oversize():
     rc = L.sedi_solve_noalloc(ctypes.byref(g), ctypes.byref(st), O.CG, b.ctypes.data, x.ctypes.data, None, 10,
                               1e-9, ctypes.byref(fe), ctypes.byref(fi))
     assert rc == -1 and b"device arrays" in L.sedi_last_error()
-    big = sedi._Grid(256, 256, 256)  # 3 vectors of 128 MiB: above the 64 MiB array
+    big = sedi._Grid(256, 256, 256)  # 5 vectors of 128 MiB: above the 64 MiB array
     rc = L.sedi_solve_noalloc(ctypes.byref(big), ctypes.byref(st), O.CG, b.ctypes.data, x.ctypes.data, None, 10,
                               1e-9, ctypes.byref(fe), ctypes.byref(fi))
     assert rc == -1 and b"allocation-free limit" in L.sedi_last_error()
