@@ -703,7 +703,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_swap_kernel(BCtl c, long long 
 }
 
 // ---- t += x.*v, q += v.*v over the batch's probes in order ------------------
-// x = x_hi + x_lo (+ CG's pending apend * p: pend non-null); the probes are
+// x = x_hi + x_lo (+ CG's pending apend_j * p_j: pend.p[j] non-null); the probes are
 // the lanes of `lanes`, in its order
 struct LaneList {
     int n;

@@ -114,7 +114,7 @@ def test_batch_eigenvector_one_iteration():
 
 
 @pytest.mark.parametrize("prec", [sedi.FP64, sedi.FP32_MIXED])
-def test_update_pending_when_finishing_at_odd_iteration(prec):
+def test_update_pending_when_finishing_at_iteration_one(prec):
     """CG applies solution updates in groups; a right-hand side that finishes
     inside a group has updates pending.  Here the eigenvector finishes at
     iteration 1 while its block-mates go on (the next direction pass applies
