@@ -71,8 +71,9 @@ PLAIN = dict(samples=16, seed=7, tol=1e-9)
 BATCH = dict(samples=64, seed=11, tol=1e-8, rep_every=50)
 CMP_SAMPLES = 16
 # dominant kernel per solver and the FP64 vectors it streams per iteration
-# (CG: see upd_values)
-DOMINANT = {2: ("cg_update_kernel", None), 3: ("pcg_iter_kernel", 16), 1: ("bicgstab_update_kernel", 7)}
+# (CG and pipelined CG, whose solution updates come in groups: see
+# upd_values, pcg_values)
+DOMINANT = {2: ("cg_update_kernel", None), 3: ("pcg_iter_kernel", None), 1: ("bicgstab_update_kernel", 7)}
 # batched mixed CG: batch_upd_kernel streams p and r in and r out every
 # iteration, and at every fourth iteration also x_lo in and out and the three
 # previous directions in (the solution updates of four iterations are applied
@@ -90,6 +91,14 @@ def upd_values(iters, group):
     iteration, plus x in and out and the group's other directions in at every
     group's last iteration."""
     return 3 * iters + (2 + group - 1) * (iters // group)
+
+
+def pcg_values(iters, group):
+    """Vectors the plain pipelined-CG pass streams for `iters` iterations with
+    solution updates in groups of `group`: 7 in and 7 out per iteration (w,
+    q, s, u, p, z, r; x waits), plus x in and out and the group's other
+    directions in at every group's last iteration."""
+    return 14 * iters + (2 + group - 1) * (iters // group)
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "sedi_ref")
 MPIRUN = "/opt/conda/bin/mpirun"
 
@@ -429,6 +438,8 @@ def run_plain(args):
     # the 4th
     if kind == 2:
         vecs = sum(upd_values(int(m), PLAIN_CG_GROUP) for m in iters)
+    elif kind == 3:  # pipelined CG groups them the same way: 14 vectors, 19 at every 4th
+        vecs = sum(pcg_values(int(m), PLAIN_CG_GROUP) for m in iters)
     else:
         vecs = int(iters.sum()) * nvec
     line["roofline"] = roofline(kname, launches, kms, vecs * n * 8)

@@ -17,7 +17,7 @@ namespace sedi {
 struct DevState {
     double bb, tol, err, final_err;
     double ru, alpha, beta;          // CG (alpha, beta shared with the others)
-    double apend[3];                 // CG: alphas of the group's iterations whose x updates are pending
+    double apend[3];                 // CG, pipelined CG: alphas of the group's iterations whose x updates are pending
     double gamma, delta, err_pending;  // pipelined CG
     double rho, rho_prev, omega, ss, cbw;  // BiCGStab
     int maxiter, iter, done, code, final_iter, nhist, sexit, kind;
@@ -48,13 +48,14 @@ struct PlainCore {
     hipStream_t stream = nullptr;
     Ctl ctl{};
     double* const* vv = nullptr;  // device table of v[0..6] (BiCGStab)
-    double* v[8] = {};
+    double* v[11] = {};
 };
 
 // Vectors a solver of `kind` keeps: CG r, p x4 (one per iteration of a group
-// of solution updates); pipelined CG r, u, w x2, z, q, s, p; BiCGStab r x2,
+// of solution updates); pipelined CG r, u, w x2, z, q, s, p x4; BiCGStab r x2,
 // r^, p x2, v x2.
-inline int plain_vectors(int kind) { return kind == SEDI_CG ? 5 : (kind == SEDI_PCG ? 8 : 7); }
+inline int plain_vectors(int kind) { return kind == SEDI_CG ? 5 : (kind == SEDI_PCG ? 11 : 7); }
+static_assert(sizeof(PlainCore::v) / sizeof(double*) == 11, "PlainCore::v holds the most vectors a solver keeps");
 
 // Bytes of the control block for `nblocks` workgroups (3 partial sums each).
 inline size_t plain_ctl_bytes(int nblocks) { return 512 + sizeof(double) * 3 * (size_t)nblocks; }

@@ -274,19 +274,31 @@ struct PassPcgInit {
 
 // Pipelined CG iteration (PipelinedCg.hpp:157-223): m = w.*iM, n = A m, the
 // eight recurrences in the reference's order, and the next <r,u>, <w,u>, <r,r>.
-template <class PC>
+template <class PC, bool XS>
 struct PassPcgIter {
     static constexpr int ND = 3, D = SEDI_PREFETCH_PCG;
     double a, b, c;
     const double* wold;
     double* wnew;
-    double *x, *r, *u, *z, *q, *s, *p;
+    double *x, *r, *u, *z, *q, *s;
+    const double* pold;
+    double* pnew;
+    const double* pend[3];
     PC iM;
-    double alpha, beta;
+    double alpha, beta, apend[3];
     struct Raw { double w, im; };
-    struct Aux { double q, s, u, p, z, r, x; };
+    struct Aux { double q, s, u, p, z, r, x, pp[3]; };
     __device__ Raw load(cidx_t i) const { return {ldc(wold, i), iM.at(i)}; }
-    __device__ Aux load_aux(cidx_t i) const { return {ldc(q, i), ldc(s, i), ldc(u, i), ldc(p, i), ldc(z, i), ldc(r, i), ldc(x, i)}; }
+    __device__ Aux load_aux(cidx_t i) const {
+        Aux o;
+        o.q = ldc(q, i); o.s = ldc(s, i); o.u = ldc(u, i); o.p = ldc(pold, i); o.z = ldc(z, i); o.r = ldc(r, i);
+        if (XS) {
+            o.x = ldc(x, i);
+#pragma unroll
+            for (int j = 0; j < 3; j++) o.pp[j] = ldc(pend[j], i);
+        }
+        return o;
+    }
     __device__ double f(const Raw& c) const { return c.w * c.im; }
     __device__ void finish(cidx_t i, const Raw& c, const Aux& o, double m, double n, double (&acc)[ND]) const {
         const double qv = 1.0 * m + beta * o.q;
@@ -294,11 +306,17 @@ struct PassPcgIter {
         const double pv = 1.0 * o.u + beta * o.p;
         const double zv = 1.0 * n + beta * o.z;
         const double rv = 1.0 * o.r + (-alpha) * sv;
-        const double xv = 1.0 * o.x + alpha * pv;
         const double uv = 1.0 * o.u + (-alpha) * qv;
         const double wv = 1.0 * c.w + (-alpha) * zv;
-        stp(q, i, qv); stp(s, i, sv); stp(p, i, pv); stp(z, i, zv);
-        stp(r, i, rv); stp(x, i, xv); stp(u, i, uv); stp(wnew, i, wv);
+        stp(q, i, qv); stp(s, i, sv); stp(pnew, i, pv); stp(z, i, zv);
+        stp(r, i, rv); stp(u, i, uv); stp(wnew, i, wv);
+        if (XS) {
+            // the group's pending x += alpha_j p_j, then this iteration's
+            double xm = o.x;
+#pragma unroll
+            for (int j = 0; j < 3; j++) xm = 1.0 * xm + apend[j] * o.pp[j];
+            stp(x, i, 1.0 * xm + alpha * pv);
+        }
         acc[0] += rv * uv;
         acc[1] += wv * uv;
         acc[2] += rv * rv;
@@ -582,22 +600,43 @@ __global__ __launch_bounds__(PTHREADS, 4) void pcg_init_kernel(TileGrid g, Ctl c
     st->iter = 1;
 }
 
+template <class PC, int ZC, bool XS>
+__device__ __forceinline__ void run_pcg_iter(const TileGrid& g, const DevState* cst, double a, double b, double c,
+                                             double* w0, double* w1, double* x, double* r, double* u, double* z,
+                                             double* q, double* s, const CgDirs& d, int it, PC iM,
+                                             double (&lds)[2][PTY + 2][TX + 2], double (&acc)[3]) {
+    // iteration 1 reads w from w0 (written by pcg_init_kernel) and p from p0
+    PassPcgIter<PC, XS> P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s,
+                          d.at((it + 3) & 3), d.at(it & 3), {d.p[1], d.p[2], d.p[3]}, iM, cst->alpha, cst->beta,
+                          {cst->apend[0], cst->apend[1], cst->apend[2]}};
+    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (it & 1) == 0);
+}
+
+// Pipelined CG's solution updates come in groups of four iterations, as
+// plain CG's (the x update is the last of the eight recurrences and nothing
+// else reads x): the direction of iteration it is written to p[it % 4], the
+// 4th iteration of a group adds the three pending updates and its own.
 template <class PC, int ZC>
 __global__ __launch_bounds__(PTHREADS, 4) void pcg_iter_kernel(TileGrid g, Ctl ctl, double a, double b,
                                                             double c, double* w0, double* w1,
                                                             double* x, double* r, double* u,
                                                             double* z, double* q, double* s,
-                                                            double* p, PC iM) {
+                                                            CgDirs d, PC iM) {
     const DevState* cst = ctl.st;
     if (cst->done) return;
     PASS_SHARED;
     const int it = cst->iter;
-    // iteration 1 reads w from w0 (written by pcg_init_kernel)
-    PassPcgIter<PC> P{a, b, c, (it & 1) ? w0 : w1, (it & 1) ? w1 : w0, x, r, u, z, q, s, p, iM,
-                  cst->alpha, cst->beta};
+    const int jp = it & 3;
     double acc[3];
-    run_tiles<decltype(P), ZC, PTY>(P, g, lds, acc, (it & 1) == 0);
+    if (jp == 0) run_pcg_iter<PC, ZC, true>(g, cst, a, b, c, w0, w1, x, r, u, z, q, s, d, it, iM, lds, acc);
+    else run_pcg_iter<PC, ZC, false>(g, cst, a, b, c, w0, w1, x, r, u, z, q, s, d, it, iM, lds, acc);
     if (!grid_sum<3, PTY>(acc, ctl.partials, ctl.counter, red, &lastflag) || threadIdx.x != 0) return;
+    {
+        // the alpha this pass used waits for the group's last iteration
+        DevState* st = ctl.st;
+        if (jp == 0) st->apend[0] = st->apend[1] = st->apend[2] = 0.0;
+        else st->apend[jp - 1] = st->alpha;
+    }
     DevState* st = ctl.st;
     const double err = st->err_pending;  // rr before this iteration's update (:227)
     st->err = err;
@@ -824,7 +863,7 @@ static int core_setup(const PlainCore& k, PC pc, const double* b, double* x) {
     if (k.kind == SEDI_PCG) {
         const KGeom& kp = geom_for("pcg_init_kernel", pcg_init_kernel<PC, 16>, k.shape, PassPcgInit<PC>::D);
         SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_init_kernel", pcg_init_kernel, PC, kp, k.stream, kp.tg, k.ctl,
-                        k.a, k.b, k.c, (const double*)k.v[0], k.v[1], k.v[2], k.v[4], k.v[5], k.v[6], k.v[7], pc));
+                        k.a, k.b, k.c, (const double*)k.v[0], k.v[1], k.v[2], k.v[4], k.v[5], k.v[6], k.v[7], pc));  // p0
     }
     return SEDI_OK;
 }
@@ -841,9 +880,10 @@ static int core_iterations(const PlainCore& k, int count, PC pc, double* x) {
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("cg_update_kernel", cg_update_kernel, PC, g2, k.stream, g2.tg,
                             k.ctl, k.a, k.b, k.c, d, x, k.v[0], pc));
         } else if (k.kind == SEDI_PCG) {
-            const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, k.shape, PassPcgIter<PC>::D);
+            const KGeom& g1 = geom_for("pcg_iter_kernel", pcg_iter_kernel<PC, 16>, k.shape, PassPcgIter<PC, false>::D);
             SEDI_HIP(SEDI_ZC_LAUNCH_PC("pcg_iter_kernel", pcg_iter_kernel, PC, g1, k.stream, g1.tg,
-                            k.ctl, k.a, k.b, k.c, k.v[2], k.v[3], x, k.v[0], k.v[1], k.v[4], k.v[5], k.v[6], k.v[7], pc));
+                            k.ctl, k.a, k.b, k.c, k.v[2], k.v[3], x, k.v[0], k.v[1], k.v[4], k.v[5], k.v[6],
+                            CgDirs{{k.v[7], k.v[8], k.v[9], k.v[10]}}, pc));
         } else {
             const KGeom& g1 = geom_for("bicgstab_direction_kernel", bicgstab_direction_kernel<PC, 16>, k.shape, PassBicgDirection<PC>::D);
             const KGeom& g2 = geom_for("bicgstab_half_kernel", bicgstab_half_kernel<PC, 16>, k.shape, PassBicgHalf<PC>::D);
@@ -856,13 +896,16 @@ static int core_iterations(const PlainCore& k, int count, PC pc, double* x) {
     return SEDI_OK;
 }
 
-// CG's pending solution updates of a solve that ended inside a group.
+// CG's and pipelined CG's pending solution updates of a solve that ended
+// inside a group (their direction buffers: CG v[1..4], pipelined CG v[7..10]).
 static int core_flush(const PlainCore& k, double* x) {
-    if (k.kind != SEDI_CG) return SEDI_OK;
+    if (k.kind != SEDI_CG && k.kind != SEDI_PCG) return SEDI_OK;
     const long long n = k.shape.n();
     const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4LL * num_cus());
+    const CgDirs d = k.kind == SEDI_CG ? CgDirs{{k.v[1], k.v[2], k.v[3], k.v[4]}}
+                                       : CgDirs{{k.v[7], k.v[8], k.v[9], k.v[10]}};
     SEDI_HIP(launch("cg_flush_kernel", cg_flush_kernel, dim3(std::max(1u, blocks)), dim3(256), k.stream,
-                    (const DevState*)k.ctl.st, n, x, CgDirs{{k.v[1], k.v[2], k.v[3], k.v[4]}}));
+                    (const DevState*)k.ctl.st, n, x, d));
     return SEDI_OK;
 }
 
