@@ -26,8 +26,12 @@ ARCH = os.environ.get("SEDI_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-# Packed-FP32 instructions are switched off for every device compile.
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# Packed-FP32 instructions are switched off for every device compile.  The
+# march unrolls a tile's planes fully (#pragma unroll); the largest pass body
+# (batched pipelined CG's x step, 32 planes) exceeds LLVM's default size limit
+# for that, which left its ring of loads in scratch memory.
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-pragma-unroll-threshold=65536"]
 
 LIB_SOURCES = ["runtime.cpp", "api.cpp", "mt_jump.cpp", "h5io.cpp", "tuning.cpp", "plain.hip", "rng.hip", "batched.hip",
                "batched_pcg.hip", "batched_bicg.hip", "estimator.hip", "serve.hip"]

@@ -19,6 +19,9 @@ struct BatchResult {
 
 struct BRun;
 
+// vector slots a batch can hold (b, x_lo and a solver's vectors)
+constexpr int BATCH_MAX_SLOTS = 16;
+
 class BatchSolver {
   public:
     ~BatchSolver();
@@ -96,7 +99,7 @@ class BatchSolver {
     int rep_every_ = 50;
     hipStream_t stream_ = nullptr;
     DevBuf vec_, ctl_, hist_;
-    void* v_[12] = {};
+    void* v_[BATCH_MAX_SLOTS] = {};
     double* xhi_ = nullptr;
     int* pinned_ = nullptr;
     int hist_cap_ = 0;

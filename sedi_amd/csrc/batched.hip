@@ -477,23 +477,25 @@ __global__ __launch_bounds__(NTHREADS) void batch_rep_kernel(TileGrid g, BCtl c,
     brb_update(c, rb);
 }
 
-// ---- x_hi += x_lo, x_lo = 0 for the RHS being replaced ------------------------
-// CG: before its replacement, for the right-hand sides that asked for one
-// (c.rep; the replacement clears it), also adding the pending updates.
-// Pipelined CG: after its replacement, for those it replaced (c.fold, cleared
-// here).  Runs for a block whether or not it is done (the last update may
-// have finished it).  x_hi + x_lo is unchanged up to the one FP64 rounding of
-// the sum, which the final x = x_hi + x_lo would make anyway.
+// ---- x_hi += x_lo (+ the pending updates), x_lo = 0 -------------------------
+// Runs ahead of a replacement, for the right-hand sides that asked for one
+// (c.rep; the replacement clears it) and for pipelined CG's right-hand sides
+// that finished with updates pending (c.fold, cleared here): their x is then
+// final, and the same whether their block goes on (it would otherwise apply
+// them at its next group's end, rounded to FP32) or not (the read-out adds
+// them in FP64).  Runs for a block whether or not it is done.  x_hi + x_lo is
+// unchanged up to the one FP64 rounding of the sum, which the final
+// x = x_hi + x_lo would make anyway.
 template <class T>
 __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl c, double* xhi,
-                                                              V4<T>* xlo, PendDirs<T> pend, bool before_rep) {
+                                                              V4<T>* xlo, PendDirs<T> pend) {
     const int rb = blockIdx.y;
-    int* fp = (before_rep ? c.rep : c.fold) + 4 * rb;
-    const int f0 = fp[0], f1 = fp[1], f2 = fp[2], f3 = fp[3];
-    if (!(f0 | f1 | f2 | f3)) return;
+    const int* rp = c.rep + 4 * rb;
+    int* fp = c.fold + 4 * rb;
+    const int fl[4] = {rp[0] | fp[0], rp[1] | fp[1], rp[2] | fp[2], rp[3] | fp[3]};
+    if (!(fl[0] | fl[1] | fl[2] | fl[3])) return;
     __shared__ double red[NWAVES];
     __shared__ int lastflag;
-    const int fl[4] = {f0, f1, f2, f3};
     double ap[CG_NPEND][4];
     const V4<T>* pq[CG_NPEND];
 #pragma unroll
@@ -521,7 +523,7 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
                 double v = *h + (double)xl[k];
 #pragma unroll
                 for (int j = 0; j < CG_NPEND; j++)
-                    if (pq[j]) v = v + ap[j][k] * (double)pp[j][k];
+                    if (pq[j] && ap[j][k] != 0.0) v = v + ap[j][k] * (double)pp[j][k];
                 *h = v;
                 xl[k] = (T)0;
             }
@@ -533,9 +535,9 @@ __global__ __launch_bounds__(NTHREADS) void batch_fold_kernel(long long n, BCtl 
         threadIdx.x != 0)
         return;
     for (int k = 0; k < 4; k++)
-        if (fp[k] && pend.p[0])
+        if (fl[k] && pend.p[0])
             for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + 4 * rb + k] = 0.0;
-    if (!before_rep) fp[0] = fp[1] = fp[2] = fp[3] = 0;
+    fp[0] = fp[1] = fp[2] = fp[3] = 0;
 }
 
 // ---- lane compaction ---------------------------------------------------------
@@ -637,7 +639,7 @@ __global__ __launch_bounds__(256) void batch_compact_kernel(BCtl c, int nrb) {
 }
 
 struct VSlots {
-    void* p[12];
+    void* p[BATCH_MAX_SLOTS];
 };
 
 // Grid: x over cells, y over the vector slots to exchange plus one for the
@@ -783,13 +785,15 @@ __global__ void batch_unpack_kernel(long long n, int k0, int k1, const V4<T>* __
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// The direction buffers that hold CG's pending updates (p1 .. p_{CG_NPEND});
-// null for the other solvers, whose x is always current.
+// The direction buffers that hold CG's or pipelined CG's pending updates
+// (p1 .. p_{CG_NPEND}); null for BiCGStab, whose x is always current.
 template <class T>
 static PendDirs<T> pending_dirs(const BRun& r) {
     PendDirs<T> d;
     for (int j = 0; j < CG_NPEND; j++)
-        d.p[j] = r.c.kind == SEDI_CG ? static_cast<const V4<T>*>(r.v[cg_pslot(j + 1)]) : nullptr;
+        d.p[j] = r.c.kind == SEDI_CG    ? static_cast<const V4<T>*>(r.v[cg_pslot(j + 1)])
+                 : r.c.kind == SEDI_PCG ? static_cast<const V4<T>*>(r.v[pcg_pslot(j + 1)])
+                                        : nullptr;
     return d;
 }
 
@@ -817,17 +821,15 @@ static hipError_t launch_rep(const BRun& r, V4<T>* r0, V4<T>* r1, V4<T>* rhat) {
                   r.a, r.b, r.cc, r.n, xhi, r0, r1, b, rhat, im);
 }
 
-// before_rep: CG's fold ahead of its replacement (mask c.rep), else the
-// fold after pipelined CG's (mask c.fold)
 template <class T>
-static hipError_t launch_fold(const BRun& r, bool before_rep) {
+static hipError_t launch_fold(const BRun& r) {
     return launch("batch_fold_kernel", batch_fold_kernel<T>, dim3(ew_blocks(r.n), r.nrb), dim3(NTHREADS), r.stream,
-                  r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dirs<T>(r), before_rep);
+                  r.n, r.c, r.xhi, static_cast<V4<T>*>(r.v[SLOT_XLO]), pending_dirs<T>(r));
 }
 
 template <class T>
-int fold_only(const BRun& r) {
-    SEDI_HIP(launch_fold<T>(r, false));
+int fold_lanes(const BRun& r) {
+    SEDI_HIP(launch_fold<T>(r));
     return SEDI_OK;
 }
 
@@ -837,7 +839,7 @@ int fold_only(const BRun& r) {
 // by the parity of the next iteration.
 template <class T>
 int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold) {
-    if (fold) SEDI_HIP(launch_fold<T>(r, true));
+    if (fold) SEDI_HIP(launch_fold<T>(r));
     hipError_t e;
     if (rhat) e = launch_rep<T, true>(r, res, static_cast<V4<T>*>(r.v[BI_R1]), rhat);
     else e = launch_rep<T, false>(r, res, static_cast<V4<T>*>(r.v[CG_R1]), nullptr);
@@ -1051,7 +1053,7 @@ int BatchSolver::bind_t(int hist_need) {
     run.nrb = nrb_;
     run.a = a_; run.b = b_c_; run.cc = c_; run.im = 1.0 / a_;
     run.stream = stream_;
-    for (int k = 0; k < 12; k++) run.v[k] = v_[k];
+    for (int k = 0; k < BATCH_MAX_SLOTS; k++) run.v[k] = v_[k];
     run.xhi = xhi_;
     run.mixed = fp32_;
     run.nvec = nvec_;
@@ -1387,7 +1389,7 @@ std::vector<double> BatchSolver::history(int k) const {
 
 template int replace_and_fold<float>(const BRun&, V4<float>*, V4<float>*, bool);
 template int replace_and_fold<double>(const BRun&, V4<double>*, V4<double>*, bool);
-template int fold_only<float>(const BRun&);
-template int fold_only<double>(const BRun&);
+template int fold_lanes<float>(const BRun&);
+template int fold_lanes<double>(const BRun&);
 
 }  // namespace sedi

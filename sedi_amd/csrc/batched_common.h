@@ -294,14 +294,18 @@ struct CgDirs {
     __device__ V4<T>* at(int j) const { return j == 0 ? p[0] : j == 1 ? p[1] : j == 2 ? p[2] : p[3]; }
 };
 
-// CG's pending directions, buffers p1 .. p_{CG_NPEND} of the batch (all null:
-// not CG)
+// The pending directions of CG or pipelined CG, buffers p1 .. p_{CG_NPEND} of
+// the batch (all null: BiCGStab, whose x is always current)
 template <class T>
 struct PendDirs {
     const V4<T>* p[CG_NPEND];
 };
-// pipelined CG: r, u, w0, w1, z, q, s, p
-enum { PCG_R = SLOT_K0, PCG_U, PCG_W0, PCG_W1, PCG_Z, PCG_Q, PCG_S, PCG_P, PCG_NVEC };
+// pipelined CG: r, u, w0, w1, z, q, s, and p0 .. p_{CG_XG-1}: its x updates
+// are grouped like CG's (the direction of iteration it is in p[it % CG_XG])
+enum { PCG_R = SLOT_K0, PCG_U, PCG_W0, PCG_W1, PCG_Z, PCG_Q, PCG_S, PCG_P };
+constexpr int PCG_NVEC = PCG_P + CG_XG;
+__host__ __device__ constexpr int pcg_pslot(int j) { return PCG_P + j; }
+static_assert(PCG_NVEC <= BATCH_MAX_SLOTS && CG_NVEC <= BATCH_MAX_SLOTS, "vector slots of a batch");
 // BiCGStab: r0, r1, r^, p0, p1, v0, v1
 enum { BI_R0 = SLOT_K0, BI_R1, BI_RH, BI_P0, BI_P1, BI_V0, BI_V1, BI_NVEC };
 
@@ -313,7 +317,7 @@ struct BRun {
     BCtl c;
     double a, b, cc, im;
     hipStream_t stream;
-    void* v[12];
+    void* v[BATCH_MAX_SLOTS];
     double* xhi;  // FP64 part of x ([K][N]; mixed form only)
     bool mixed;
     int nvec;     // vector slots in use (v[0 .. nvec-1])
@@ -328,9 +332,10 @@ template <class T> int pcg_iteration(const BRun& r);
 template <class T> int bicg_iteration(const BRun& r);
 // the FP64 replacement r = b - A x_hi with <r,u> (CG) or <r,r^> (BiCGStab),
 // preceded, when `fold` (CG), by the fold of x_lo and the pending updates
-// into x_hi (batched.hip); fold_only: pipelined CG's fold after its own
-// replacement
+// into x_hi (batched.hip).  fold_lanes: that fold for the right-hand sides
+// flagged in c.rep (ahead of a replacement, which clears the flags) or in
+// c.fold (pipelined CG's lanes that finished with updates pending; cleared).
 template <class T> int replace_and_fold(const BRun& r, V4<T>* res, V4<T>* rhat, bool fold);
-template <class T> int fold_only(const BRun& r);
+template <class T> int fold_lanes(const BRun& r);
 
 }  // namespace sedi

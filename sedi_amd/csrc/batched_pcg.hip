@@ -86,8 +86,13 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_init_kernel(TileGrid g, BC
 }
 
 // ---- iteration (PipelinedCg.hpp:157-223): m = w.*iM, n = A m, the eight ----
-// recurrences in the reference's order, the next <r,u>, <w,u>, <r,r>
-template <class T>
+// recurrences in the reference's order, the next <r,u>, <w,u>, <r,r>.
+// x is write-only (the last recurrence), so its updates come in groups of
+// CG_XG iterations, as CG's: p is written to p[it % CG_XG], iterations that
+// are not a multiple of CG_XG leave x alone (their alphas wait in c.apend),
+// the others add the pending apend_j p_j and their own alpha p in one
+// read-modify-write.  XS (that last iteration) is a compile-time choice.
+template <class T, bool XS>
 struct PassBPcgIter {
     using C = compute_t<T>;
     using V = V4<C>;
@@ -96,14 +101,24 @@ struct PassBPcgIter {
     C a, b, c;
     const S* wold;
     S* wnew;
-    S *x, *r, *u, *z, *q, *s, *p;
+    S *x, *r, *u, *z, *q, *s;
+    const S* pold;
+    S* pnew;
+    const S* pend[CG_NPEND];
     C im;
-    V alpha, beta;
+    V alpha, beta, apend[CG_NPEND];
     struct Raw { S w; };
-    struct Aux { S q, s, u, p, z, r, x; };
+    struct Aux { S q, s, u, p, z, r, x, pp[CG_NPEND]; };
     __device__ Raw load(cidx_t i) const { return {ldc(wold, i)}; }
     __device__ Aux load_aux(cidx_t i) const {
-        return {ldc(q, i), ldc(s, i), ldc(u, i), ldc(p, i), ldc(z, i), ldc(r, i), ldc(x, i)};
+        Aux o;
+        o.q = ldc(q, i); o.s = ldc(s, i); o.u = ldc(u, i); o.p = ldc(pold, i); o.z = ldc(z, i); o.r = ldc(r, i);
+        if (XS) {
+            o.x = ldc(x, i);
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) o.pp[j] = ldc(pend[j], i);
+        }
+        return o;
     }
     __device__ V f(const Raw& o) const { return im * cvt<C>(o.w); }
     __device__ void finish(cidx_t i, const Raw& c, const Aux& o, const V& m, const V& nv,
@@ -114,11 +129,16 @@ struct PassBPcgIter {
         const V pv = uo + beta * cvt<C>(o.p);
         const V zv = nv + beta * cvt<C>(o.z);
         const S rs = cvt<T>(cvt<C>(o.r) - alpha * sv);
-        const S xs = cvt<T>(cvt<C>(o.x) + alpha * pv);
         const S us = cvt<T>(uo - alpha * qv);
         const S ws = cvt<T>(w - alpha * zv);
-        stv(q, i, cvt<T>(qv)); stv(s, i, cvt<T>(sv)); stv(p, i, cvt<T>(pv)); stv(z, i, cvt<T>(zv));
-        stv(r, i, rs); stv(x, i, xs); stv(u, i, us); stv(wnew, i, ws);
+        stv(q, i, cvt<T>(qv)); stv(s, i, cvt<T>(sv)); stv(pnew, i, cvt<T>(pv)); stv(z, i, cvt<T>(zv));
+        stv(r, i, rs); stv(u, i, us); stv(wnew, i, ws);
+        if (XS) {
+            V xv = cvt<C>(o.x);
+#pragma unroll
+            for (int j = 0; j < CG_NPEND; j++) xv = xv + apend[j] * cvt<C>(o.pp[j]);
+            stv(x, i, cvt<T>(xv + alpha * pv));
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             acc[k] += (double)rs[k] * (double)us[k];
@@ -128,12 +148,41 @@ struct PassBPcgIter {
     }
 };
 
+template <class T, int ZC, bool XS>
+__device__ __forceinline__ void run_pcg_iter(const TileGrid& g, const BCtl& c, compute_t<T> a, compute_t<T> b,
+                                             compute_t<T> cc, long long off, int it, V4<T>* w0, V4<T>* w1,
+                                             V4<T>* x, V4<T>* r, V4<T>* u, V4<T>* z, V4<T>* q, V4<T>* s,
+                                             const CgDirs<T>& P4, compute_t<T> im,
+                                             const V4<compute_t<T>>& alpha, const V4<compute_t<T>>& beta,
+                                             V4<compute_t<T>> (&lds)[2][TY + 2][TX + 2], double (&acc)[12]) {
+    using C = compute_t<T>;
+    const int rb = blockIdx.y;
+    // iteration 1 reads w from w0 (written by the set-up pass) and p from p0
+    PassBPcgIter<T, XS> P;
+    P.a = a; P.b = b; P.c = cc;
+    P.wold = ((it & 1) ? w0 : w1) + off;
+    P.wnew = ((it & 1) ? w1 : w0) + off;
+    P.x = x + off; P.r = r + off; P.u = u + off; P.z = z + off; P.q = q + off; P.s = s + off;
+    P.pold = P4.at((it + CG_XG - 1) % CG_XG) + off;
+    P.pnew = P4.at(it % CG_XG) + off;
+    P.im = im;
+    P.alpha = alpha;
+    P.beta = beta;
+#pragma unroll
+    for (int j = 0; j < CG_NPEND; j++) {
+        P.pend[j] = P4.p[j + 1] + off;
+#pragma unroll
+        for (int k = 0; k < 4; k++) P.apend[j][k] = XS ? uniform((C)c.apend[j * c.K + 4 * rb + k]) : (C)0;
+    }
+    run_tiles<PassBPcgIter<T, XS>, ZC>(P, g, lds, acc, (it & 1) == 0);
+}
+
 template <class T, int ZC>
 __global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BCtl c, compute_t<T> a,
                                                                   compute_t<T> b, compute_t<T> cc, long long n,
                                                                   V4<T>* w0, V4<T>* w1, V4<T>* x, V4<T>* r,
                                                                   V4<T>* u, V4<T>* z, V4<T>* q, V4<T>* s,
-                                                                  V4<T>* p, compute_t<T> im) {
+                                                                  CgDirs<T> P4, compute_t<T> im) {
     using C = compute_t<T>;
     const int rb = blockIdx.y;
     if (c.rb_done[rb]) return;
@@ -141,30 +190,46 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_iter_kernel(TileGrid g, BC
     __shared__ double red[12 * NWAVES];
     __shared__ int lastflag;
     const int it = c.iter[rb];
+    const int jp = it % CG_XG;
     const long long off = (long long)rb * n;
     V4<C> alpha, beta;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const bool d = c.done[4 * rb + k];
-        alpha[k] = d ? (C)0 : (C)c.alpha[4 * rb + k];
-        beta[k] = d ? (C)0 : (C)c.beta[4 * rb + k];
+        alpha[k] = d ? (C)0 : uniform((C)c.alpha[4 * rb + k]);
+        beta[k] = d ? (C)0 : uniform((C)c.beta[4 * rb + k]);
     }
-    // iteration 1 reads w from w0 (written by the set-up pass)
-    PassBPcgIter<T> P{a, b, cc, ((it & 1) ? w0 : w1) + off, ((it & 1) ? w1 : w0) + off, x + off, r + off,
-                      u + off, z + off, q + off, s + off, p + off, im, alpha, beta};
     double acc[12];
-    run_tiles<PassBPcgIter<T>, ZC>(P, g, lds, acc, (it & 1) == 0);
+    if (jp == 0)
+        run_pcg_iter<T, ZC, true>(g, c, a, b, cc, off, it, w0, w1, x, r, u, z, q, s, P4, im, alpha, beta, lds, acc);
+    else
+        run_pcg_iter<T, ZC, false>(g, c, a, b, cc, off, it, w0, w1, x, r, u, z, q, s, P4, im, alpha, beta, lds, acc);
     if (!grid_sum<12>(acc, c.partials + rb * c.pstride, c.counters + rb, red, &lastflag) ||
         threadIdx.x != 0)
         return;
+    // the alpha each right-hand side used in this pass (0 for a finished one)
+    // waits for the group's last iteration, which applies them all
+    for (int q4 = 0; q4 < 4; q4++) {
+        const int k = 4 * rb + q4;
+        const double used = c.done[k] ? 0.0 : c.alpha[k];
+        if (jp == 0)
+            for (int j = 0; j < CG_NPEND; j++) c.apend[j * c.K + k] = 0.0;
+        else
+            c.apend[(jp - 1) * c.K + k] = used;
+    }
     for (int q4 = 0; q4 < 4; q4++) {
         const int k = 4 * rb + q4;
         if (c.done[k]) continue;
         const double err = c.errp[k];  // rr before this iteration's update (:227)
         c.err[k] = err;
         c.hist[(long long)k * c.hstride + it - 1] = err;
-        if (err <= c.tolv[k]) { bset_done(c, k, 0, it, err); continue; }
-        if (it >= c.maxv[k]) { bcg_exhausted(c, k, err); continue; }
+        if (err <= c.tolv[k] || it >= c.maxv[k]) {
+            if (err <= c.tolv[k]) bset_done(c, k, 0, it, err);
+            else bcg_exhausted(c, k, err);
+            // mixed: updates still pending go into x_hi now (batch_fold_kernel)
+            if (c.mixed && jp != 0) c.fold[k] = 1;
+            continue;
+        }
         // scalars of iteration it+1 (PipelinedCg.hpp:167-178)
         const double gamma_old = c.gamma[k];
         const double gamma = acc[q4], delta = acc[4 + q4];
@@ -239,7 +304,7 @@ struct PassBPcgRepA {
 template <class T, int ZC>
 __global__ __launch_bounds__(NTHREADS) void batch_pcg_repa_kernel(TileGrid g, BCtl c, double a, double b,
                                                                   double cc, long long n, const double* xhi,
-                                                                  const V4<T>* xlo, const V4<T>* p,
+                                                                  const V4<T>* xlo, CgDirs<T> P4,
                                                                   const V4<T>* rhs, V4<T>* r, V4<T>* s,
                                                                   V4<T>* u, V4<T>* q, double im) {
     const int rb = blockIdx.y;
@@ -255,7 +320,8 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_repa_kernel(TileGrid g, BC
         P.xh[k] = xhi + (long long)(4 * rb + k) * n;
         P.rep[k] = rp[k];
     }
-    P.xlo = xlo + off; P.p = p + off; P.rhs = rhs + off;
+    // the direction of the iteration that asked for the replacement
+    P.xlo = xlo + off; P.p = P4.at((c.iter[rb] - 1) % CG_XG) + off; P.rhs = rhs + off;
     P.r = r + off; P.s = s + off; P.u = u + off; P.q = q + off;
     P.im = im;
     double acc[1];
@@ -326,7 +392,6 @@ __global__ __launch_bounds__(NTHREADS) void batch_pcg_repb_kernel(TileGrid g, BC
         const int k = 4 * rb + q4;
         if (!c.rep[k]) continue;
         c.rep[k] = 0;
-        c.fold[k] = 1;
         const double gamma = acc[q4], delta = acc[4 + q4];
         const double e = sqrt(acc[8 + q4] / c.bb[k]);
         if (e > c.restart * c.errp[k]) {  // the recursion drifted: restart
@@ -363,20 +428,25 @@ int pcg_iteration(const BRun& r) {
     const C a = (C)r.a, b = (C)r.b, cc = (C)r.cc, im = (C)r.im;
     auto V = [&](int s) { return static_cast<V4<T>*>(r.v[s]); };
     const KGeom g = bgeom("batch_pcg_iter_kernel", batch_pcg_iter_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
+    CgDirs<T> P4;
+    for (int j = 0; j < 4; j++) P4.p[j] = j < CG_XG ? V(pcg_pslot(j)) : nullptr;
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_iter_kernel", batch_pcg_iter_kernel, T, g, dim3(g.nblocks, r.nrb),
                              r.stream, g.tg, r.c, a, b, cc, r.n, V(PCG_W0), V(PCG_W1), V(SLOT_XLO), V(PCG_R),
-                             V(PCG_U), V(PCG_Z), V(PCG_Q), V(PCG_S), V(PCG_P), im));
+                             V(PCG_U), V(PCG_Z), V(PCG_Q), V(PCG_S), P4, im));
     if (!r.mixed) return SEDI_OK;
+    // x_lo and the pending updates into x_hi for the right-hand sides about to
+    // be replaced and those that just finished, then the two replacement sweeps
+    if (int rc = fold_lanes<T>(r)) return rc;
     const KGeom ga = bgeom("batch_pcg_repa_kernel", batch_pcg_repa_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_repa_kernel", batch_pcg_repa_kernel, T, ga, dim3(ga.nblocks, r.nrb),
                              r.stream, ga.tg, r.c, r.a, r.b, r.cc, r.n, (const double*)r.xhi,
-                             (const V4<T>*)V(SLOT_XLO), (const V4<T>*)V(PCG_P), (const V4<T>*)V(SLOT_B),
+                             (const V4<T>*)V(SLOT_XLO), P4, (const V4<T>*)V(SLOT_B),
                              V(PCG_R), V(PCG_S), V(PCG_U), V(PCG_Q), r.im));
     const KGeom gb = bgeom("batch_pcg_repb_kernel", batch_pcg_repb_kernel<T, 16>, r.shape, SEDI_PREFETCH_BATCH);
     SEDI_HIP(SEDI_BZC_LAUNCH("batch_pcg_repb_kernel", batch_pcg_repb_kernel, T, gb, dim3(gb.nblocks, r.nrb),
                              r.stream, gb.tg, r.c, r.a, r.b, r.cc, r.n, (const V4<T>*)V(PCG_U),
                              (const V4<T>*)V(PCG_Q), (const V4<T>*)V(PCG_R), V(PCG_W0), V(PCG_W1), V(PCG_Z)));
-    return fold_only<T>(r);
+    return SEDI_OK;
 }
 
 template int pcg_setup<float>(const BRun&, int, int);
