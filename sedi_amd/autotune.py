@@ -62,9 +62,11 @@ PLAIN_KERNELS = {2: ["cg_direction_kernel", "cg_update_kernel"], 3: ["pcg_iter_k
 BATCH_KERNELS = {2: ["batch_dir_kernel", "batch_upd_kernel"], 3: ["batch_pcg_iter_kernel"],
                  1: ["batch_bicg_dir_kernel", "batch_bicg_half_kernel", "batch_bicg_upd_kernel"]}
 # device bytes per right-hand side and cell of a batch (FP32 storage + FP64
-# solution for the mixed form), for sizing k to the memory
-BYTES_MIXED = {2: 32, 3: 48, 1: 44}
-BYTES_FP64 = {2: 48, 3: 80, 1: 72}
+# solution for the mixed form), for sizing k to the memory: CG b, x_lo, r x2,
+# p x4; pipelined CG b, x_lo, r, u, w x2, z, q, s, p x4; BiCGStab b, x_lo, r x2,
+# r^, p x2, v x2
+BYTES_MIXED = {2: 40, 3: 60, 1: 44}
+BYTES_FP64 = {2: 64, 3: 104, 1: 72}
 K_CHOICES = (8, 16, 32, 64)
 PLAIN_ITERS = 600
 
