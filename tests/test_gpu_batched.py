@@ -144,18 +144,20 @@ def test_update_pending_when_finishing_at_iteration_one(prec):
     assert np.array_equal(x0[0], x[0]) and it0[0] == iters[0]
 
 
+@pytest.mark.parametrize("kind", [O.CG, O.PCG])
 @pytest.mark.parametrize("prec", [sedi.FP64, sedi.FP32_MIXED])
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
-def test_updates_pending_at_every_place_in_a_group(prec, m):
-    """CG applies solution updates in groups of four iterations, so a
-    right-hand side that finishes inside a group has one to three pending.  A
-    sum of m eigenvectors with distinct eigenvalues is solved by m CG
-    iterations in exact arithmetic: in FP64 it finishes at iteration m = 1..5
-    (inside a group, or at its end) while two random probes of its block go
-    on, and alone in its block.  Its solution matches the dense solve (FP64:
-    1e-12 normwise; mixed, which solves with b rounded to FP32 and takes more
-    iterations: 1e-8 normwise against the dense solve of that rounded b), and
-    every right-hand side's result is bitwise the same in the mix and alone."""
+def test_updates_pending_at_every_place_in_a_group(kind, prec, m):
+    """CG and pipelined CG apply solution updates in groups of four
+    iterations, so a right-hand side that finishes inside a group has one to
+    three pending.  A sum of m eigenvectors with distinct eigenvalues is
+    solved by m CG iterations in exact arithmetic (pipelined CG stops one
+    later): in FP64 it finishes at every place in a group while two random
+    probes of its block go on, and alone in its block.  Its solution matches
+    the dense solve (FP64: 1e-12 normwise; mixed, which solves with b rounded
+    to FP32 and takes more iterations: 1e-8 normwise against the dense solve
+    of that rounded b), and every right-hand side's result is bitwise the
+    same in the mix and alone."""
     shape = (8, 8, 8)
     n = 512
     i = np.arange(1, 9)
@@ -165,11 +167,11 @@ def test_updates_pending_at_every_place_in_a_group(prec, m):
         rhs += (s[:, None, None] * s[None, :, None] * s[None, None, :]).reshape(-1)
     A = np.stack([O.stencil_apply(shape, e) for e in np.eye(n)], axis=1)
     b = np.vstack([rhs, probes(9, shape, 0, 2)])
-    bs = sedi.Batch(shape, 4, prec)
+    bs = sedi.Batch(shape, 4, prec, kind=kind)
     x, iters, errs, codes = bs.solve(b, maxiter=200, tol=1e-10)
-    assert np.all(codes == 0) and np.all(iters[1:] > 5)
+    assert np.all(codes == 0) and np.all(iters[1:] > 6)
     if prec == sedi.FP64:
-        assert iters[0] == m
+        assert iters[0] == m + (kind == O.PCG)
         assert normwise(x[0], np.linalg.solve(A, rhs)) < 1e-12
     else:
         assert normwise(x[0], np.linalg.solve(A, rhs.astype(np.float32).astype(np.float64))) < 1e-8

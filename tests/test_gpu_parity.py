@@ -205,14 +205,16 @@ def test_eigenvector_rhs_one_iteration(kind):
     assert np.allclose(x, phi / lam, atol=1e-14)
 
 
+@pytest.mark.parametrize("kind", [O.CG, O.PCG])
 @pytest.mark.parametrize("m", [1, 2, 3, 4, 5, 6, 7, 8])
-def test_cg_finishes_at_every_place_in_a_group(m):
-    """Plain CG applies its solution updates in groups of four iterations; a
-    solve that ends inside a group gets its one to three pending updates from
-    cg_flush_kernel.  A sum of m eigenvectors with distinct eigenvalues is
-    solved by exactly m CG iterations: the solve ends at iteration m (every
-    place in a group, twice) and x matches the dense solve to 1e-12
-    normwise, and the oracle's x (one update per iteration) to rounding."""
+def test_finishes_at_every_place_in_a_group(kind, m):
+    """Plain CG and pipelined CG apply their solution updates in groups of
+    four iterations; a solve that ends inside a group gets its one to three
+    pending updates from cg_flush_kernel.  A sum of m eigenvectors with
+    distinct eigenvalues is solved by exactly m CG iterations (pipelined CG
+    stops one later, its test lagging by one): the solve ends at every place
+    in a group, twice, and x matches the dense solve to 1e-12 normwise, and
+    the oracle's x (one update per iteration) to rounding."""
     shape = (8, 8, 8)
     i = np.arange(1, 9)
     b = np.zeros(512)
@@ -220,11 +222,12 @@ def test_cg_finishes_at_every_place_in_a_group(m):
         s = np.sin(np.pi * a * i / 9)
         b += (s[:, None, None] * s[None, :, None] * s[None, None, :]).reshape(-1)
     A = np.stack([O.stencil_apply(shape, e) for e in np.eye(512)], axis=1)
-    code, x, fe, fi, hist = gpu_solve(O.CG, shape, b, maxiter=50, tol=1e-10)
-    assert (code, fi) == (0, m)
+    last = m + (kind == O.PCG)
+    code, x, fe, fi, hist = gpu_solve(kind, shape, b, maxiter=50, tol=1e-10)
+    assert (code, fi) == (0, last)
     assert normwise(x, np.linalg.solve(A, b)) < 1e-12
-    ocode, ox, ofe, ofi, oh = O.solve(O.CG, shape, b, maxiter=50, tol=1e-10)
-    assert (ocode, ofi) == (0, m) and normwise(x, ox) < 1e-13
+    ocode, ox, ofe, ofi, oh = O.solve(kind, shape, b, maxiter=50, tol=1e-10)
+    assert (ocode, ofi) == (0, last) and normwise(x, ox) < 1e-13
 
 
 def test_estimator_error_exit_code():
