@@ -104,7 +104,7 @@ int sedi_solve(sedi_solver *solver, const double *b, double *x,
  * it runs on until the process ends.  Same kernels, launch plan, result
  * codes and final_err/final_iter as sedi_solver_create + sedi_solve on the
  * same grid (so the same x, bitwise); no err history is kept.  A solve
- * needing more (vectors: 3 for CG, 8 for pipelined CG, 7 for BiCGStab, each
+ * needing more (vectors: 5 for CG, 11 for pipelined CG, 7 for BiCGStab, each
  * 8*n1*n2*n3 bytes, plus 8*maxiter bytes and ~50 KB) returns SEDI_ERR_ARG.
  * Launches on the null stream and returns when the solve is done.  Calls on
  * one device share the array: calls that overlap on one device race. */
