@@ -45,8 +45,10 @@ extern "C" {
 #define SEDI_PCG      3
 
 /* Arithmetic of a solve.  FP64: the reference's (real_t = double,
- * stochastic-estimator.cc:29).  FP32_MIXED: FP32 vectors and stencil, FP64
- * dots, scalars, residual replacement and diagonal accumulator. */
+ * stochastic-estimator.cc:29).  FP32_MIXED: FP32 vector storage (CG computes
+ * its stencil and updates in FP32, pipelined CG and BiCGStab in FP64
+ * registers), FP64 dots, scalars, residual replacement and diagonal
+ * accumulator. */
 #define SEDI_FP64        0
 #define SEDI_FP32_MIXED  1
 
