@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 // Prefetch distance (planes) of the march in each pass.
 #ifndef SEDI_PREFETCH_APPLY
@@ -32,6 +33,12 @@
 #endif
 #ifndef SEDI_PREFETCH_PCG
 #define SEDI_PREFETCH_PCG 1
+#endif
+// the pipelined-CG pass with the scalar Jacobi preconditioner (the
+// estimator's): two planes (at 256^3 372 -> 364 us a launch); with a vector
+// preconditioner two would push the pass past 128 VGPRs into scratch
+#ifndef SEDI_PREFETCH_PCG_SCALAR
+#define SEDI_PREFETCH_PCG_SCALAR 2
 #endif
 #ifndef SEDI_PREFETCH_BICG
 #define SEDI_PREFETCH_BICG 2
@@ -276,7 +283,8 @@ struct PassPcgInit {
 // eight recurrences in the reference's order, and the next <r,u>, <w,u>, <r,r>.
 template <class PC, bool XS>
 struct PassPcgIter {
-    static constexpr int ND = 3, D = SEDI_PREFETCH_PCG;
+    static constexpr int ND = 3;
+    static constexpr int D = std::is_same<PC, PrecondScalar>::value ? SEDI_PREFETCH_PCG_SCALAR : SEDI_PREFETCH_PCG;
     double a, b, c;
     const double* wold;
     double* wnew;
